@@ -1,0 +1,256 @@
+"""Host-side mirror of the reference's interface on numpy arrays.
+
+Same names, argument meaning and return conventions as the reference's C
+helpers and solve wrappers (an int return code where the reference returns
+one, outputs in place of its pointer arguments), backed by
+libcapellini_sptrsv.so:
+
+  read_mtx                  CUDA/ourWrtFst/read_mtx.h:6-191
+  change2tran               CUDA/ourWrtFst/read_mtx.h:194-257
+  get_x_b                   CUDA/ourWrtFst/read_mtx.h:260-278
+  matrix_layer              CUDA/ourWrtFst/tranpose.h:78-142
+  matrix_transposition      CUDA/ourWrtFst/tranpose.h:36-75
+  matrix_warp4              CUDA/mix/tranpose.h:1423-1477
+  spts_syncfree_hip_csr_wrt CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:66-242
+  spts_syncfree_hip_csr     CUDA/our2Part/spts_syncfree_cuda_csr.h:133-299
+  spts_syncfree_hip_mix     CUDA/mix/spts_syncfree_cuda_csr_wrt.h:146-352
+  sptrsv_syncfree_hip_csc   CUDA/SyncFree_csc/spts_syncfree_cuda2.h:89-292
+
+Precision follows the dtype of the value array (float64 -> the _d entry,
+float32 -> the _s entry), the runtime counterpart of the VALUE_TYPE switch.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_int, POINTER
+
+import numpy as np
+
+from . import _native as N
+
+__all__ = [
+    "read_mtx", "change2tran", "get_x_b", "matrix_layer", "matrix_transposition", "matrix_warp4",
+    "spts_syncfree_hip_csr_wrt", "spts_syncfree_hip_csr", "spts_syncfree_hip_mix", "sptrsv_syncfree_hip_csc",
+    "sptrsv_csr_solve", "validate_csr", "generate", "uniform", "datasize", "make_config", "CONFIGS",
+]
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pi(a):
+    return a.ctypes.data_as(POINTER(c_int))
+
+
+def _pv(a):
+    return a.ctypes.data_as(POINTER(c_double) if a.dtype == np.float64 else POINTER(ctypes.c_float))
+
+
+def _suffix(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "d"
+    if dt == np.float32:
+        return "s"
+    raise TypeError(f"values must be float64 or float32, got {dt}")
+
+
+def _take(ptr, count, dtype):
+    """Copy a malloc'ed C array into numpy and free it."""
+    L = N.lib()
+    if count == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        out = np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+    L.cap_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return out
+
+
+# --------------------------------------------------------------- host helpers
+def read_mtx(filename: str, dtype=np.float64):
+    """Returns (ret, m, n, nnzA, row_ptr, col_idx, val); arrays are None when ret != 0."""
+    L = N.lib()
+    sfx = _suffix(dtype)
+    m, n, nnz = c_int(), c_int(), c_int()
+    rp, ci = POINTER(c_int)(), POINTER(c_int)()
+    v = POINTER(c_double)() if sfx == "d" else POINTER(ctypes.c_float)()
+    ret = getattr(L, f"cap_read_mtx_{sfx}")(str(filename).encode(), byref(m), byref(n), byref(nnz),
+                                             byref(rp), byref(ci), byref(v))
+    if ret != 0:
+        return ret, 0, 0, 0, None, None, None
+    return (ret, m.value, n.value, nnz.value, _take(rp, m.value + 1, np.int32), _take(ci, nnz.value, np.int32),
+            _take(v, nnz.value, np.dtype(dtype)))
+
+
+def change2tran(m: int, nnzA: int, row_ptr, col_idx, val, keep_values: bool = False):
+    """Returns (nnzL, row_ptr_L, col_idx_L, val_L)."""
+    L = N.lib()
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    nnzL = c_int()
+    orp, oci = POINTER(c_int)(), POINTER(c_int)()
+    ov = POINTER(c_double)() if sfx == "d" else POINTER(ctypes.c_float)()
+    N.check(getattr(L, f"cap_change2tran_{sfx}")(m, nnzA, _pi(rp), _pi(ci), _pv(val), int(keep_values),
+                                                 byref(nnzL), byref(orp), byref(oci), byref(ov)), "change2tran")
+    k = nnzL.value
+    return k, _take(orp, m + 1, np.int32), _take(oci, k, np.int32), _take(ov, k, val.dtype)
+
+
+def get_x_b(m: int, n: int, row_ptr, col_idx, val):
+    """Returns (x_ref, b): x_ref = 1, b = A x_ref."""
+    L = N.lib()
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    x = np.empty(n, dtype=val.dtype)
+    b = np.empty(m, dtype=val.dtype)
+    getattr(L, f"cap_get_x_b_{sfx}")(m, n, _pi(rp), _pi(ci), _pv(val), _pv(x), _pv(b))
+    return x, b
+
+
+def matrix_layer(m: int, n: int, nnz: int, row_ptr, col_idx, val=None, return_levels: bool = False):
+    """Returns (layer, parallelism) [, per-row level]."""
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    lv = np.zeros(max(m, 1), dtype=np.int32)
+    layer, par = c_int(), c_double()
+    N.lib().cap_matrix_layer(m, _pi(rp), _pi(ci), byref(layer), byref(par), _pi(lv))
+    if return_levels:
+        return layer.value, par.value, lv[:m]
+    return layer.value, par.value
+
+
+def matrix_transposition(m: int, n: int, nnz: int, row_ptr, col_idx, val):
+    """Returns (csc_row_idx, csc_col_ptr, csc_val)."""
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    cr = np.empty(max(nnz, 1), np.int32)
+    cp = np.empty(n + 1, np.int32)
+    cv = np.empty(max(nnz, 1), val.dtype)
+    getattr(N.lib(), f"cap_transpose_{sfx}")(m, n, nnz, _pi(rp), _pi(ci), _pv(val), _pi(cr), _pi(cp), _pv(cv))
+    return cr[:nnz], cp, cv[:nnz]
+
+
+def matrix_warp4(m: int, n: int, nnz: int, row_ptr, col_idx=None, val=None, border: int = 16, chunk: int = 64):
+    """Returns (Len, warp_num[:Len], warp_occupy, element_occupy)."""
+    rp = _i32(row_ptr)
+    wn = np.zeros(m + 1, np.int32)
+    ln, wo, eo = c_int(), c_double(), c_double()
+    N.lib().cap_matrix_warp4(m, nnz, _pi(rp), int(border), int(chunk), byref(ln), _pi(wn), byref(wo), byref(eo))
+    return ln.value, wn[:ln.value].copy(), wo.value, eo.value
+
+
+def validate_csr(n: int, nnz: int, row_ptr, col_idx, val):
+    """Returns (code, bad_row) of the solve-layout check."""
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    bad = c_int(-1)
+    code = getattr(N.lib(), f"cap_validate_csr_{sfx}")(n, nnz, _pi(rp), _pi(ci), _pv(val), byref(bad))
+    return code, bad.value
+
+
+def datasize(m: int, n: int, nnzL: int, value_bytes: int, len_warp_num: int = 0) -> float:
+    return N.lib().cap_datasize(m, n, nnzL, value_bytes, len_warp_num)
+
+
+# ---------------------------------------------------- benchmark-compatible solves
+def spts_syncfree_hip_csr_wrt(row_ptr, col_idx, val, m: int, n: int, nnzL: int):
+    """Returns (ret, solve_time_ms, gflops, bandwith)."""
+    return _bench("spts_syncfree_hip_csr_wrt", row_ptr, col_idx, val, m, n, nnzL, mix=False)
+
+
+def spts_syncfree_hip_csr(row_ptr, col_idx, val, m: int, n: int, nnzL: int):
+    """our2Part.  Returns (ret, solve_time_ms, gflops, bandwith)."""
+    return _bench("spts_syncfree_hip_csr", row_ptr, col_idx, val, m, n, nnzL, mix=False)
+
+
+def spts_syncfree_hip_mix(row_ptr, col_idx, val, m: int, n: int, nnzL: int):
+    """Returns (ret, solve_time_ms, gflops, bandwith, pre_time_ms, warp_occupy, element_occupy)."""
+    return _bench("spts_syncfree_hip_mix", row_ptr, col_idx, val, m, n, nnzL, mix=True)
+
+
+def _bench(name, row_ptr, col_idx, val, m, n, nnzL, mix):
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    out = [c_double() for _ in range(6 if mix else 3)]
+    ret = getattr(N.lib(), f"{name}_{sfx}")(_pi(rp), _pi(ci), _pv(val), m, n, nnzL, *[byref(o) for o in out])
+    return (ret, *[o.value for o in out])
+
+
+def sptrsv_syncfree_hip_csc(csc_col_ptr, csc_row_idx, csc_val, m: int, n: int, nnz: int, b, x_ref):
+    """Returns (ret, x, pre_time_ms, solve_time_ms, gflops, bandwidth)."""
+    cv = np.ascontiguousarray(csc_val)
+    sfx = _suffix(cv.dtype)
+    cp, ri = _i32(csc_col_ptr), _i32(csc_row_idx)
+    b = np.ascontiguousarray(b, dtype=cv.dtype)
+    xr = np.ascontiguousarray(x_ref, dtype=cv.dtype)
+    x = np.zeros(n, dtype=cv.dtype)
+    out = [c_double() for _ in range(4)]
+    ret = getattr(N.lib(), f"sptrsv_syncfree_hip_csc_{sfx}")(_pi(cp), _pi(ri), _pv(cv), m, n, nnz, _pv(x), _pv(b),
+                                                              _pv(xr), *[byref(o) for o in out])
+    return (ret, x, *[o.value for o in out])
+
+
+def sptrsv_csr_solve(row_ptr, col_idx, val, b, algo: str = "wrtfst", sync: str = "value", border: int = 0,
+                     spin_limit: int = 0):
+    """One-shot host solve L x = b on the GPU; raises CapError on failure."""
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    rp, ci = _i32(row_ptr), _i32(col_idx)
+    n = len(rp) - 1
+    b = np.ascontiguousarray(b, dtype=val.dtype)
+    x = np.zeros(n, dtype=val.dtype)
+    o = N.default_opts(algo=algo, sync=sync, border=border, spin_limit=spin_limit)
+    N.check(getattr(N.lib(), f"sptrsv_csr_solve_{sfx}")(n, int(rp[-1]), _pi(rp), _pi(ci), _pv(val), _pv(b), _pv(x),
+                                                         byref(o)), "sptrsv_csr_solve")
+    return x
+
+
+# ----------------------------------------------------------------- generators
+_KIND = {"band": 0, "level": 1, "bernoulli": 2, "band_rowcount": 3}
+
+
+def generate(kind: str, n: int = 0, offdiag: int = 0, band: int = 0, levels: int = 0, width: int = 0,
+             depth: int = 0, p: float = 0.0, seed: int = 0, dtype=np.float64):
+    """Seeded synthetic lower-triangular CSR (see cap_generate_d).  Returns (row_ptr, col_idx, val)."""
+    g = N.cap_gen_params(_KIND[kind], n, offdiag, band, levels, width, depth, p, seed)
+    sfx = _suffix(dtype)
+    L = N.lib()
+    nnz = c_int()
+    rp = np.zeros(n + 1, np.int32)
+    N.check(getattr(L, f"cap_generate_{sfx}")(byref(g), byref(nnz), _pi(rp), None, None), "generate")
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    v = np.zeros(max(nnz.value, 1), np.dtype(dtype))
+    N.check(getattr(L, f"cap_generate_{sfx}")(byref(g), byref(nnz), _pi(rp), _pi(ci), _pv(v)), "generate")
+    return rp, ci[:nnz.value], v[:nnz.value]
+
+
+def uniform(seed: int, count: int, lo: float = -1.0, hi: float = 1.0):
+    out = np.empty(count, np.float64)
+    N.lib().cap_uniform_d(seed, count, lo, hi, out.ctypes.data_as(POINTER(c_double)))
+    return out
+
+
+# The benchmark workloads (sizes as specified for the reference comparison).
+CONFIGS = {
+    0: dict(kind="band", n=100_000, offdiag=3, band=1000),
+    1: dict(kind="level", n=2_000_000, levels=200, width=10_000, depth=3, offdiag=2),
+    2: dict(kind="bernoulli", n=500_000, band=128, p=0.3),
+    4: dict(kind="level", n=1_000_000, levels=100, width=10_000, depth=3, offdiag=2),
+}
+
+
+def make_config(cfg: int, seed: int = 0, dtype=np.float64, scale: float = 1.0):
+    """Matrix of workload `cfg` (0, 1, 2, 4), optionally shrunk by `scale` for tests."""
+    c = dict(CONFIGS[cfg])
+    if scale != 1.0:
+        if c["kind"] == "level":
+            c["levels"] = max(2, int(c["levels"] * scale))
+            c["n"] = c["levels"] * c["width"]
+        else:
+            c["n"] = max(2, int(c["n"] * scale))
+    return generate(seed=seed, dtype=dtype, **c)

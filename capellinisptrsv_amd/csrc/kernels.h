@@ -1,0 +1,482 @@
+// Device kernels of the gfx950 sync-free lower-triangular solve.
+//
+// Execution model shared by every solve kernel:
+//  * Work is handed out by an atomic ticket, one atomicAdd per 64-lane
+//    wavefront (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:24 takes one per
+//    thread).  Tickets are issued in dispatch order, so every row a wave
+//    depends on belongs to a wave that is already resident or finished; that
+//    is what makes spinning safe without a grid barrier.
+//  * Readiness of x[c] is observed either through the value itself (SYNC
+//    VALUE: x starts as a signalling-NaN sentinel that arithmetic can never
+//    produce; a relaxed agent-scope 32/64-bit atomic load both tests and
+//    fetches it) or through an int flag per row holding the solve's epoch
+//    (SYNC FLAG: release store / acquire fence, agent scope; the epoch makes
+//    resets between solves unnecessary).
+//  * Thread-per-row kernels use the write-first loop of
+//    spts_syncfree_cuda_csr_wrt.h:37-55: a lane publishes x[i] from inside the
+//    loop in which it polls, so lanes of one wavefront that depend on each
+//    other always make progress.  Each lane polls a window of up to WIN
+//    dependencies at once and consumes the ready prefix in stored order, so
+//    the accumulation order is the sequential one (bit-exact against the
+//    oracle's fused forward substitution).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cap {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // 4 wavefronts per workgroup
+constexpr int kWin = 4;      // dependencies polled concurrently per lane
+
+template <typename T>
+struct VBits;
+template <>
+struct VBits<double> {
+    typedef unsigned long long U;
+    static constexpr U kSentinel = 0xFFF0000000000001ULL;  // negative signalling NaN
+    static constexpr U kQNaN = 0x7FF8000000000000ULL;
+};
+template <>
+struct VBits<float> {
+    typedef unsigned int U;
+    static constexpr U kSentinel = 0xFF800001u;
+    static constexpr U kQNaN = 0x7FC00000u;
+};
+
+template <typename U>
+__device__ __forceinline__ U ld_agent(const U *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename U>
+__device__ __forceinline__ void st_agent(U *p, U v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double as_val(unsigned long long u) { return __longlong_as_double((long long)u); }
+__device__ __forceinline__ float as_val(unsigned int u) { return __uint_as_float(u); }
+__device__ __forceinline__ unsigned long long as_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ __forceinline__ unsigned int as_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ double fmac(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// hardware FP / int atomic add at agent scope (global_atomic_add_f64 / _f32)
+template <typename V>
+__device__ __forceinline__ void atomic_add_agent(V *p, V v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Readiness protocol state shared by the solve kernels.
+//  flags/epoch: SYNC FLAG only.  spin_limit: polls without progress after
+//  which a lane gives up, publishes NaN (so every dependent row, and with it
+//  the whole grid, still finishes) and sets bit 0 of *err; <= 0 = unbounded.
+struct SyncCtx {
+    int *flags;
+    int epoch;
+    int spin_limit;
+    int *err;
+};
+
+__device__ __forceinline__ bool give_up(const SyncCtx &sc, int idle)
+{
+    if (sc.spin_limit > 0 && idle > sc.spin_limit) {
+        atomicOr(sc.err, 1);
+        return true;
+    }
+    return false;
+}
+
+// One ticket per wavefront, broadcast to all lanes.
+__device__ __forceinline__ int wave_ticket(unsigned *ticket, unsigned amount)
+{
+    unsigned t = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(ticket, amount);
+    return (int)__shfl(t, 0);
+}
+
+// Publish x[i] in the value protocol: the sentinel can never be published
+// (division results are never signalling NaNs; the guard keeps that true for
+// any input bit pattern).
+template <typename T>
+__device__ __forceinline__ void publish_value(T *x, long long idx, T xi)
+{
+    typedef typename VBits<T>::U U;
+    U o = as_bits(xi);
+    if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;
+    st_agent(reinterpret_cast<U *>(x) + idx, o);
+}
+
+// ------------------------------------------------------------------ fills
+template <typename U>
+__global__ void __launch_bounds__(kBlock) k_fill(U *__restrict__ p, long long count, U value)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = value;
+}
+
+// Per-solve reset: tickets (and the value protocol's sentinel fill of x) in
+// one launch, so no memset node is needed when a caller captures solves in a
+// graph.
+template <typename U>
+__global__ void __launch_bounds__(kBlock)
+    k_prepare(U *__restrict__ p, long long count, U value, unsigned *__restrict__ ticket, int n_ticket)
+{
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_ticket) ticket[threadIdx.x] = 0u;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = value;
+}
+
+// ------------------------------------------------- write-first row solver
+// Publishes x[o] (o = row*k + q) in the protocol SYNC (1 value, 2 flag).
+template <typename T, int SYNC>
+__device__ __forceinline__ void publish(T *x, const SyncCtx &sc, long long o, T xi)
+{
+    if (SYNC == 1) {
+        publish_value(x, o, xi);
+    } else {
+        x[o] = xi;
+        __hip_atomic_store(sc.flags + o, sc.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Blocking wait for x[o] (used where every dependency belongs to an earlier
+// wavefront).  Returns the value; NaN if the watchdog fired.
+template <typename T, int SYNC>
+__device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long o)
+{
+    typedef typename VBits<T>::U U;
+    const U *xu = reinterpret_cast<const U *>(x);
+    int idle = 0;
+    if (SYNC == 1) {
+        U raw;
+        while ((raw = ld_agent(xu + o)) == VBits<T>::kSentinel)
+            if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
+        return as_val(raw);
+    } else {
+        while (ld_agent(sc.flags + o) != sc.epoch)
+            if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return as_val(xu[o]);
+    }
+}
+
+// Solves row i (off-diagonal entries [j, d), diagonal at d) for right-hand
+// side q of k (x and b interleaved: x[row*k + q]) and publishes it.  `s`
+// carries a partial sum when the caller already consumed a prefix (two-part).
+template <typename T, int SYNC>
+__device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *__restrict__ val,
+                                           const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
+                                           int i, int j, int d, int k, int q, T s)
+{
+    typedef typename VBits<T>::U U;
+    U *xu = reinterpret_cast<U *>(x);
+    int cw[kWin];
+    T vw[kWin];
+    U xw[kWin];
+    bool rdy[kWin];
+    int cnt = min(kWin, d - j), used = 0, idle = 0;
+#pragma unroll
+    for (int t = 0; t < kWin; ++t) {
+        rdy[t] = false;
+        xw[t] = 0;
+        cw[t] = t < cnt ? ci[j + t] : 0;
+        vw[t] = t < cnt ? val[j + t] : (T)0;
+    }
+    // The loop exits only when every lane of the wavefront has published
+    // (wave-uniform exit).  With a per-lane exit the compiler may legally
+    // move the publishing store to the loop exit, where a finished lane waits
+    // for the rest of its wavefront -- which may be waiting for that store.
+    bool done = false;
+    do {
+        if (!done) {
+            if (used < cnt) {
+                if (SYNC == 1) {
+#pragma unroll
+                    for (int t = 0; t < kWin; ++t)
+                        if (t >= used && t < cnt && !rdy[t]) xw[t] = ld_agent(xu + (long long)cw[t] * k + q);
+#pragma unroll
+                    for (int t = 0; t < kWin; ++t)
+                        if (t >= used && t < cnt && !rdy[t]) rdy[t] = xw[t] != VBits<T>::kSentinel;
+                } else {
+                    int fl[kWin];
+                    bool any = false;
+#pragma unroll
+                    for (int t = 0; t < kWin; ++t)
+                        fl[t] = (t >= used && t < cnt && !rdy[t]) ? ld_agent(sc.flags + (long long)cw[t] * k + q)
+                                                                 : 0;
+#pragma unroll
+                    for (int t = 0; t < kWin; ++t)
+                        if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) any = true;
+                    if (any) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+                        for (int t = 0; t < kWin; ++t)
+                            if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) {
+                                xw[t] = xu[(long long)cw[t] * k + q];
+                                rdy[t] = true;
+                            }
+                    }
+                }
+                // consume the ready prefix in stored order
+                const int before = used;
+#pragma unroll
+                for (int t = 0; t < kWin; ++t)
+                    if (t == used && t < cnt && rdy[t]) {
+                        s = fmac(vw[t], as_val(xw[t]), s);
+                        ++used;
+                    }
+                if (used != before) {
+                    idle = 0;
+                } else if (give_up(sc, ++idle)) {
+                    s = as_val(VBits<T>::kQNaN);  // poison this row; it still publishes below
+                    used = cnt;
+                    j = d;
+                }
+                if (used == cnt && j < d) {
+                    j += cnt;
+                    if (j < d) {  // next window
+                        cnt = min(kWin, d - j);
+                        used = 0;
+#pragma unroll
+                        for (int t = 0; t < kWin; ++t) {
+                            rdy[t] = false;
+                            if (t < cnt) {
+                                cw[t] = ci[j + t];
+                                vw[t] = val[j + t];
+                            }
+                        }
+                    }
+                }
+            }
+            if (used == cnt && j >= d) {  // write first: publish inside the polling loop
+                const long long o = (long long)i * k + q;
+                publish<T, SYNC>(x, sc, o, (b[o] - s) / val[d]);
+                asm volatile("" ::: "memory");
+                done = true;
+            }
+        }
+    } while (__any(!done));
+}
+
+// ourWrtFst (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:10-57): one lane
+// per row and right-hand side; a wavefront holds 64 >> lpr_log2 rows and
+// 1 << lpr_log2 lanes per row, lane q of a row owning right-hand side q
+// (lanes with q >= k idle).
+template <typename T, int SYNC>
+__global__ void __launch_bounds__(kBlock)
+    k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
+             const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
+             unsigned *__restrict__ ticket)
+{
+    const int base = wave_ticket(ticket, (unsigned)(kWave >> lpr_log2));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = base + (lane >> lpr_log2);
+    const int q = lane & ((1 << lpr_log2) - 1);
+    if (i >= n || q >= k) return;
+    wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0);
+}
+
+// our2Part (CUDA/our2Part/spts_syncfree_cuda_csr.h:10-124): dependencies on
+// rows of earlier wavefronts (col < first row of this wave) are waited for
+// one by one; the rest is the write-first loop.  Single right-hand side.
+template <typename T, int SYNC>
+__global__ void __launch_bounds__(kBlock)
+    k_twopart(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
+              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, unsigned *__restrict__ ticket)
+{
+    const int base = wave_ticket(ticket, kWave);
+    const int i = base + (threadIdx.x & (kWave - 1));
+    if (i >= n) return;
+    int j = rp[i];
+    const int d = rp[i + 1] - 1;
+    T s = (T)0;
+    for (; j < d; ++j) {
+        const int c = ci[j];
+        if (c >= base) break;
+        s = fmac(val[j], wait_value<T, SYNC>(x, sc, c), s);
+    }
+    wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, j, d, 1, 0, s);
+}
+
+// mix (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:15-137): unit u covers rows
+// [unit[u], unit[u+1]).  A unit of one row is solved by the whole wavefront
+// (lane-strided products, blocking waits -- every dependency is an earlier
+// unit -- and a butterfly reduction); a longer unit is solved thread-per-row
+// with the write-first loop.  Single right-hand side.
+template <typename T, int SYNC>
+__global__ void __launch_bounds__(kBlock)
+    k_mix(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
+          const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, const int *__restrict__ unit, int n_units,
+          unsigned *__restrict__ ticket)
+{
+    const int u = wave_ticket(ticket, 1);
+    if (u >= n_units) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r0 = unit[u], r1 = unit[u + 1];
+    if (r1 - r0 > 1) {
+        const int i = r0 + lane;
+        if (i >= r1) return;
+        wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0);
+        return;
+    }
+    const int row = r0;
+    const int d = rp[row + 1] - 1;
+    T s = (T)0;
+    for (int j = rp[row] + lane; j < d; j += kWave) s = fmac(val[j], wait_value<T, SYNC>(x, sc, ci[j]), s);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) publish<T, SYNC>(x, sc, row, (b[row] - s) / val[d]);
+}
+
+// ------------------------------------------------ SyncFree CSC (K4 / K5)
+// sptrsv_syncfree_cuda_analyser (CUDA/SyncFree_csc/spts_syncfree_cuda2.h:10-21)
+__global__ void __launch_bounds__(kBlock) k_indegree(const int *__restrict__ idx, int nnz, int *__restrict__ deg)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nnz) atomic_add_agent(deg + idx[g], 1);
+}
+
+// sptrsv_syncfree_cuda_executor_update (spts_syncfree_cuda2.h:24-86) at
+// wave64: one wavefront per column; wait until only the diagonal's own
+// count remains, x_i = (b_i - left_sum_i) / diag, then every lane scatters
+// left_sum[r] += x_i * v and releases one count of row r.  FORWARD takes
+// the diagonal from the first entry of the column, BACKWARD from the last.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_csc(const int *__restrict__ cp, const int *__restrict__ ri, const T *__restrict__ cv,
+          int *__restrict__ deg, T *__restrict__ left_sum, const T *__restrict__ b, T *__restrict__ x, int m,
+          int backward, SyncCtx sc, unsigned *__restrict__ ticket)
+{
+    typedef typename VBits<T>::U U;
+    int col = wave_ticket(ticket, 1);
+    if (col >= m) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (backward) col = m - 1 - col;
+    const int dpos = backward ? cp[col + 1] - 1 : cp[col];
+    int idle = 0;
+    bool failed = false;
+    while (ld_agent(deg + col) != 1)
+        if (give_up(sc, ++idle)) {
+            failed = true;
+            break;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
+    const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
+    const int start = backward ? cp[col] : cp[col] + 1;
+    const int stop = backward ? cp[col + 1] - 1 : cp[col + 1];
+    for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(left_sum + ri[j], xi * cv[j]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(deg + ri[j], -1);
+    if (lane == 0) x[col] = xi;
+}
+
+// ---------------------------------------------------------------- adjoint
+// grad_b = L^{-T} g: the CSC executor's BACKWARD mode
+// (spts_syncfree_cuda2.h:50-55, :70-76) applied to the CSR of L read as the
+// CSC of L^T, at thread granularity: one lane per node i (tickets taken in
+// descending i), write-first so lanes of one wave never block each other.
+// cnt[c] counts the off-diagonal entries of column c of L still unapplied.
+// The same lane writes grad_val[j] = -grad_b[i] * x[col_j] for its row.
+__global__ void __launch_bounds__(kBlock)
+    k_adjoint(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
+              const double *__restrict__ g, const double *__restrict__ xfwd, int *__restrict__ cnt,
+              double *__restrict__ left_sum, double *__restrict__ grad_b, double *__restrict__ grad_val, int n,
+              SyncCtx sc, unsigned *__restrict__ ticket)
+{
+    const int base = wave_ticket(ticket, kWave);
+    const int t = base + (threadIdx.x & (kWave - 1));
+    if (t >= n) return;
+    const int i = n - 1 - t;
+    const int a = rp[i], d = rp[i + 1] - 1;
+    int idle = 0;
+    bool done = false;
+    do {  // wave-uniform exit, as in wrtfst_row
+        if (!done) {
+            const bool ready = ld_agent(cnt + i) == 0;
+            const bool failed = !ready && give_up(sc, ++idle);
+            if (ready || failed) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const double ls = as_val(ld_agent(reinterpret_cast<const unsigned long long *>(left_sum) + i));
+                const double y = failed ? as_val(VBits<double>::kQNaN) : (g[i] - ls) / val[d];
+                grad_b[i] = y;
+                for (int j = a; j < d; ++j) atomic_add_agent(left_sum + ci[j], y * val[j]);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                for (int j = a; j < d; ++j) atomic_add_agent(cnt + ci[j], -1);
+                if (grad_val)
+                    for (int j = a; j <= d; ++j) grad_val[j] = -y * xfwd[ci[j]];
+                asm volatile("" ::: "memory");
+                done = true;
+            }
+        }
+    } while (__any(!done));
+}
+
+// off-diagonal column counts of L (the adjoint's in-degrees)
+__global__ void __launch_bounds__(kBlock)
+    k_offdiag_colcount(const int *__restrict__ rp, const int *__restrict__ ci, int n, int *__restrict__ cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = rp[i + 1] - 1;
+    for (int j = rp[i]; j < d; ++j) atomic_add_agent(cnt + ci[j], 1);
+}
+
+// ------------------------------------------------------------ refinement
+// r = b - L x in FP64 (rows in CSR order, fused multiply-add chain), written
+// as FP32 for the next correction solve; also accumulates max|x| for the
+// convergence test (non-negative doubles order like their bit patterns).
+__global__ void __launch_bounds__(kBlock)
+    k_residual32(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
+                 const double *__restrict__ b, const double *__restrict__ x, float *__restrict__ r32, int n, int k,
+                 unsigned long long *__restrict__ xmax_bits)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)n * k) return;
+    const int i = (int)(g / k), q = (int)(g - (long long)i * k);
+    double s = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[(long long)ci[j] * k + q], s);
+    r32[g] = (float)(b[g] - s);
+    atomicMax(xmax_bits, (unsigned long long)__double_as_longlong(fabs(x[g])));
+}
+
+// x += d (FP32 correction) and max|d| for the convergence test
+__global__ void __launch_bounds__(kBlock)
+    k_accumulate(double *__restrict__ x, const float *__restrict__ dcor, long long count,
+                 unsigned long long *__restrict__ dmax_bits, int first)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const double dv = (double)dcor[g];
+    x[g] = first ? dv : x[g] + dv;
+    atomicMax(dmax_bits, (unsigned long long)__double_as_longlong(fabs(dv)));
+}
+
+__global__ void k_cast_f32(const double *__restrict__ a, float *__restrict__ o, long long count)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < count) o[g] = (float)a[g];
+}
+
+// Batch: local column indices -> columns of the block-diagonal matrix
+// (matrix of each row found by binary search over row_off).
+__global__ void __launch_bounds__(kBlock)
+    k_globalize(const int *__restrict__ row_off, int n_mat, const int *__restrict__ rp,
+                const int *__restrict__ ci_local, int n, int *__restrict__ ci)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = n_mat - 1;
+    while (lo < hi) {  // largest m with row_off[m] <= i
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const int off = row_off[lo];
+    for (int j = rp[i]; j < rp[i + 1]; ++j) ci[j] = ci_local[j] + off;
+}
+
+}  // namespace cap

@@ -1,0 +1,1061 @@
+// C ABI of the gfx950 SpTRSV: plans (device-resident solves, adjoint,
+// FP32+FP64 refinement), batches, one-shot host solves and the
+// benchmark-compatible wrappers that replace the reference's host wrappers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/capellini_sptrsv.h"
+#include "cap_internal.h"
+#include "kernels.h"
+
+using namespace cap;
+
+namespace {
+
+// Threshold of the mix selector for 64-row chunks: a chunk whose mean
+// nnz/row reaches it is solved wavefront-per-row.  The reference hard-codes
+// 10 for 32-lane warps (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:188); this
+// default is the value measured best on MI355X for the banded workloads.
+constexpr int kMixBorderDefault = 16;
+constexpr int kSpinLimitDefault = 1 << 24;  // polls without progress before the watchdog fires
+constexpr int kTickets = 4;
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline unsigned blocks_for(long long threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
+inline int lpr_log2_for(int k)
+{
+    int l = 0;
+    while ((1 << l) < k) ++l;
+    return l;
+}
+
+}  // namespace
+
+struct cap_plan_s {
+    int n = 0, nnz = 0, vt = CAP_F64, vbytes = 8;
+    int algo = CAP_ALGO_WRTFST, sync = CAP_SYNC_VALUE, border = kMixBorderDefault;
+    int device = 0;
+    int *rp = nullptr, *ci = nullptr;
+    void *val = nullptr;
+    bool owns_matrix = false;
+    // synchronisation state
+    int *flags = nullptr;
+    long long flags_len = 0;
+    int epoch = 0;
+    unsigned *ticket = nullptr;
+    int *err = nullptr;
+    int spin_limit = kSpinLimitDefault;
+    // mix
+    int *unit = nullptr;
+    int n_units = 0, mix_len = 0;
+    double wocc = 0, eocc = 0;
+    // CSC executor (forward)
+    int *cp = nullptr, *cri = nullptr, *deg0 = nullptr, *deg = nullptr;
+    void *cv = nullptr, *left_sum = nullptr;
+    // adjoint
+    int *cnt0 = nullptr, *cnt = nullptr;
+    double *adj_ls = nullptr;
+    // refinement
+    float *val32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+    long long refine_len = 0;
+    unsigned long long *norms = nullptr;
+    hipGraphExec_t refine_graph = nullptr;
+    const void *graph_b = nullptr;
+    void *graph_x = nullptr;
+    int graph_rhs = 0;
+};
+
+struct cap_batch_s {
+    cap_plan plan = nullptr;
+    int k = 1;
+    int n_mat = 0;
+};
+
+namespace {
+
+// ------------------------------------------------------------ device check
+int check_device(int *dev)
+{
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) {
+        std::fprintf(stderr, "capellini: hipGetDeviceCount -> %d (%s), count %d\n", (int)e, hipGetErrorString(e),
+                     count);
+        return CAP_ERR_NO_DEVICE;
+    }
+    CAP_HIP_TRY(hipGetDevice(dev));
+    hipDeviceProp_t prop;
+    CAP_HIP_TRY(hipGetDeviceProperties(&prop, *dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::fprintf(stderr, "capellini: device %d is %s, this library is built for gfx950\n", *dev,
+                     prop.gcnArchName);
+        return CAP_ERR_NO_DEVICE;
+    }
+    return CAP_OK;
+}
+
+template <typename P>
+int dev_alloc(P **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, count * sizeof(P)) != hipSuccess) {
+        *p = nullptr;
+        return CAP_ERR_ALLOC;
+    }
+    return CAP_OK;
+}
+
+#define CAP_TRY(expr)                       \
+    do {                                    \
+        int cap_rc_ = (expr);               \
+        if (cap_rc_ != CAP_OK) return cap_rc_; \
+    } while (0)
+
+void plan_free(cap_plan p)
+{
+    if (!p) return;
+    if (p->owns_matrix) {
+        (void)hipFree(p->rp);
+        (void)hipFree(p->ci);
+        (void)hipFree(p->val);
+    }
+    void *bufs[] = {p->flags, p->ticket, p->err, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
+                    p->cnt0, p->cnt, p->adj_ls, p->val32, p->r32, p->d32, p->norms};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->refine_graph) (void)hipGraphExecDestroy(p->refine_graph);
+    delete p;
+}
+
+int ensure_flags(cap_plan p, long long len, hipStream_t st)
+{
+    if (p->flags_len >= len) return CAP_OK;
+    if (p->flags) CAP_HIP_TRY(hipFree(p->flags));
+    p->flags = nullptr;
+    CAP_TRY(dev_alloc(&p->flags, (size_t)len));
+    k_prepare<unsigned><<<blocks_for(len), kBlock, 0, st>>>((unsigned *)p->flags, len, 0u, p->ticket, 0);
+    CAP_HIP_TRY(hipGetLastError());
+    p->flags_len = len;
+    p->epoch = 0;
+    return CAP_OK;
+}
+
+// Per-solve reset (tickets; sentinel fill of x in the value protocol).
+template <typename T>
+int launch_prepare(cap_plan p, int sync, void *x, long long count, hipStream_t st)
+{
+    typedef typename VBits<T>::U U;
+    if (sync == CAP_SYNC_VALUE) {
+        const unsigned nb = std::max(1u, std::min(blocks_for(count), 2048u));
+        k_prepare<U><<<nb, kBlock, 0, st>>>((U *)x, count, VBits<T>::kSentinel, p->ticket, kTickets);
+    } else {
+        CAP_TRY(ensure_flags(p, count, st));
+        if (++p->epoch == 0x7fffffff) {  // wrap: clear the flags once
+            k_prepare<unsigned><<<blocks_for(p->flags_len), kBlock, 0, st>>>((unsigned *)p->flags, p->flags_len,
+                                                                             0u, p->ticket, 0);
+            p->epoch = 1;
+        }
+        k_prepare<U><<<1, kBlock, 0, st>>>((U *)x, 0, (U)0, p->ticket, kTickets);
+    }
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+// The solve kernel itself (after launch_prepare).
+template <typename T>
+int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x, int rhs, hipStream_t st)
+{
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
+    const int n = p->n;
+    if (n == 0) return CAP_OK;
+    if (algo == CAP_ALGO_WRTFST) {
+        if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
+        const int l = lpr_log2_for(rhs);
+        const long long waves = ((long long)n + (kWave >> l) - 1) / (kWave >> l);
+        const unsigned nb = blocks_for(waves * kWave);
+        if (sync == CAP_SYNC_VALUE)
+            k_wrtfst<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
+        else
+            k_wrtfst<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
+    } else if (algo == CAP_ALGO_TWOPART) {
+        if (rhs != 1) return CAP_ERR_UNSUPPORTED;
+        const unsigned nb = blocks_for(((long long)n + kWave - 1) / kWave * kWave);
+        if (sync == CAP_SYNC_VALUE)
+            k_twopart<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, p->ticket);
+        else
+            k_twopart<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, p->ticket);
+    } else if (algo == CAP_ALGO_MIX) {
+        if (rhs != 1) return CAP_ERR_UNSUPPORTED;
+        const unsigned nb = blocks_for((long long)p->n_units * kWave);
+        if (sync == CAP_SYNC_VALUE)
+            k_mix<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
+        else
+            k_mix<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
+    } else if (algo == CAP_ALGO_CSC) {
+        if (rhs != 1) return CAP_ERR_UNSUPPORTED;
+        const unsigned nb = blocks_for((long long)n * kWave);
+        k_csc<T><<<nb, kBlock, 0, st>>>(p->cp, p->cri, (const T *)p->cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc,
+                                        p->ticket);
+    } else {
+        return CAP_ERR_UNSUPPORTED;
+    }
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+// CSC executor reset: in-degrees restored, left sums cleared (kernels only).
+template <typename T>
+int launch_csc_reset(cap_plan p, hipStream_t st)
+{
+    typedef typename VBits<T>::U U;
+    CAP_HIP_TRY(hipMemcpyAsync(p->deg, p->deg0, sizeof(int) * (size_t)p->n, hipMemcpyDeviceToDevice, st));
+    k_prepare<U><<<blocks_for(p->n), kBlock, 0, st>>>((U *)p->left_sum, p->n, (U)0, p->ticket, kTickets);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+template <typename T>
+int solve_t(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
+{
+    if (algo == CAP_ALGO_CSC) {
+        CAP_TRY(launch_csc_reset<T>(p, st));
+    } else {
+        CAP_TRY(launch_prepare<T>(p, sync, x, (long long)p->n * rhs, st));
+    }
+    return launch_kernel<T>(p, algo, sync, (const T *)p->val, (const T *)b, (T *)x, rhs, st);
+}
+
+int plan_solve_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
+{
+    return p->vt == CAP_F64 ? solve_t<double>(p, algo, sync, b, x, rhs, st)
+                            : solve_t<float>(p, algo, sync, b, x, rhs, st);
+}
+
+// ----------------------------------------------------------- plan analysis
+int build_mix(cap_plan p, const std::vector<int> &rp_host, int border)
+{
+    std::vector<int> wn((size_t)p->n + 1);
+    int len = 0;
+    cap_matrix_warp4(p->n, p->nnz, rp_host.data(), border, kWave, &len, wn.data(), &p->wocc, &p->eocc);
+    if (p->unit) CAP_HIP_TRY(hipFree(p->unit));
+    p->unit = nullptr;
+    CAP_TRY(dev_alloc(&p->unit, (size_t)len));
+    CAP_HIP_TRY(hipMemcpy(p->unit, wn.data(), sizeof(int) * (size_t)len, hipMemcpyHostToDevice));
+    p->mix_len = len;
+    p->n_units = len - 1;
+    p->border = border;
+    return CAP_OK;
+}
+
+template <typename T>
+int build_csc(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
+{
+    const int n = p->n, nnz = p->nnz;
+    std::vector<int> ci((size_t)nnz), cp((size_t)n + 1), cr((size_t)nnz);
+    std::vector<T> v((size_t)nnz), cv((size_t)nnz);
+    CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(T) * nnz, hipMemcpyDeviceToHost));
+    if (sizeof(T) == 8)
+        cap_transpose_d(n, n, nnz, rp_host.data(), ci.data(), (const double *)(const void *)v.data(), cr.data(),
+                        cp.data(), (double *)(void *)cv.data());
+    else
+        cap_transpose_s(n, n, nnz, rp_host.data(), ci.data(), (const float *)(const void *)v.data(), cr.data(),
+                        cp.data(), (float *)(void *)cv.data());
+    CAP_TRY(dev_alloc(&p->cp, (size_t)n + 1));
+    CAP_TRY(dev_alloc(&p->cri, (size_t)nnz));
+    CAP_TRY(dev_alloc((T **)&p->cv, (size_t)nnz));
+    CAP_TRY(dev_alloc(&p->deg0, (size_t)n));
+    CAP_TRY(dev_alloc(&p->deg, (size_t)n));
+    CAP_TRY(dev_alloc((T **)&p->left_sum, (size_t)n));
+    CAP_HIP_TRY(hipMemcpy(p->cp, cp.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    CAP_HIP_TRY(hipMemcpy(p->cri, cr.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
+    CAP_HIP_TRY(hipMemcpy(p->cv, cv.data(), sizeof(T) * (size_t)nnz, hipMemcpyHostToDevice));
+    k_prepare<unsigned><<<blocks_for(n), kBlock, 0, st>>>((unsigned *)p->deg0, n, 0u, p->ticket, 0);
+    k_indegree<<<blocks_for(nnz), kBlock, 0, st>>>(p->cri, nnz, p->deg0);
+    CAP_HIP_TRY(hipGetLastError());
+    CAP_HIP_TRY(hipStreamSynchronize(st));
+    return CAP_OK;
+}
+
+int ensure_csc(cap_plan p, hipStream_t st)
+{
+    if (p->cp) return CAP_OK;
+    std::vector<int> rp((size_t)p->n + 1);
+    CAP_HIP_TRY(hipMemcpy(rp.data(), p->rp, sizeof(int) * ((size_t)p->n + 1), hipMemcpyDeviceToHost));
+    return p->vt == CAP_F64 ? build_csc<double>(p, rp, st) : build_csc<float>(p, rp, st);
+}
+
+// ----------------------------------------------------------- autotuner
+struct TuneKey {
+    int n, nnz, vt, rhs;
+    unsigned long long fp;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(n, nnz, vt, rhs, fp) < std::tie(o.n, o.nnz, o.vt, o.rhs, o.fp);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::tuple<int, int, int>> g_tune_cache;
+
+unsigned long long fingerprint(const std::vector<int> &rp)
+{
+    unsigned long long h = 1469598103934665603ULL;
+    const size_t n = rp.size();
+    for (int s = 0; s < 257 && n; ++s) {
+        h ^= (unsigned long long)(unsigned)rp[(n - 1) * (size_t)s / 256];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, hipStream_t st, float *ms)
+{
+    hipEvent_t e0, e1;
+    CAP_HIP_TRY(hipEventCreate(&e0));
+    CAP_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int rc = CAP_OK;
+    for (int r = 0; r < 4 && rc == CAP_OK; ++r) {
+        rc = plan_solve_with(p, algo, sync, b, x, 1, st);
+        if (rc != CAP_OK) break;
+        (void)hipEventRecord(e0, st);
+        rc = plan_solve_with(p, algo, sync, b, x, 1, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (r > 0) best = std::min(best, t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = best;
+    return rc;
+}
+
+int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
+{
+    TuneKey key{p->n, p->nnz, p->vt, 1, fingerprint(rp_host)};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            p->algo = std::get<0>(it->second);
+            p->sync = std::get<1>(it->second);
+            if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, std::get<2>(it->second));
+            return CAP_OK;
+        }
+    }
+    // candidates: thread-per-row in both protocols, mix at three thresholds
+    void *b = nullptr, *x = nullptr;
+    CAP_HIP_TRY(hipMalloc(&b, (size_t)p->n * p->vbytes + 8));
+    CAP_HIP_TRY(hipMalloc(&x, (size_t)p->n * p->vbytes + 8));
+    CAP_HIP_TRY(hipMemsetAsync(b, 0, (size_t)p->n * p->vbytes, st));
+    struct Cand { int algo, sync, border; };
+    std::vector<Cand> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0}, {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0}};
+    const double avg = (double)p->nnz / std::max(1, p->n);
+    for (int bd : {8, 16, 32})
+        if (avg * 4 >= bd) cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd});
+    float best = 1e30f;
+    Cand pick = cands[0];
+    int rc = CAP_OK;
+    for (const Cand &c : cands) {
+        if (c.algo == CAP_ALGO_MIX && (rc = build_mix(p, rp_host, c.border)) != CAP_OK) break;
+        float t = 0;
+        if ((rc = time_candidate(p, c.algo, c.sync, b, x, st, &t)) != CAP_OK) break;
+        if (t < best) {
+            best = t;
+            pick = c;
+        }
+    }
+    (void)hipFree(b);
+    (void)hipFree(x);
+    if (rc != CAP_OK) return rc;
+    p->algo = pick.algo;
+    p->sync = pick.sync;
+    if (pick.algo == CAP_ALGO_MIX) CAP_TRY(build_mix(p, rp_host, pick.border));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[key] = std::make_tuple(pick.algo, pick.sync, pick.border);
+    return CAP_OK;
+}
+
+template <typename T>
+int validate_device_matrix(cap_plan p, std::vector<int> *rp_host)
+{
+    const int n = p->n, nnz = p->nnz;
+    std::vector<int> ci((size_t)nnz);
+    std::vector<T> v((size_t)nnz);
+    CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
+    CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(T) * (size_t)nnz, hipMemcpyDeviceToHost));
+    int bad = -1;
+    if (sizeof(T) == 8)
+        return cap_validate_csr_d(n, nnz, rp_host->data(), ci.data(), (const double *)(const void *)v.data(), &bad);
+    return cap_validate_csr_s(n, nnz, rp_host->data(), ci.data(), (const float *)(const void *)v.data(), &bad);
+}
+
+int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci, const void *val, int vt,
+                     const cap_opts *opts, hipStream_t st)
+{
+    cap_opts o;
+    cap_opts_default(&o);
+    if (opts) o = *opts;
+    if (!out || n < 0 || nnz < n || (vt != CAP_F64 && vt != CAP_F32) || !rp || (nnz > 0 && (!ci || !val)))
+        return CAP_ERR_ARG;
+    *out = nullptr;
+    int dev = 0;
+    CAP_TRY(check_device(&dev));
+    cap_plan p = new cap_plan_s();
+    p->n = n;
+    p->nnz = nnz;
+    p->vt = vt;
+    p->vbytes = vt == CAP_F64 ? 8 : 4;
+    p->device = dev;
+    int rc = CAP_OK;
+    std::vector<int> rp_host((size_t)n + 1);
+    do {
+        if (o.host_inputs) {
+            if (o.validate) {
+                int bad = -1;
+                rc = vt == CAP_F64 ? cap_validate_csr_d(n, nnz, rp, ci, (const double *)val, &bad)
+                                   : cap_validate_csr_s(n, nnz, rp, ci, (const float *)val, &bad);
+                if (rc != CAP_OK) break;
+            }
+            p->owns_matrix = true;
+            if ((rc = dev_alloc(&p->rp, (size_t)n + 1)) != CAP_OK) break;
+            if ((rc = dev_alloc(&p->ci, (size_t)nnz)) != CAP_OK) break;
+            if (hipMalloc(&p->val, (size_t)nnz * p->vbytes + 8) != hipSuccess) { rc = CAP_ERR_ALLOC; break; }
+            if (hipMemcpy(p->rp, rp, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->ci, ci, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->val, val, (size_t)nnz * p->vbytes, hipMemcpyHostToDevice) != hipSuccess) {
+                rc = CAP_ERR_HIP;
+                break;
+            }
+            std::memcpy(rp_host.data(), rp, sizeof(int) * ((size_t)n + 1));
+        } else {
+            p->rp = const_cast<int *>(rp);
+            p->ci = const_cast<int *>(ci);
+            p->val = const_cast<void *>(val);
+            if (hipMemcpy(rp_host.data(), rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = CAP_ERR_HIP;
+                break;
+            }
+            if (o.validate) {
+                rc = vt == CAP_F64 ? validate_device_matrix<double>(p, &rp_host)
+                                   : validate_device_matrix<float>(p, &rp_host);
+                if (rc != CAP_OK) break;
+            }
+        }
+        if ((rc = dev_alloc(&p->ticket, kTickets)) != CAP_OK) break;
+        if ((rc = dev_alloc(&p->err, 4)) != CAP_OK) break;
+        if (hipMemset(p->err, 0, 16) != hipSuccess || hipMemset(p->ticket, 0, 16) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
+        p->algo = o.algo;
+        p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
+        const int border = o.border > 0 ? o.border : kMixBorderDefault;
+        if (p->algo == CAP_ALGO_MIX) {
+            rc = build_mix(p, rp_host, border);
+        } else if (p->algo == CAP_ALGO_CSC) {
+            rc = ensure_csc(p, st);
+        } else if (p->algo == CAP_ALGO_AUTO) {
+            rc = autotune(p, rp_host, st);
+        } else if (p->algo != CAP_ALGO_WRTFST && p->algo != CAP_ALGO_TWOPART) {
+            rc = CAP_ERR_ARG;
+        }
+    } while (0);
+    if (rc != CAP_OK) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return CAP_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void cap_opts_default(cap_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->algo = CAP_ALGO_AUTO;
+    o->sync = CAP_SYNC_AUTO;
+    o->validate = 1;
+}
+
+int cap_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int cap_plan_create(cap_plan *plan, int n, int nnz, const int *row_ptr, const int *col_idx, const void *val,
+                    int value_type, const cap_opts *opts, void *stream)
+{
+    return plan_create_impl(plan, n, nnz, row_ptr, col_idx, val, value_type, opts, S(stream));
+}
+
+int cap_plan_solve(cap_plan p, const void *b, void *x, int rhs, void *stream)
+{
+    if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
+    return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
+}
+
+int cap_plan_info(cap_plan p, int *algo, int *sync, int *border, int *mix_len, double *wocc, double *eocc)
+{
+    if (!p) return CAP_ERR_ARG;
+    if (algo) *algo = p->algo;
+    if (sync) *sync = p->sync;
+    if (border) *border = p->border;
+    if (mix_len) *mix_len = p->mix_len;
+    if (wocc) *wocc = p->wocc;
+    if (eocc) *eocc = p->eocc;
+    return CAP_OK;
+}
+
+// Reads and clears the watchdog word: CAP_OK, or CAP_ERR_NOT_CONVERGED when
+// a wait gave up since the last call (the affected rows hold NaN).
+int cap_plan_status(cap_plan p, void *stream)
+{
+    if (!p) return CAP_ERR_ARG;
+    int e = 0;
+    CAP_HIP_TRY(hipStreamSynchronize(S(stream)));
+    CAP_HIP_TRY(hipMemcpy(&e, p->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) CAP_HIP_TRY(hipMemset(p->err, 0, sizeof(int)));
+    return e ? CAP_ERR_WATCHDOG : CAP_OK;
+}
+
+int cap_plan_destroy(cap_plan p)
+{
+    if (!p) return CAP_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    plan_free(p);
+    return CAP_OK;
+}
+
+int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, void *grad_val, void *stream)
+{
+    if (!p || !g || !x || !grad_b) return CAP_ERR_ARG;
+    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
+    hipStream_t st = S(stream);
+    const int n = p->n;
+    if (!p->cnt0) {
+        CAP_TRY(dev_alloc(&p->cnt0, (size_t)n));
+        CAP_TRY(dev_alloc(&p->cnt, (size_t)n));
+        CAP_TRY(dev_alloc(&p->adj_ls, (size_t)n));
+        k_prepare<unsigned><<<blocks_for(n), kBlock, 0, st>>>((unsigned *)p->cnt0, n, 0u, p->ticket, 0);
+        k_offdiag_colcount<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, n, p->cnt0);
+        CAP_HIP_TRY(hipGetLastError());
+    }
+    if (n == 0) return CAP_OK;
+    CAP_HIP_TRY(hipMemcpyAsync(p->cnt, p->cnt0, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
+    k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
+                                                                    p->ticket, kTickets);
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
+    k_adjoint<<<blocks_for(((long long)n + kWave - 1) / kWave * kWave), kBlock, 0, st>>>(
+        p->rp, p->ci, (const double *)p->val, (const double *)g, (const double *)x, p->cnt, p->adj_ls,
+        (double *)grad_b, (double *)grad_val, n, sc, p->ticket);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- refinement
+namespace {
+
+int ensure_refine(cap_plan p, int rhs, hipStream_t st)
+{
+    const long long len = (long long)p->n * rhs;
+    if (!p->val32) {
+        CAP_TRY(dev_alloc(&p->val32, (size_t)p->nnz));
+        k_cast_f32<<<blocks_for(p->nnz), kBlock, 0, st>>>((const double *)p->val, p->val32, p->nnz);
+        CAP_HIP_TRY(hipGetLastError());
+        CAP_TRY(dev_alloc(&p->norms, 4));
+    }
+    if (p->refine_len < len) {
+        if (p->r32) CAP_HIP_TRY(hipFree(p->r32));
+        if (p->d32) CAP_HIP_TRY(hipFree(p->d32));
+        p->r32 = p->d32 = nullptr;
+        CAP_TRY(dev_alloc(&p->r32, (size_t)len));
+        CAP_TRY(dev_alloc(&p->d32, (size_t)len));
+        p->refine_len = len;
+    }
+    return CAP_OK;
+}
+
+// One refinement step on `st`: r32 = b - L x (or b when first), d = L32^{-1} r32
+// in FP32 (value protocol), x += d; norms[0] = max|d| bits, norms[1] = max|x|.
+int refine_step(cap_plan p, const double *b, double *x, int rhs, bool first, hipStream_t st)
+{
+    const long long len = (long long)p->n * rhs;
+    k_prepare<unsigned long long><<<1, kBlock, 0, st>>>(p->norms, 2, 0ULL, p->ticket, 0);
+    if (first)
+        k_cast_f32<<<blocks_for(len), kBlock, 0, st>>>(b, p->r32, len);
+    else
+        k_residual32<<<blocks_for(len), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, b, x, p->r32, p->n,
+                                                          rhs, p->norms + 1);
+    CAP_HIP_TRY(hipGetLastError());
+    CAP_TRY(launch_prepare<float>(p, CAP_SYNC_VALUE, p->d32, len, st));
+    CAP_TRY(launch_kernel<float>(p, CAP_ALGO_WRTFST, CAP_SYNC_VALUE, p->val32, p->r32, p->d32, rhs, st));
+    k_accumulate<<<blocks_for(len), kBlock, 0, st>>>(x, p->d32, len, p->norms, first ? 1 : 0);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, int max_iter, int *iters,
+                 double *last_rel, hipStream_t st)
+{
+    if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
+    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
+    CAP_TRY(ensure_refine(p, rhs, st));
+    // The first two steps (solve from b, one correction) are one graph:
+    // captured once per (b, x, rhs), replayed on later calls.
+    if (!p->refine_graph || p->graph_b != b || p->graph_x != x || p->graph_rhs != rhs) {
+        if (p->refine_graph) CAP_HIP_TRY(hipGraphExecDestroy(p->refine_graph));
+        p->refine_graph = nullptr;
+        hipStream_t cs;
+        CAP_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t graph;
+        CAP_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        int rc = refine_step(p, b, x, rhs, true, cs);
+        if (rc == CAP_OK) rc = refine_step(p, b, x, rhs, false, cs);
+        hipError_t ec = hipStreamEndCapture(cs, &graph);
+        if (rc != CAP_OK || ec != hipSuccess) {
+            (void)hipStreamDestroy(cs);
+            return rc != CAP_OK ? rc : CAP_ERR_HIP;
+        }
+        CAP_HIP_TRY(hipGraphInstantiate(&p->refine_graph, graph, nullptr, nullptr, 0));
+        CAP_HIP_TRY(hipGraphDestroy(graph));
+        CAP_HIP_TRY(hipStreamDestroy(cs));
+        p->graph_b = b;
+        p->graph_x = x;
+        p->graph_rhs = rhs;
+    }
+    double prev_rel = 1.0, rel = 1.0;
+    int it = 0;
+    unsigned long long nb[2];
+    int rc = CAP_OK;
+    CAP_HIP_TRY(hipGraphLaunch(p->refine_graph, st));
+    it = 2;
+    while (true) {
+        CAP_HIP_TRY(hipMemcpyAsync(nb, p->norms, sizeof(nb), hipMemcpyDeviceToHost, st));
+        CAP_HIP_TRY(hipStreamSynchronize(st));
+        double dmax, xmax;
+        std::memcpy(&dmax, &nb[0], 8);
+        std::memcpy(&xmax, &nb[1], 8);
+        prev_rel = it == 2 ? 1.0 : rel;
+        rel = xmax > 0 ? dmax / xmax : dmax;
+        // the error left after this correction is about rel times the
+        // observed contraction rel/prev_rel
+        const double est = rel * std::min(1.0, rel / std::max(prev_rel, 1e-300));
+        if (rel <= tol || est <= tol) break;
+        if (it >= max_iter) {
+            rc = CAP_ERR_NOT_CONVERGED;
+            break;
+        }
+        CAP_TRY(refine_step(p, b, x, rhs, false, st));
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (last_rel) *last_rel = rel;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int cap_plan_solve_refined(cap_plan p, const double *b, double *x, int rhs, double tol, int max_iter,
+                                      int *iters, double *last_rel, void *stream)
+{
+    return refined_impl(p, b, x, rhs, tol, max_iter, iters, last_rel, S(stream));
+}
+
+// ---------------------------------------------------------------- batches
+extern "C" {
+
+int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *row_ptr, const int *col_idx,
+                     const void *val, int vt, int k, const cap_opts *opts, void *stream)
+{
+    if (!out || n_mat < 1 || !row_off || !row_ptr || !col_idx || !val || k < 1 || k > kWave ||
+        (vt != CAP_F64 && vt != CAP_F32))
+        return CAP_ERR_ARG;
+    *out = nullptr;
+    cap_opts o;
+    cap_opts_default(&o);
+    if (opts) o = *opts;
+    hipStream_t st = S(stream);
+    int dev = 0;
+    CAP_TRY(check_device(&dev));
+    const size_t vb = vt == CAP_F64 ? 8 : 4;
+    std::vector<int> ro((size_t)n_mat + 1);
+    if (o.host_inputs)
+        std::memcpy(ro.data(), row_off, sizeof(int) * ro.size());
+    else
+        CAP_HIP_TRY(hipMemcpy(ro.data(), row_off, sizeof(int) * ro.size(), hipMemcpyDeviceToHost));
+    const int n = ro[n_mat];
+    if (ro[0] != 0 || n < 0) return CAP_ERR_ARG;
+    for (int m = 0; m < n_mat; ++m)
+        if (ro[m + 1] < ro[m]) return CAP_ERR_ARG;
+    int nnz = 0;
+    if (o.host_inputs)
+        nnz = row_ptr[n];
+    else
+        CAP_HIP_TRY(hipMemcpy(&nnz, row_ptr + n, sizeof(int), hipMemcpyDeviceToHost));
+    if (nnz < n) return CAP_ERR_ARG;
+    int *d_ro = nullptr, *d_cil = nullptr, *g_rp = nullptr, *g_ci = nullptr;
+    void *g_val = nullptr;
+    int rc = CAP_OK;
+    const hipMemcpyKind kind = o.host_inputs ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    do {
+        if ((rc = dev_alloc(&d_ro, ro.size())) || (rc = dev_alloc(&g_rp, (size_t)n + 1)) ||
+            (rc = dev_alloc(&g_ci, (size_t)nnz)) || (rc = dev_alloc(&d_cil, (size_t)nnz)))
+            break;
+        if (hipMalloc(&g_val, (size_t)nnz * vb + 8) != hipSuccess) { rc = CAP_ERR_ALLOC; break; }
+        if (hipMemcpy(d_ro, ro.data(), sizeof(int) * ro.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(g_rp, row_ptr, sizeof(int) * ((size_t)n + 1), kind) != hipSuccess ||
+            hipMemcpy(d_cil, col_idx, sizeof(int) * (size_t)nnz, kind) != hipSuccess ||
+            hipMemcpy(g_val, val, (size_t)nnz * vb, kind) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        k_globalize<<<blocks_for(n), kBlock, 0, st>>>(d_ro, n_mat, g_rp, d_cil, n, g_ci);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+    } while (0);
+    (void)hipFree(d_ro);
+    (void)hipFree(d_cil);
+    cap_plan p = nullptr;
+    if (rc == CAP_OK) {
+        cap_opts po = o;
+        po.host_inputs = 0;
+        po.algo = CAP_ALGO_WRTFST;
+        if (!po.sync) po.sync = CAP_SYNC_VALUE;
+        rc = plan_create_impl(&p, n, nnz, g_rp, g_ci, g_val, vt, &po, st);
+    }
+    if (rc != CAP_OK) {
+        (void)hipFree(g_rp);
+        (void)hipFree(g_ci);
+        (void)hipFree(g_val);
+        return rc;
+    }
+    p->owns_matrix = true;
+    cap_batch bt = new cap_batch_s();
+    bt->plan = p;
+    bt->k = k;
+    bt->n_mat = n_mat;
+    *out = bt;
+    return CAP_OK;
+}
+
+int cap_batch_solve(cap_batch bt, const void *b, void *x, void *stream)
+{
+    if (!bt || !b || !x) return CAP_ERR_ARG;
+    return plan_solve_with(bt->plan, CAP_ALGO_WRTFST, bt->plan->sync, b, x, bt->k, S(stream));
+}
+
+int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol, int max_iter, int *iters,
+                            double *last_rel, void *stream)
+{
+    if (!bt) return CAP_ERR_ARG;
+    return refined_impl(bt->plan, b, x, bt->k, tol, max_iter, iters, last_rel, S(stream));
+}
+
+int cap_batch_destroy(cap_batch bt)
+{
+    if (!bt) return CAP_ERR_ARG;
+    cap_plan_destroy(bt->plan);
+    delete bt;
+    return CAP_OK;
+}
+
+// -------------------------------------------------------- one-shot solves
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int oneshot(int n, int nnz, const int *rp, const int *ci, const T *val, const T *b, T *x, const cap_opts *opts)
+{
+    if (!rp || !b || !x || n < 0) return CAP_ERR_ARG;
+    cap_opts o;
+    cap_opts_default(&o);
+    if (opts) o = *opts;
+    o.host_inputs = 1;
+    if (o.algo == CAP_ALGO_AUTO) o.algo = CAP_ALGO_WRTFST;
+    cap_plan p = nullptr;
+    CAP_TRY(plan_create_impl(&p, n, nnz, rp, ci, val, sizeof(T) == 8 ? CAP_F64 : CAP_F32, &o, nullptr));
+    T *db = nullptr, *dx = nullptr;
+    int rc = CAP_OK;
+    do {
+        if ((rc = dev_alloc(&db, (size_t)n)) || (rc = dev_alloc(&dx, (size_t)n))) break;
+        if (hipMemcpy(db, b, sizeof(T) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) { rc = CAP_ERR_HIP; break; }
+        if ((rc = plan_solve_with(p, p->algo, p->sync, db, dx, 1, nullptr)) != CAP_OK) break;
+        if (hipMemcpy(x, dx, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) { rc = CAP_ERR_HIP; break; }
+        rc = cap_plan_status(p, nullptr);
+    } while (0);
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    plan_free(p);
+    return rc;
+}
+
+// ------------------------------------------------ benchmark-compatible
+// Shared body of the CSR wrappers (spts_syncfree_cuda_csr_wrt.h:66-242).
+template <typename T>
+int bench_csr(int algo, const int *rp, const int *ci, const T *val, int m, int n, int nnzL, double *solve_time,
+              double *gflops, double *bw, double *pre_time, double *wocc, double *eocc)
+{
+    if (m != n) {
+        std::printf("This is not a square matrix, return.\n");
+        return -1;
+    }
+    if (!solve_time || !gflops || !bw) return CAP_ERR_ARG;
+    std::vector<T> x_ref((size_t)n), b((size_t)m), x((size_t)n);
+    // get_x_b (read_mtx.h:260-278)
+    if (sizeof(T) == 8)
+        cap_get_x_b_d(m, n, rp, ci, (const double *)(const void *)val, (double *)(void *)x_ref.data(),
+                      (double *)(void *)b.data());
+    else
+        cap_get_x_b_s(m, n, rp, ci, (const float *)(const void *)val, (float *)(void *)x_ref.data(),
+                      (float *)(void *)b.data());
+    double pre_ms = 0.0;
+    cap_opts o;
+    cap_opts_default(&o);
+    o.algo = algo;
+    o.sync = CAP_SYNC_VALUE;
+    o.host_inputs = 1;
+    if (algo == CAP_ALGO_MIX) {
+        // pre_time: mean host time of the chunk table over the repeats
+        // (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:185-194)
+        std::vector<int> wn((size_t)m + 1);
+        int len = 0;
+        double wo = 0, eo = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < CAP_BENCH_REPEAT; ++r)
+            cap_matrix_warp4(m, nnzL, rp, kMixBorderDefault, kWave, &len, wn.data(), &wo, &eo);
+        auto t1 = std::chrono::steady_clock::now();
+        pre_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / CAP_BENCH_REPEAT;
+    }
+    cap_plan p = nullptr;
+    CAP_TRY(plan_create_impl(&p, n, nnzL, rp, ci, val, sizeof(T) == 8 ? CAP_F64 : CAP_F32, &o, nullptr));
+    T *db = nullptr, *dx = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CAP_OK, passed = 0;
+    double total_ms = 0.0;
+    do {
+        if ((rc = dev_alloc(&db, (size_t)m)) || (rc = dev_alloc(&dx, (size_t)n))) break;
+        if (hipMemcpy(db, b.data(), sizeof(T) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        // warm-up (the reference's driver calls the wrapper twice)
+        if ((rc = plan_solve_with(p, p->algo, p->sync, db, dx, 1, nullptr)) != CAP_OK) break;
+        for (int r = 0; r < CAP_BENCH_REPEAT && rc == CAP_OK; ++r) {
+            // reset outside the timed region, as the reference does (:154-156)
+            if ((rc = launch_prepare<T>(p, p->sync, dx, n, nullptr)) != CAP_OK) break;
+            (void)hipEventRecord(e0, nullptr);
+            if ((rc = launch_kernel<T>(p, p->algo, p->sync, (const T *)p->val, db, dx, 1, nullptr)) != CAP_OK) break;
+            (void)hipEventRecord(e1, nullptr);
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = CAP_ERR_HIP; break; }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            total_ms += ms;
+            if (hipMemcpy(x.data(), dx, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = CAP_ERR_HIP;
+                break;
+            }
+            // validation of spts_syncfree_cuda_csr_wrt.h:190-201
+            int errs = 0;
+            for (int i = 0; i < n; ++i)
+                if (!(std::fabs((double)x_ref[i] - (double)x[i]) <= 0.01 * std::fabs((double)x_ref[i]))) ++errs;
+            if (!errs) ++passed;
+        }
+        if (rc == CAP_OK && cap_plan_status(p, nullptr) != CAP_OK) passed = 0;
+    } while (0);
+    int len = p->mix_len;
+    if (wocc) *wocc = p->wocc;
+    if (eocc) *eocc = p->eocc;
+    if (pre_time) *pre_time = pre_ms;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    plan_free(p);
+    if (rc != CAP_OK) return rc;
+    const double ms = total_ms / CAP_BENCH_REPEAT;
+    *solve_time = ms;
+    *gflops = 2.0 * nnzL / (1e6 * ms);
+    *bw = cap_datasize(m, n, nnzL, (int)sizeof(T), algo == CAP_ALGO_MIX ? len : 0) / (1e6 * ms);
+    if (passed != CAP_BENCH_REPEAT) {
+        std::printf("hip syncfree SpTS on L failed!\n");
+        *solve_time = -1;
+        *gflops = 0;
+        *bw = 0;
+    }
+    return 0;
+}
+
+// sptrsv_syncfree_cuda (CUDA/SyncFree_csc/spts_syncfree_cuda2.h:89-292)
+template <typename T>
+int bench_csc(const int *cp, const int *ri, const T *cv, int m, int n, int nnz, T *x, const T *b, const T *x_ref,
+              double *pre_time, double *solve_time, double *gflops, double *bw)
+{
+    if (m != n) {
+        std::printf("This is not a square matrix, return.\n");
+        return -1;
+    }
+    if (!cp || !ri || !cv || !x || !b || !x_ref || !pre_time || !solve_time || !gflops || !bw) return CAP_ERR_ARG;
+    int dev = 0;
+    CAP_TRY(check_device(&dev));
+    cap_plan p = new cap_plan_s();
+    p->n = n;
+    p->nnz = nnz;
+    p->vt = sizeof(T) == 8 ? CAP_F64 : CAP_F32;
+    p->vbytes = sizeof(T);
+    p->algo = CAP_ALGO_CSC;
+    int rc = CAP_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    T *db = nullptr, *dx = nullptr;
+    double analyse_ms = 0.0, total_ms = 0.0;
+    do {
+        if ((rc = dev_alloc(&p->ticket, kTickets)) || (rc = dev_alloc(&p->err, 4)) ||
+            (rc = dev_alloc(&p->cp, (size_t)n + 1)) || (rc = dev_alloc(&p->cri, (size_t)nnz)) ||
+            (rc = dev_alloc((T **)&p->cv, (size_t)nnz)) || (rc = dev_alloc(&p->deg0, (size_t)n)) ||
+            (rc = dev_alloc(&p->deg, (size_t)n)) || (rc = dev_alloc((T **)&p->left_sum, (size_t)n)) ||
+            (rc = dev_alloc(&db, (size_t)n)) || (rc = dev_alloc(&dx, (size_t)n)))
+            break;
+        if (hipMemset(p->err, 0, 16) != hipSuccess ||
+            hipMemcpy(p->cp, cp, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->cri, ri, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->cv, cv, sizeof(T) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(db, b, sizeof(T) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        // analysis (K4), timed over the repeats (:151-166)
+        for (int r = 0; r < CAP_BENCH_REPEAT; ++r) {
+            k_prepare<unsigned><<<blocks_for(n), kBlock, 0, nullptr>>>((unsigned *)p->deg0, n, 0u, p->ticket, 0);
+            (void)hipEventRecord(e0, nullptr);
+            k_indegree<<<blocks_for(nnz), kBlock, 0, nullptr>>>(p->cri, nnz, p->deg0);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            analyse_ms += ms;
+        }
+        if (hipGetLastError() != hipSuccess) { rc = CAP_ERR_HIP; break; }
+        for (int r = 0; r <= CAP_BENCH_REPEAT && rc == CAP_OK; ++r) {  // r == 0: warm-up
+            if ((rc = launch_csc_reset<T>(p, nullptr)) != CAP_OK) break;
+            (void)hipEventRecord(e0, nullptr);
+            if ((rc = launch_kernel<T>(p, CAP_ALGO_CSC, CAP_SYNC_VALUE, nullptr, db, dx, 1, nullptr)) != CAP_OK) break;
+            (void)hipEventRecord(e1, nullptr);
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = CAP_ERR_HIP; break; }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0) total_ms += ms;
+        }
+        if (rc == CAP_OK && hipMemcpy(x, dx, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = CAP_ERR_HIP;
+        if (rc == CAP_OK && cap_plan_status(p, nullptr) != CAP_OK) std::printf("hip syncfree SpTRSV watchdog fired\n");
+    } while (0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    plan_free(p);
+    if (rc != CAP_OK) return rc;
+    *pre_time = analyse_ms / CAP_BENCH_REPEAT;
+    const double ms = total_ms / CAP_BENCH_REPEAT;
+    *solve_time = ms;
+    *gflops = 2.0 * nnz / (1e6 * ms);
+    *bw = cap_datasize(m, n, nnz, (int)sizeof(T), 0) / (1e6 * ms);
+    // relative L1 validation (:245-264)
+    double ref = 0, res = 0;
+    for (int i = 0; i < n; ++i) {
+        ref += std::fabs((double)x_ref[i]);
+        res += std::fabs((double)x[i] - (double)x_ref[i]);
+    }
+    res = ref == 0 ? res : res / ref;
+    if (!(res < 1e-4)) {
+        std::printf("hip syncfree SpTRSV executor _NOT_ passed! |x-xref|/|xref| = %8.2e\n", res);
+        *solve_time = -1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sptrsv_csr_solve_d(int n, int nnz, const int *rp, const int *ci, const double *v, const double *b, double *x,
+                       const cap_opts *o)
+{
+    return oneshot<double>(n, nnz, rp, ci, v, b, x, o);
+}
+int sptrsv_csr_solve_s(int n, int nnz, const int *rp, const int *ci, const float *v, const float *b, float *x,
+                       const cap_opts *o)
+{
+    return oneshot<float>(n, nnz, rp, ci, v, b, x, o);
+}
+
+int spts_syncfree_hip_csr_wrt_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,
+                                double *gf, double *bw)
+{
+    return bench_csr<double>(CAP_ALGO_WRTFST, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_csr_wrt_s(const int *rp, const int *ci, const float *v, int m, int n, int nnzL, double *t,
+                                double *gf, double *bw)
+{
+    return bench_csr<float>(CAP_ALGO_WRTFST, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_csr_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,
+                            double *gf, double *bw)
+{
+    return bench_csr<double>(CAP_ALGO_TWOPART, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_csr_s(const int *rp, const int *ci, const float *v, int m, int n, int nnzL, double *t,
+                            double *gf, double *bw)
+{
+    return bench_csr<float>(CAP_ALGO_TWOPART, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_mix_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,
+                            double *gf, double *bw, double *pre, double *wo, double *eo)
+{
+    return bench_csr<double>(CAP_ALGO_MIX, rp, ci, v, m, n, nnzL, t, gf, bw, pre, wo, eo);
+}
+int spts_syncfree_hip_mix_s(const int *rp, const int *ci, const float *v, int m, int n, int nnzL, double *t,
+                            double *gf, double *bw, double *pre, double *wo, double *eo)
+{
+    return bench_csr<float>(CAP_ALGO_MIX, rp, ci, v, m, n, nnzL, t, gf, bw, pre, wo, eo);
+}
+int sptrsv_syncfree_hip_csc_d(const int *cp, const int *ri, const double *cv, int m, int n, int nnz, double *x,
+                              const double *b, const double *xr, double *pre, double *t, double *gf, double *bw)
+{
+    return bench_csc<double>(cp, ri, cv, m, n, nnz, x, b, xr, pre, t, gf, bw);
+}
+int sptrsv_syncfree_hip_csc_s(const int *cp, const int *ri, const float *cv, int m, int n, int nnz, float *x,
+                              const float *b, const float *xr, double *pre, double *t, double *gf, double *bw)
+{
+    return bench_csc<float>(cp, ri, cv, m, n, nnz, x, b, xr, pre, t, gf, bw);
+}
+
+}  // extern "C"

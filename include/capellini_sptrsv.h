@@ -1,0 +1,336 @@
+/*
+ * capellini_sptrsv.h -- C ABI of the MI355X (gfx950) sparse triangular solve.
+ *
+ * Lower-triangular CSR solve L x = b with thread-level synchronization-free
+ * kernels written for 64-lane wavefronts.  Every symbol is extern "C", takes
+ * plain pointers and sizes, and carries no HIP or torch types: a stream is a
+ * void* holding a hipStream_t (NULL = the null stream).
+ *
+ * Matrix layout expected by every solve entry (the layout the reference's
+ * change2tran produces, CUDA/ourWrtFst/read_mtx.h:194-257): square n x n CSR
+ * with int32 row_ptr[n+1] / col_idx[nnz]; every column of a row is < the
+ * row except the LAST entry, which is the diagonal; the diagonal is nonzero.
+ * Off-diagonals are accumulated in stored order (ascending after
+ * change2tran); duplicates are allowed, as the reference keeps them.  Inputs violating this are
+ * rejected with a negative code before anything is launched (the reference
+ * would hang the GPU or return NaN).
+ *
+ * Precision follows the reference's compile-time switch
+ * (CUDA/ourWrtFst/common.h:16-18): define VALUE_TYPE to float or double
+ * (default double) before including this header and the unsuffixed drop-in
+ * names resolve to the _s / _d entry points.  One shared library exports both.
+ */
+#ifndef CAPELLINI_SPTRSV_H
+#define CAPELLINI_SPTRSV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+#define CAP_OK 0
+#define CAP_ERR_NOT_SQUARE (-1)   /* m != n: the reference's only error code */
+#define CAP_ERR_ARG (-2)          /* NULL pointer, negative size, nnz mismatch */
+#define CAP_ERR_ROWPTR (-3)       /* row_ptr not monotone or not ending at nnz */
+#define CAP_ERR_UPPER (-4)        /* an off-diagonal entry with col >= row */
+#define CAP_ERR_NO_DIAG (-5)      /* last entry of a row is not its diagonal */
+#define CAP_ERR_ZERO_DIAG (-6)    /* diagonal value is zero */
+#define CAP_ERR_HIP (-7)          /* a HIP runtime call failed */
+#define CAP_ERR_NO_DEVICE (-8)    /* no usable gfx950 device */
+#define CAP_ERR_ALLOC (-9)        /* host or device allocation failed */
+#define CAP_ERR_UNSUPPORTED (-10) /* option combination not available */
+#define CAP_ERR_IO (-11)          /* Matrix Market file could not be read */
+#define CAP_ERR_NOT_CONVERGED (-12) /* refinement did not reach its tolerance */
+#define CAP_ERR_WATCHDOG (-13)    /* a dependency wait exceeded spin_limit polls;
+                                     the rows involved hold NaN */
+
+/* value types */
+#define CAP_F64 0
+#define CAP_F32 1
+
+/* solve algorithms (cap_opts.algo) */
+#define CAP_ALGO_AUTO 0    /* autotuner's choice (cached per matrix shape) */
+#define CAP_ALGO_WRTFST 1  /* thread-per-row write-first (ourWrtFst) */
+#define CAP_ALGO_TWOPART 2 /* thread-per-row, blocking for earlier waves (our2Part) */
+#define CAP_ALGO_MIX 3     /* per-64-row chunk: thread or wavefront per row (mix) */
+#define CAP_ALGO_CSC 4     /* wavefront-per-column in-degree scatter (SyncFree_csc) */
+
+/* readiness protocol (cap_opts.sync) */
+#define CAP_SYNC_AUTO 0
+#define CAP_SYNC_VALUE 1 /* x itself is the flag: x starts as a signalling-NaN
+                            sentinel; relaxed agent-scope 64/32-bit atomics */
+#define CAP_SYNC_FLAG 2  /* separate int flags published with a release store
+                            and read with acquire loads (agent scope) */
+
+typedef struct cap_opts {
+    int algo;         /* CAP_ALGO_* (0 = auto) */
+    int sync;         /* CAP_SYNC_* (0 = auto) */
+    int border;       /* mix: nnz/row threshold for wavefront mode (0 = tuned) */
+    int validate;     /* 1 = check the CSR layout before the first solve */
+    int host_inputs;  /* 1 = row_ptr/col_idx/val are host pointers (copied) */
+    int spin_limit;   /* polls without progress before a wait gives up
+                         (0 = default 2^24); the watchdog never lets a
+                         kernel spin forever on a malformed input */
+    int reserved[7];
+} cap_opts;
+
+/* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
+void cap_opts_default(cap_opts *o);
+
+/* ------------------------------------------------------ plan (device) API
+ * A plan binds one matrix resident in device memory.  With host_inputs = 0
+ * the plan keeps the caller's device pointers (they must outlive the plan);
+ * with host_inputs = 1 it copies them to the device.  Analysis (validation,
+ * the mix chunk table, the CSC in-degree) runs once at creation.
+ * cap_plan_solve is asynchronous on `stream`; b and x are device pointers of
+ * n*rhs values interleaved as x[i*rhs + k] (serialref.h:57-60 layout). */
+typedef struct cap_plan_s *cap_plan;
+
+int cap_plan_create(cap_plan *plan, int n, int nnz, const int *row_ptr,
+                    const int *col_idx, const void *val, int value_type,
+                    const cap_opts *opts, void *stream);
+int cap_plan_solve(cap_plan plan, const void *b, void *x, int rhs, void *stream);
+/* Synchronizes `stream`, then reads and clears the plan's watchdog word:
+ * CAP_OK, or CAP_ERR_WATCHDOG if any wait gave up since the last call. */
+int cap_plan_status(cap_plan plan, void *stream);
+/* Which algorithm / sync protocol the plan resolved to, and its mix table. */
+int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
+                  int *mix_len, double *warp_occupy, double *element_occupy);
+int cap_plan_destroy(cap_plan plan);
+
+/* Adjoint of x = L^{-1} b: grad_b = L^{-T} g (column-driven sync-free
+ * transposed solve on the same CSR; in-degree counters + atomic left sums) and
+ * grad_val[k] = -grad_b[row_k] * x[col_k] on the pattern (grad_val may be
+ * NULL).  FP64 plans only.  Device pointers, asynchronous on `stream`. */
+int cap_plan_adjoint(cap_plan plan, const void *g, const void *x, void *grad_b,
+                     void *grad_val, void *stream);
+
+/* FP32 solve inside FP64 iterative refinement.  `plan64` is an FP64 plan of
+ * L (used for the FP64 residual r = b - L x); the FP32 copy of the values is
+ * made once and cached inside it.  Iterates until max|x_k - x_{k-1}| <=
+ * tol * max|x_k| or max_iter; writes the iteration count and the final
+ * relative correction.  Device pointers, synchronous (returns after the last
+ * check). */
+int cap_plan_solve_refined(cap_plan plan64, const double *b, double *x, int rhs,
+                           double tol, int max_iter, int *iters,
+                           double *last_rel_correction, void *stream);
+
+/* ------------------------------------------------------------ batched API
+ * Many independent lower-triangular matrices with k interleaved right-hand
+ * sides each (1 <= k <= 64), solved in one launch with per-matrix slices of
+ * the readiness state.  Matrices are concatenated: matrix m owns rows
+ * [row_off[m], row_off[m+1]); row_ptr (n_total+1 entries) indexes the
+ * concatenated col_idx/val; column indices are LOCAL to their matrix.  b/x
+ * are concatenated the same way, k interleaved values per row. */
+typedef struct cap_batch_s *cap_batch;
+int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
+                     const int *row_ptr, const int *col_idx, const void *val,
+                     int value_type, int k, const cap_opts *opts, void *stream);
+int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
+int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
+                            double tol, int max_iter, int *iters,
+                            double *last_rel_correction, void *stream);
+int cap_batch_destroy(cap_batch batch);
+
+/* ------------------------------------------------- one-shot solves (host)
+ * sptrsv_csr_solve_{d,s}: validate, copy to the device, solve, copy x back.
+ * All pointers are host pointers.  `opts` may be NULL. */
+int sptrsv_csr_solve_d(int n, int nnz, const int *row_ptr, const int *col_idx,
+                       const double *val, const double *b, double *x,
+                       const cap_opts *opts);
+int sptrsv_csr_solve_s(int n, int nnz, const int *row_ptr, const int *col_idx,
+                       const float *val, const float *b, float *x,
+                       const cap_opts *opts);
+
+/* -------------------------------------------- benchmark-compatible entries
+ * Same parameter lists and semantics as the reference's wrappers: build
+ * x_ref = 1 and b = L*x_ref internally (get_x_b), run a warm-up plus
+ * CAP_BENCH_REPEAT timed solves (flags/x reset outside the timed region, as
+ * the reference does), validate every repetition with the reference's test
+ * (|x - x_ref| > 0.01|x_ref| counts as an error), and report
+ *   solve_time = mean ms, gflops = 2*nnzL/(1e6*ms),
+ *   bandwith  = dataSize/(1e6*ms), dataSize = 4(n+1) + 4(nnzL+m)
+ *               + V*nnzL + 2V*n (+4*Len for mix).
+ * A failed validation sets solve_time = -1, gflops = bandwith = 0 and still
+ * returns 0; a non-square input returns -1.  Other negative codes are this
+ * library's input checks (the reference does not check). */
+#define CAP_BENCH_REPEAT 100
+
+/* replaces spts_syncfree_cuda_csr_wrt, CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:66-75 */
+int spts_syncfree_hip_csr_wrt_d(const int *csrRowPtrL, const int *csrColIdxL,
+                                const double *csrValL, int m, int n, int nnzL,
+                                double *solve_time, double *gflops, double *bandwith);
+int spts_syncfree_hip_csr_wrt_s(const int *csrRowPtrL, const int *csrColIdxL,
+                                const float *csrValL, int m, int n, int nnzL,
+                                double *solve_time, double *gflops, double *bandwith);
+
+/* replaces spts_syncfree_cuda_csr (our2Part), CUDA/our2Part/spts_syncfree_cuda_csr.h:133-142 */
+int spts_syncfree_hip_csr_d(const int *csrRowPtrL, const int *csrColIdxL,
+                            const double *csrValL, int m, int n, int nnzL,
+                            double *solve_time, double *gflops, double *bandwith);
+int spts_syncfree_hip_csr_s(const int *csrRowPtrL, const int *csrColIdxL,
+                            const float *csrValL, int m, int n, int nnzL,
+                            double *solve_time, double *gflops, double *bandwith);
+
+/* replaces the mix spts_syncfree_cuda_csr_wrt, CUDA/mix/spts_syncfree_cuda_csr_wrt.h:146-158
+ * (pre_time = mean ms of the chunk-table construction over CAP_BENCH_REPEAT) */
+int spts_syncfree_hip_mix_d(const int *csrRowPtrL, const int *csrColIdxL,
+                            const double *csrValL, int m, int n, int nnzL,
+                            double *solve_time, double *gflops, double *bandwith,
+                            double *pre_time, double *warp_occupy,
+                            double *element_occupy);
+int spts_syncfree_hip_mix_s(const int *csrRowPtrL, const int *csrColIdxL,
+                            const float *csrValL, int m, int n, int nnzL,
+                            double *solve_time, double *gflops, double *bandwith,
+                            double *pre_time, double *warp_occupy,
+                            double *element_occupy);
+
+/* replaces sptrsv_syncfree_cuda, CUDA/SyncFree_csc/spts_syncfree_cuda2.h:89-102
+ * (CSC input, x output; validation = relative L1 error < 1e-4, :245-264) */
+int sptrsv_syncfree_hip_csc_d(const int *cscColPtr, const int *cscRowIdx,
+                              const double *cscVal, int m, int n, int nnz,
+                              double *x, const double *b, const double *x_ref,
+                              double *pre_time, double *solve_time,
+                              double *gflops, double *bandwidth);
+int sptrsv_syncfree_hip_csc_s(const int *cscColPtr, const int *cscRowIdx,
+                              const float *cscVal, int m, int n, int nnz,
+                              float *x, const float *b, const float *x_ref,
+                              double *pre_time, double *solve_time,
+                              double *gflops, double *bandwidth);
+
+/* ------------------------------------------------------- host helpers
+ * MI355X-build restatements of the reference's host preprocessing.  Arrays
+ * returned through T** are malloc'ed; release them with cap_free. */
+
+/* read_mtx, CUDA/ourWrtFst/read_mtx.h:6-191 (strict size line of
+ * CUDA/ourWrtFst/mmio.h:334-365).  Returns 0, -1 open, -2 banner,
+ * -3 complex, -4 size line, CAP_ERR_IO malformed entry. */
+int cap_read_mtx_d(const char *path, int *m, int *n, int *nnzA, int **row_ptr,
+                   int **col_idx, double **val);
+int cap_read_mtx_s(const char *path, int *m, int *n, int *nnzA, int **row_ptr,
+                   int **col_idx, float **val);
+
+/* change2tran, CUDA/ourWrtFst/read_mtx.h:194-257 (unit-valued strictly lower
+ * part of the leading m x m block + unit diagonal).  keep_values = 1 keeps
+ * A's values instead of 1 and sets the diagonal to 1 + sum|offdiag| (the
+ * opt-in real-valued mode; the reference always uses 1). */
+int cap_change2tran_d(int m, int nnzA, const int *rpA, const int *ciA,
+                      const double *vA, int keep_values, int *nnzL, int **rpL,
+                      int **ciL, double **vL);
+int cap_change2tran_s(int m, int nnzA, const int *rpA, const int *ciA,
+                      const float *vA, int keep_values, int *nnzL, int **rpL,
+                      int **ciL, float **vL);
+
+/* get_x_b, read_mtx.h:260-278 (x_ref = 1, b = L x_ref in CSR order) */
+void cap_get_x_b_d(int m, int n, const int *rp, const int *ci, const double *v,
+                   double *x_ref, double *b);
+void cap_get_x_b_s(int m, int n, const int *rp, const int *ci, const float *v,
+                   float *x_ref, float *b);
+
+/* matrix_layer, CUDA/ourWrtFst/tranpose.h:78-142 (1-based level count,
+ * parallelism = m / levels); row_level may be NULL. */
+int cap_matrix_layer(int m, const int *rp, const int *ci, int *levels,
+                     double *parallelism, int *row_level);
+
+/* matrix_warp4, CUDA/mix/tranpose.h:1423-1477 with chunk = rows per
+ * selector chunk (64 = one wavefront on gfx950; the reference's WARP_SIZE).
+ * warp_num must hold m+1 ints. */
+void cap_matrix_warp4(int m, int nnz, const int *rp, int border, int chunk,
+                      int *len, int *warp_num, double *warp_occupy,
+                      double *element_occupy);
+
+/* matrix_transposition, CUDA/ourWrtFst/tranpose.h:36-75 */
+void cap_transpose_d(int m, int n, int nnz, const int *rp, const int *ci,
+                     const double *v, int *csc_row, int *csc_colptr,
+                     double *csc_val);
+void cap_transpose_s(int m, int n, int nnz, const int *rp, const int *ci,
+                     const float *v, int *csc_row, int *csc_colptr,
+                     float *csc_val);
+
+/* Layout check of the solve contract above; returns CAP_OK or the first
+ * violation's code and (if bad_row != NULL) its row. */
+int cap_validate_csr_d(int n, int nnz, const int *rp, const int *ci,
+                       const double *v, int *bad_row);
+int cap_validate_csr_s(int n, int nnz, const int *rp, const int *ci,
+                       const float *v, int *bad_row);
+
+/* Synthetic generators for the benchmark workloads (splitmix64 seeded; the
+ * same seed gives the same matrix on every host).  kind:
+ *   0  band: row i gets min(k_max, i) distinct columns uniform in
+ *      [max(0, i-band), i) (k_max = offdiag), values U[-1,1],
+ *      diagonal 1 + sum|offdiag|
+ *   1  level: n = levels*width; a row of level l >= 1 gets 1..offdiag
+ *      distinct columns uniform over the rows of levels [max(0,l-depth), l)
+ *   2  banded Bernoulli: every column in [max(0,i-band), i) kept with
+ *      probability p
+ *   3  band with per-row nnz: row i gets min(U{0..offdiag-1}, i) distinct
+ *      columns in [max(0,i-band), i)
+ * Two passes: call with ci/v = NULL to get *nnz and row_ptr, then again
+ * with arrays of that size. */
+typedef struct cap_gen_params {
+    int kind, n, offdiag, band, levels, width, depth;
+    double p;
+    unsigned long long seed;
+} cap_gen_params;
+int cap_generate_d(const cap_gen_params *g, int *nnz, int *rp, int *ci, double *v);
+int cap_generate_s(const cap_gen_params *g, int *nnz, int *rp, int *ci, float *v);
+/* fills out[0..count) with U[lo,hi) doubles from splitmix64(seed) */
+void cap_uniform_d(unsigned long long seed, long long count, double lo, double hi, double *out);
+
+/* reference metric formulas, spts_syncfree_cuda_csr_wrt.h:206-215 */
+double cap_datasize(int m, int n, int nnzL, int value_bytes, int len_warp_num);
+
+/* device queries */
+int cap_device_count(void);
+const char *cap_strerror(int code);
+const char *cap_version(void);
+void cap_free(void *p);
+
+/* --------------------------------------------- VALUE_TYPE drop-in names */
+#ifndef VALUE_TYPE
+#define VALUE_TYPE double
+#endif
+static inline int spts_syncfree_hip_csr_wrt(const int *rp, const int *ci, const VALUE_TYPE *v,
+                                            int m, int n, int nnzL, double *t, double *gf,
+                                            double *bw)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? spts_syncfree_hip_csr_wrt_d(rp, ci, (const double *)(const void *)v, m, n, nnzL, t, gf, bw)
+               : spts_syncfree_hip_csr_wrt_s(rp, ci, (const float *)(const void *)v, m, n, nnzL, t, gf, bw);
+}
+static inline int spts_syncfree_hip_csr(const int *rp, const int *ci, const VALUE_TYPE *v,
+                                        int m, int n, int nnzL, double *t, double *gf,
+                                        double *bw)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? spts_syncfree_hip_csr_d(rp, ci, (const double *)(const void *)v, m, n, nnzL, t, gf, bw)
+               : spts_syncfree_hip_csr_s(rp, ci, (const float *)(const void *)v, m, n, nnzL, t, gf, bw);
+}
+static inline int spts_syncfree_hip_mix(const int *rp, const int *ci, const VALUE_TYPE *v,
+                                        int m, int n, int nnzL, double *t, double *gf,
+                                        double *bw, double *pre, double *wocc, double *eocc)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? spts_syncfree_hip_mix_d(rp, ci, (const double *)(const void *)v, m, n, nnzL, t, gf, bw, pre, wocc, eocc)
+               : spts_syncfree_hip_mix_s(rp, ci, (const float *)(const void *)v, m, n, nnzL, t, gf, bw, pre, wocc, eocc);
+}
+static inline int sptrsv_syncfree_hip_csc(const int *cp, const int *ri, const VALUE_TYPE *v,
+                                          int m, int n, int nnz, VALUE_TYPE *x,
+                                          const VALUE_TYPE *b, const VALUE_TYPE *xr,
+                                          double *pre, double *t, double *gf, double *bw)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? sptrsv_syncfree_hip_csc_d(cp, ri, (const double *)(const void *)v, m, n, nnz,
+                                           (double *)(void *)x, (const double *)(const void *)b,
+                                           (const double *)(const void *)xr, pre, t, gf, bw)
+               : sptrsv_syncfree_hip_csc_s(cp, ri, (const float *)(const void *)v, m, n, nnz,
+                                           (float *)(void *)x, (const float *)(const void *)b,
+                                           (const float *)(const void *)xr, pre, t, gf, bw);
+}
+
+#ifdef __cplusplus
+}
+#endif
+#endif

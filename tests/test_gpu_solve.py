@@ -1,0 +1,213 @@
+"""GPU parity of the forward solve kernels (through the C ABI) against the oracle.
+
+Bar: bit-exact wherever the kernel's accumulation order is the oracle's
+sequential fused order (thread-per-row paths in both readiness protocols,
+every right-hand-side count); FP64 within 1e-12 relative max-norm where the
+order differs (mix wavefront rows: lane-strided partial sums + butterfly;
+CSC: atomic scatter); FP32 within 1e-5 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from capellinisptrsv_amd import api
+from conftest import random_lower, rel_maxnorm
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0):
+    import torch
+    from capellinisptrsv_amd import Plan
+    plan = Plan(torch.from_numpy(rp).to(cuda), torch.from_numpy(ci).to(cuda), torch.from_numpy(v).to(cuda),
+                algo=algo, sync=sync, border=border)
+    bt = torch.from_numpy(np.ascontiguousarray(b.reshape(-1, k) if k > 1 else b)).to(cuda)
+    x = plan.solve(bt)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    info = plan.info()
+    out = x.cpu().numpy().reshape(-1)
+    plan.close()
+    return out, info
+
+
+def _matrices():
+    yield "band", random_lower(3000, max_off=5, band=40, seed=1)
+    yield "dup", random_lower(2000, max_off=6, band=30, seed=2, dup=True)
+    yield "cfg0", api.make_config(0, seed=0)
+    yield "cfg1_small", api.make_config(1, seed=0, scale=0.05)  # 10 levels x 10,000 rows
+    yield "cfg2_small", api.make_config(2, seed=0, scale=0.02)  # n = 10,000, ~38 nnz/row
+    yield "chain", (np.r_[0, 1, np.arange(3, 2 * 4000, 2)].astype(np.int32),
+                    np.r_[0, np.c_[np.arange(0, 3999), np.arange(1, 4000)].reshape(-1)].astype(np.int32),
+                    np.r_[1.0, np.tile([0.5, 2.0], 3999)])
+
+
+MATS = list(_matrices())
+
+
+@pytest.mark.parametrize("algo,sync", [("wrtfst", "value"), ("wrtfst", "flag"), ("twopart", "value"),
+                                       ("twopart", "flag")])
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, name, mat):
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(7, n)
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    x, info = _solve(cuda, rp, ci, v, b, algo, sync)
+    assert info["algo"] == algo and info["sync"] == sync
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
+    # and within tolerance of the reference's unfused serial arithmetic
+    assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=False)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("algo,sync", [("wrtfst", "value"), ("wrtfst", "flag"), ("twopart", "value")])
+@pytest.mark.parametrize("name,mat", MATS[:4], ids=[m[0] for m in MATS[:4]])
+def test_fp32_bit_exact_and_within_1e5(cuda, algo, sync, name, mat):
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(11, n)
+    v32, b32 = v.astype(np.float32), b.astype(np.float32)
+    x, _ = _solve(cuda, rp, ci, v32, b32, algo, sync)
+    ref32 = oracle.csr_forward(rp, ci, v32, b32, fused=True)
+    assert np.array_equal(x.view(np.uint32), ref32.view(np.uint32))
+    assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=True)) <= FP32_TOL
+
+
+@pytest.mark.parametrize("border", [4, 16, 10**6])
+@pytest.mark.parametrize("sync", ["value", "flag"])
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_mix(cuda, border, sync, name, mat):
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(3, n)
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    x, info = _solve(cuda, rp, ci, v, b, "mix", sync, border=border)
+    ln, wn, wo, eo = oracle.matrix_warp4(n, len(ci), rp, border, 64)
+    assert info["mix_len"] == ln and abs(info["warp_occupy"] - wo) < 1e-15
+    assert rel_maxnorm(x, ref) <= FP64_TOL
+    if border >= 10**6:  # every chunk thread-per-row: sequential order, bit-exact
+        assert wo == 0.0
+        assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_csc_syncfree(cuda, name, mat):
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(5, n)
+    x, info = _solve(cuda, rp, ci, v, b, "csc", "value")
+    assert info["algo"] == "csc"
+    assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=False)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("k", [2, 3, 8, 16, 32, 64])
+@pytest.mark.parametrize("sync", ["value", "flag"])
+def test_multi_rhs_bit_exact(cuda, k, sync):
+    rp, ci, v = api.make_config(0, seed=3, scale=0.2)
+    n = len(rp) - 1
+    b = api.uniform(100 + k, n * k)
+    ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
+    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", sync, k=k)
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+def test_unit_lower_known_answer(cuda):
+    """change2tran's all-ones L with b = L*1 has the exact solution x = 1 (read_mtx.h:260-278)."""
+    rng = np.random.default_rng(4)
+    n = 20000
+    rows, cols = [], []
+    for i in range(n):
+        for c in rng.integers(0, n, size=5):
+            rows.append(i)
+            cols.append(int(c))
+        rows.append(i)
+        cols.append(i)
+    order = np.lexsort((np.arange(len(rows)), rows))
+    rpA = np.searchsorted(np.array(rows)[order], np.arange(n + 1)).astype(np.int32)
+    ciA = np.array(cols, np.int32)[order]
+    vA = np.ones(len(ciA))
+    nnzL, rp, ci, v = api.change2tran(n, len(ciA), rpA, ciA, vA)
+    x_ref, b = api.get_x_b(n, n, rp, ci, v)
+    for algo in ("wrtfst", "twopart", "mix", "csc"):
+        x, _ = _solve(cuda, rp, ci, v, b, algo, "value", border=4)
+        assert np.array_equal(x, x_ref), algo
+
+
+def test_diagonal_and_tiny(cuda):
+    for n in (1, 2, 63, 64, 65):
+        rp = np.arange(n + 1, dtype=np.int32)
+        ci = np.arange(n, dtype=np.int32)
+        v = np.linspace(1, 2, n)
+        b = np.linspace(-1, 1, n)
+        for algo in ("wrtfst", "twopart", "mix", "csc"):
+            x, _ = _solve(cuda, rp, ci, v, b, algo, "value")
+            assert np.array_equal(x, b / v), (n, algo)
+
+
+def test_unsorted_offdiagonals_follow_stored_order(cuda):
+    rp, ci, v = random_lower(3000, max_off=6, band=100, seed=9, shuffle=True)
+    b = api.uniform(1, len(rp) - 1)
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", "value")
+    assert np.array_equal(x, ref)
+
+
+def test_nan_input_does_not_hang(cuda):
+    """A NaN in b is published as an ordinary (quiet) NaN; dependents finish."""
+    rp, ci, v = random_lower(5000, max_off=3, band=20, seed=5)
+    b = api.uniform(2, len(rp) - 1)
+    b[100] = np.nan
+    b[200] = np.frombuffer(np.uint64(0xFFF0000000000001).tobytes(), np.float64)[0]  # the sentinel's bits
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", "value")
+    assert np.array_equal(np.isnan(x), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.array_equal(x[ok], ref[ok])
+
+
+def test_invalid_inputs_rejected_before_launch(cuda):
+    import torch
+    from capellinisptrsv_amd import Plan, CapError
+    rp, ci, v = random_lower(500, seed=6)
+    r = next(i for i in range(40, 500) if rp[i + 1] - rp[i] >= 2)  # a row with an off-diagonal
+    upper = ci.copy()
+    upper[rp[r]] = r + 3
+    no_diag = ci.copy()
+    no_diag[rp[r + 1] - 1] = 0
+    zero_diag = v.copy()
+    zero_diag[rp[r + 1] - 1] = 0.0
+    bad_rp = rp.copy()
+    bad_rp[r + 1] = bad_rp[r] - 1  # row r ends before it starts
+    cases = {"upper": (rp, upper, v, -4), "no_diag": (rp, no_diag, v, -5), "zero_diag": (rp, ci, zero_diag, -6),
+             "rowptr": (bad_rp, ci, v, -3)}
+    for name, (rr, c, vv, code) in cases.items():
+        with pytest.raises(CapError) as e:
+            Plan(torch.from_numpy(rr).to(cuda), torch.from_numpy(c).to(cuda), torch.from_numpy(vv).to(cuda),
+                 algo="wrtfst")
+        assert e.value.code == code, name
+
+
+def test_watchdog_forced_from_host(cuda):
+    """A 1-poll spin limit makes waits give up: rows publish NaN, the grid drains, status reports it."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = api.make_config(1, seed=0, scale=0.02)
+    d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+    plan = Plan(*d, algo="wrtfst", sync="value", spin_limit=1)
+    b = torch.ones(len(rp) - 1, dtype=torch.float64, device=cuda)
+    x = plan.solve(b)
+    torch.cuda.synchronize()
+    assert plan.status() == -13
+    assert plan.status() == 0  # cleared
+    assert torch.isnan(x).any()
+
+
+def test_autotuner_picks_a_correct_path(cuda):
+    rp, ci, v = api.make_config(2, seed=1, scale=0.01)
+    b = api.uniform(9, len(rp) - 1)
+    x, info = _solve(cuda, rp, ci, v, b, "auto", "auto")
+    assert info["algo"] in ("wrtfst", "mix", "twopart")
+    assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=True)) <= FP64_TOL
