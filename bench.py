@@ -1,0 +1,206 @@
+"""Benchmark: sync-free lower-triangular solve on MI355X.
+
+Workload (one "step"): the FP64 forward solve L x = b of the wide-level
+matrix -- n = 2,000,000 rows in 200 levels of 10,000, each row 1-2
+off-diagonals drawn from the rows of the previous 1-3 levels, off-diagonals
+U[-1,1], diagonal 1 + sum|offdiag|, b U[-1,1] -- with the matrix, b and x
+resident in HBM.  A step is the per-solve reset plus the solve kernel.
+
+Prints one JSON line:
+  value        GFLOPS = 2 * nnzL / step time (nnzL includes the diagonal)
+  roofline     solve kernel alone: algorithmic bytes / mean kernel time vs
+               8 TB/s; traffic from the committed rocprofv3 counter summary
+  cpu_baseline the reference's serial executor (serialref.h, compiled from
+               its own sources into oracle/_ref) on one host core, same matrix
+  extras       FP32 solve, FP32 solve + FP64 refinement to 1e-12 (error vs
+               the CPU FP64 result), MALL-flushed (cold) step time
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SpTRSV GFLOPS + HBM GB/s; FP32+refinement vs FP64 solve at 1e-12 rel. error"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("wide-level lower-triangular CSR, n=2,000,000 (200 levels x 10,000 rows, 1-2 off-diagonals "
+            "uniform over the previous 1-3 levels), FP64 forward solve, single RHS")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solve is a single-GPU kernel and this build runs on one MI355X "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from capellinisptrsv_amd import Plan, api
+
+    dev = torch.device("cuda:0")
+    log("generating the wide-level matrix")
+    rp, ci, v = api.make_config(1, seed=0)
+    n, nnz = len(rp) - 1, len(ci)
+    levels = api.matrix_layer(n, n, nnz, rp, ci, v)[0]
+    b_h = api.uniform(1, n)
+    d_rp, d_ci, d_v = (torch.from_numpy(a).to(dev) for a in (rp, ci, v))
+    b = torch.from_numpy(b_h).to(dev)
+    x = torch.empty_like(b)
+
+    plan = Plan(d_rp, d_ci, d_v)  # autotuned algorithm / protocol / grid
+    info = plan.info()
+    log(f"plan: {info}")
+
+    def step():
+        plan.solve(b, out=x)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    assert plan.status() == 0, "a dependency wait gave up"
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    x64 = x.clone()
+
+    # the solve kernel alone, HIP events on the stream it is launched on
+    stream = torch.cuda.current_stream()
+    kms = []
+    for _ in range(max(args.steps, 20)):
+        plan.reset(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.launch(b, x)
+        e1.record(stream)
+        e1.synchronize()
+        kms.append(e0.elapsed_time(e1))
+    kernel_ms = float(np.mean(kms))
+    assert torch.equal(x, x64)
+
+    # cold: MALL (256 MB) flushed before every step, each step timed alone
+    flush = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    cold = []
+    for _ in range(10):
+        flush.fill_(1.0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        e1.synchronize()
+        cold.append(e0.elapsed_time(e1))
+    del flush
+
+    # FP32 solve, and FP32 solves inside FP64 refinement
+    plan32 = Plan(d_rp, d_ci, d_v.float(), algo=info["algo"] if info["algo"] != "csc" else "wrtfst")
+    b32, x32 = b.float(), torch.empty(n, dtype=torch.float32, device=dev)
+    for _ in range(args.warmup):
+        plan32.solve(b32, out=x32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan32.solve(b32, out=x32)
+    torch.cuda.synchronize()
+    fp32_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    xr = torch.empty_like(b)
+    for _ in range(3):
+        _, iters, rel = plan.solve_refined(b, tol=1e-12, out=xr)
+    torch.cuda.synchronize()
+    reps = max(5, args.steps // 5)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, iters, rel = plan.solve_refined(b, tol=1e-12, out=xr)
+    torch.cuda.synchronize()
+    refined_ms = (time.perf_counter() - t0) * 1e3 / reps
+
+    # algorithmic bytes of one solve: row_ptr, col_idx, values, b read once, x written once
+    vb = 8
+    alg_bytes = 4 * (n + 1) + (4 + vb) * nnz + 2 * vb * n
+    ref_datasize = api.datasize(n, n, nnz, vb)  # the reference's formula (includes its int flag array)
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_wrtfst_wide_level_2M.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("algo") == info["algo"] and t.get("n") == n:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": round(2.0 * nnz / (ms_per_step * 1e6), 3), "unit": "GFLOPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOAD, "n": n, "nnzL": nnz, "levels": levels, "algo": info["algo"],
+                   "sync": info["sync"], "seed": 0},
+        "gbs_algorithmic": round(alg_bytes / (ms_per_step * 1e-3) / 1e9, 2),
+        "gbs_reference_formula": round(ref_datasize / (ms_per_step * 1e-3) / 1e9, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "k_wrtfst" if info["algo"] == "wrtfst" else "k_" + info["algo"],
+                     "kernel_ms": round(kernel_ms, 5), "algorithmic_bytes": alg_bytes},
+        "per_level_ns": round(kernel_ms * 1e6 / levels, 1),
+        "cold_ms_per_step": round(float(np.median(cold)), 5),
+        "fp32_ms_per_step": round(fp32_ms, 5),
+        "refined_ms_per_solve": round(refined_ms, 5), "refined_iterations": iters,
+        "refined_over_fp64": round(refined_ms / ms_per_step, 3),
+    }
+
+    # CPU baseline: the reference's serial executor on the same matrix (1 core)
+    xc = None
+    if not args.no_cpu:
+        import oracle
+        cr, cp, cv = oracle.transpose(n, n, nnz, rp, ci, v)
+        indeg = oracle.csc_analyser(cr, n, nnz)
+        if oracle.ref_available("d64"):
+            kind = "reference"
+
+            def cpu_solve():
+                return oracle.ref_csc_executor(cp, cr, cv, indeg, n, n, 0, 1, b_h)
+        else:
+            kind = "port"
+
+            def cpu_solve():
+                return oracle.csc_executor(cp, cr, cv, indeg, n, n, 0, 1, b_h)
+        xc = cpu_solve()
+        times, start = [], time.perf_counter()
+        while time.perf_counter() - start < args.cpu_seconds and len(times) < 1000:
+            t = time.perf_counter()
+            cpu_solve()
+            times.append(time.perf_counter() - t)
+        cpu_ms = float(np.median(times)) * 1e3
+        out["cpu_baseline"] = {"value": round(2.0 * nnz / (cpu_ms * 1e6), 4), "unit": "GFLOPS", "cores": 1,
+                               "kind": kind, "ms_per_solve": round(cpu_ms, 3),
+                               "sample": f"{len(times)} full solves of the same n=2,000,000 matrix (median), "
+                                         f"serial column-driven executor, host {os.cpu_count()} cores visible"}
+        ref = xc
+    else:
+        import oracle
+        ref = oracle.csr_forward(rp, ci, v, b_h, fused=False)
+    den = float(np.max(np.abs(ref)))
+    out["fp64_err_vs_cpu"] = float(np.max(np.abs(x64.cpu().numpy() - ref)) / den)
+    out["refined_err_vs_cpu_fp64"] = float(np.max(np.abs(xr.cpu().numpy() - ref)) / den)
+    out["fp32_err_vs_cpu_fp64"] = float(np.max(np.abs(x32.double().cpu().numpy() - ref)) / den)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

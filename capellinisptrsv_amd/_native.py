@@ -46,7 +46,8 @@ class CapError(RuntimeError):
 
 class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
-                ("host_inputs", c_int), ("spin_limit", c_int), ("reserved", c_int * 7)]
+                ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
+                ("reserved", c_int * 6)]
 
 
 class cap_gen_params(Structure):
@@ -68,6 +69,8 @@ SIGNATURES = {
                                 POINTER(cap_opts), c_void_p]),
     "cap_plan_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_status": (c_int, [c_void_p, c_void_p]),
+    "cap_plan_reset": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

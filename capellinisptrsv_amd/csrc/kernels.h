@@ -1,11 +1,14 @@
 // Device kernels of the gfx950 sync-free lower-triangular solve.
 //
 // Execution model shared by every solve kernel:
-//  * Work is handed out by an atomic ticket, one atomicAdd per 64-lane
-//    wavefront (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:24 takes one per
-//    thread).  Tickets are issued in dispatch order, so every row a wave
-//    depends on belongs to a wave that is already resident or finished; that
-//    is what makes spinning safe without a grid barrier.
+//  * Work is handed out by an atomic ticket, one atomicAdd per workgroup
+//    (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:24 takes one per thread; a
+//    single counter serialises at ~11 ns per increment on MI355X, so per-thread
+//    or per-wavefront tickets alone would cost 0.1-0.4 ms on 2M rows).  The
+//    grid is persistent: a workgroup takes tickets until the rows run out.
+//    Tickets are only taken by running workgroups, so every row a workgroup
+//    waits for belongs to a workgroup that is running or finished; that is
+//    what makes spinning safe without a grid barrier.
 //  * Readiness of x[c] is observed either through the value itself (SYNC
 //    VALUE: x starts as a signalling-NaN sentinel that arithmetic can never
 //    produce; a relaxed agent-scope 32/64-bit atomic load both tests and
@@ -87,12 +90,21 @@ __device__ __forceinline__ bool give_up(const SyncCtx &sc, int idle)
     return false;
 }
 
-// One ticket per wavefront, broadcast to all lanes.
-__device__ __forceinline__ int wave_ticket(unsigned *ticket, unsigned amount)
+// Persistent ticket loop: the workgroup takes `per_block` work items at a time
+// (one atomicAdd, broadcast through LDS) and calls body(base) until `total`
+// items are handed out.  body must not contain workgroup barriers.
+template <typename F>
+__device__ __forceinline__ void ticket_loop(unsigned *ticket, int per_block, int total, F body)
 {
-    unsigned t = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(ticket, amount);
-    return (int)__shfl(t, 0);
+    __shared__ int s_base;
+    while (true) {
+        if (threadIdx.x == 0) s_base = (int)atomicAdd(ticket, (unsigned)per_block);
+        __syncthreads();
+        const int base = s_base;
+        __syncthreads();
+        if (base >= total) return;
+        body(base);
+    }
 }
 
 // Publish x[i] in the value protocol: the sentinel can never be published
@@ -262,21 +274,21 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 }
 
 // ourWrtFst (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:10-57): one lane
-// per row and right-hand side; a wavefront holds 64 >> lpr_log2 rows and
-// 1 << lpr_log2 lanes per row, lane q of a row owning right-hand side q
-// (lanes with q >= k idle).
+// per row and right-hand side; a workgroup ticket covers kBlock >> lpr_log2
+// rows with 1 << lpr_log2 lanes per row, lane q of a row owning right-hand
+// side q (lanes with q >= k idle).
 template <typename T, int SYNC>
 __global__ void __launch_bounds__(kBlock)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
              unsigned *__restrict__ ticket)
 {
-    const int base = wave_ticket(ticket, (unsigned)(kWave >> lpr_log2));
-    const int lane = threadIdx.x & (kWave - 1);
-    const int i = base + (lane >> lpr_log2);
-    const int q = lane & ((1 << lpr_log2) - 1);
-    if (i >= n || q >= k) return;
-    wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0);
+    const int q = threadIdx.x & ((1 << lpr_log2) - 1);
+    const int r = threadIdx.x >> lpr_log2;
+    ticket_loop(ticket, kBlock >> lpr_log2, n, [&](int base) {
+        const int i = base + r;
+        if (i < n && q < k) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0);
+    });
 }
 
 // our2Part (CUDA/our2Part/spts_syncfree_cuda_csr.h:10-124): dependencies on
@@ -287,18 +299,20 @@ __global__ void __launch_bounds__(kBlock)
     k_twopart(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
               const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, unsigned *__restrict__ ticket)
 {
-    const int base = wave_ticket(ticket, kWave);
-    const int i = base + (threadIdx.x & (kWave - 1));
-    if (i >= n) return;
-    int j = rp[i];
-    const int d = rp[i + 1] - 1;
-    T s = (T)0;
-    for (; j < d; ++j) {
-        const int c = ci[j];
-        if (c >= base) break;
-        s = fmac(val[j], wait_value<T, SYNC>(x, sc, c), s);
-    }
-    wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, j, d, 1, 0, s);
+    ticket_loop(ticket, kBlock, n, [&](int base) {
+        const int wave_begin = base + (threadIdx.x & ~(kWave - 1));
+        const int i = base + threadIdx.x;
+        if (i >= n) return;
+        int j = rp[i];
+        const int d = rp[i + 1] - 1;
+        T s = (T)0;
+        for (; j < d; ++j) {  // rows of earlier wavefronts: blocking waits
+            const int c = ci[j];
+            if (c >= wave_begin) break;
+            s = fmac(val[j], wait_value<T, SYNC>(x, sc, c), s);
+        }
+        wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, j, d, 1, 0, s);
+    });
 }
 
 // mix (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:15-137): unit u covers rows
@@ -312,23 +326,24 @@ __global__ void __launch_bounds__(kBlock)
           const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, const int *__restrict__ unit, int n_units,
           unsigned *__restrict__ ticket)
 {
-    const int u = wave_ticket(ticket, 1);
-    if (u >= n_units) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int r0 = unit[u], r1 = unit[u + 1];
-    if (r1 - r0 > 1) {
-        const int i = r0 + lane;
-        if (i >= r1) return;
-        wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0);
-        return;
-    }
-    const int row = r0;
-    const int d = rp[row + 1] - 1;
-    T s = (T)0;
-    for (int j = rp[row] + lane; j < d; j += kWave) s = fmac(val[j], wait_value<T, SYNC>(x, sc, ci[j]), s);
+    ticket_loop(ticket, kBlock / kWave, n_units, [&](int base) {
+        const int u = base + (int)(threadIdx.x / kWave);
+        if (u >= n_units) return;
+        const int r0 = unit[u], r1 = unit[u + 1];
+        if (r1 - r0 > 1) {
+            const int i = r0 + lane;
+            if (i < r1) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0);
+            return;
+        }
+        const int row = r0;
+        const int d = rp[row + 1] - 1;
+        T s = (T)0;
+        for (int j = rp[row] + lane; j < d; j += kWave) s = fmac(val[j], wait_value<T, SYNC>(x, sc, ci[j]), s);
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) publish<T, SYNC>(x, sc, row, (b[row] - s) / val[d]);
+        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) publish<T, SYNC>(x, sc, row, (b[row] - s) / val[d]);
+    });
 }
 
 // ------------------------------------------------ SyncFree CSC (K4 / K5)
@@ -351,27 +366,29 @@ __global__ void __launch_bounds__(kBlock)
           int backward, SyncCtx sc, unsigned *__restrict__ ticket)
 {
     typedef typename VBits<T>::U U;
-    int col = wave_ticket(ticket, 1);
-    if (col >= m) return;
     const int lane = threadIdx.x & (kWave - 1);
-    if (backward) col = m - 1 - col;
-    const int dpos = backward ? cp[col + 1] - 1 : cp[col];
-    int idle = 0;
-    bool failed = false;
-    while (ld_agent(deg + col) != 1)
-        if (give_up(sc, ++idle)) {
-            failed = true;
-            break;
-        }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
-    const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
-    const int start = backward ? cp[col] : cp[col] + 1;
-    const int stop = backward ? cp[col + 1] - 1 : cp[col + 1];
-    for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(left_sum + ri[j], xi * cv[j]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(deg + ri[j], -1);
-    if (lane == 0) x[col] = xi;
+    ticket_loop(ticket, kBlock / kWave, m, [&](int base) {
+        int col = base + (int)(threadIdx.x / kWave);
+        if (col >= m) return;
+        if (backward) col = m - 1 - col;
+        const int dpos = backward ? cp[col + 1] - 1 : cp[col];
+        int idle = 0;
+        bool failed = false;
+        while (ld_agent(deg + col) != 1)
+            if (give_up(sc, ++idle)) {
+                failed = true;
+                break;
+            }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
+        const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
+        const int start = backward ? cp[col] : cp[col] + 1;
+        const int stop = backward ? cp[col + 1] - 1 : cp[col + 1];
+        for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(left_sum + ri[j], xi * cv[j]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(deg + ri[j], -1);
+        if (lane == 0) x[col] = xi;
+    });
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -387,32 +404,33 @@ __global__ void __launch_bounds__(kBlock)
               double *__restrict__ left_sum, double *__restrict__ grad_b, double *__restrict__ grad_val, int n,
               SyncCtx sc, unsigned *__restrict__ ticket)
 {
-    const int base = wave_ticket(ticket, kWave);
-    const int t = base + (threadIdx.x & (kWave - 1));
-    if (t >= n) return;
-    const int i = n - 1 - t;
-    const int a = rp[i], d = rp[i + 1] - 1;
-    int idle = 0;
-    bool done = false;
-    do {  // wave-uniform exit, as in wrtfst_row
-        if (!done) {
-            const bool ready = ld_agent(cnt + i) == 0;
-            const bool failed = !ready && give_up(sc, ++idle);
-            if (ready || failed) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const double ls = as_val(ld_agent(reinterpret_cast<const unsigned long long *>(left_sum) + i));
-                const double y = failed ? as_val(VBits<double>::kQNaN) : (g[i] - ls) / val[d];
-                grad_b[i] = y;
-                for (int j = a; j < d; ++j) atomic_add_agent(left_sum + ci[j], y * val[j]);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                for (int j = a; j < d; ++j) atomic_add_agent(cnt + ci[j], -1);
-                if (grad_val)
-                    for (int j = a; j <= d; ++j) grad_val[j] = -y * xfwd[ci[j]];
-                asm volatile("" ::: "memory");
-                done = true;
+    ticket_loop(ticket, kBlock, n, [&](int base) {
+        const int t = base + threadIdx.x;
+        if (t >= n) return;
+        const int i = n - 1 - t;
+        const int a = rp[i], d = rp[i + 1] - 1;
+        int idle = 0;
+        bool done = false;
+        do {  // wave-uniform exit, as in wrtfst_row
+            if (!done) {
+                const bool ready = ld_agent(cnt + i) == 0;
+                const bool failed = !ready && give_up(sc, ++idle);
+                if (ready || failed) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const double ls = as_val(ld_agent(reinterpret_cast<const unsigned long long *>(left_sum) + i));
+                    const double y = failed ? as_val(VBits<double>::kQNaN) : (g[i] - ls) / val[d];
+                    grad_b[i] = y;
+                    for (int j = a; j < d; ++j) atomic_add_agent(left_sum + ci[j], y * val[j]);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    for (int j = a; j < d; ++j) atomic_add_agent(cnt + ci[j], -1);
+                    if (grad_val)
+                        for (int j = a; j <= d; ++j) grad_val[j] = -y * xfwd[ci[j]];
+                    asm volatile("" ::: "memory");
+                    done = true;
+                }
             }
-        }
-    } while (__any(!done));
+        } while (__any(!done));
+    });
 }
 
 // off-diagonal column counts of L (the adjoint's in-degrees)

@@ -53,9 +53,13 @@ struct cap_plan_s {
     int *flags = nullptr;
     long long flags_len = 0;
     int epoch = 0;
+    // control block (one 4 KiB allocation): tickets at +0, watchdog word at
+    // +256, refinement norms at +512
+    unsigned char *ctrl = nullptr;
     unsigned *ticket = nullptr;
     int *err = nullptr;
     int spin_limit = kSpinLimitDefault;
+    int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
     // mix
     int *unit = nullptr;
     int n_units = 0, mix_len = 0;
@@ -69,7 +73,7 @@ struct cap_plan_s {
     // refinement
     float *val32 = nullptr, *r32 = nullptr, *d32 = nullptr;
     long long refine_len = 0;
-    unsigned long long *norms = nullptr;
+    unsigned long long *norms = nullptr;  // inside ctrl
     hipGraphExec_t refine_graph = nullptr;
     const void *graph_b = nullptr;
     void *graph_x = nullptr;
@@ -131,12 +135,34 @@ void plan_free(cap_plan p)
         (void)hipFree(p->ci);
         (void)hipFree(p->val);
     }
-    void *bufs[] = {p->flags, p->ticket, p->err, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
-                    p->cnt0, p->cnt, p->adj_ls, p->val32, p->r32, p->d32, p->norms};
+    void *bufs[] = {p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
+                    p->cnt0, p->cnt, p->adj_ls, p->val32, p->r32, p->d32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->refine_graph) (void)hipGraphExecDestroy(p->refine_graph);
     delete p;
+}
+
+// Control block and device shape (compute units for the persistent grid).
+int init_ctrl(cap_plan p)
+{
+    CAP_TRY(dev_alloc(&p->ctrl, 4096));
+    CAP_HIP_TRY(hipMemset(p->ctrl, 0, 4096));
+    p->ticket = reinterpret_cast<unsigned *>(p->ctrl);
+    p->err = reinterpret_cast<int *>(p->ctrl + 256);
+    p->norms = reinterpret_cast<unsigned long long *>(p->ctrl + 512);
+    int ncu = 0;
+    CAP_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
+    p->ncu = ncu > 0 ? ncu : 256;
+    return CAP_OK;
+}
+
+// Persistent grid: never more workgroups than the work needs, never more
+// than bpc per compute unit (more resident pollers only add memory traffic).
+inline unsigned grid_for(const cap_plan p, long long work_blocks)
+{
+    const long long cap = (long long)p->ncu * p->bpc;
+    return (unsigned)std::max(1LL, std::min(work_blocks, cap));
 }
 
 int ensure_flags(cap_plan p, long long len, hipStream_t st)
@@ -183,29 +209,29 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     if (algo == CAP_ALGO_WRTFST) {
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
-        const long long waves = ((long long)n + (kWave >> l) - 1) / (kWave >> l);
-        const unsigned nb = blocks_for(waves * kWave);
+        const int rows_per_block = kBlock >> l;
+        const unsigned nb = grid_for(p, ((long long)n + rows_per_block - 1) / rows_per_block);
         if (sync == CAP_SYNC_VALUE)
             k_wrtfst<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
         else
             k_wrtfst<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
     } else if (algo == CAP_ALGO_TWOPART) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
-        const unsigned nb = blocks_for(((long long)n + kWave - 1) / kWave * kWave);
+        const unsigned nb = grid_for(p, ((long long)n + kBlock - 1) / kBlock);
         if (sync == CAP_SYNC_VALUE)
             k_twopart<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, p->ticket);
         else
             k_twopart<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, p->ticket);
     } else if (algo == CAP_ALGO_MIX) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
-        const unsigned nb = blocks_for((long long)p->n_units * kWave);
+        const unsigned nb = grid_for(p, ((long long)p->n_units * kWave + kBlock - 1) / kBlock);
         if (sync == CAP_SYNC_VALUE)
             k_mix<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
         else
             k_mix<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
-        const unsigned nb = blocks_for((long long)n * kWave);
+        const unsigned nb = grid_for(p, ((long long)n * kWave + kBlock - 1) / kBlock);
         k_csc<T><<<nb, kBlock, 0, st>>>(p->cp, p->cri, (const T *)p->cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc,
                                         p->ticket);
     } else {
@@ -235,6 +261,13 @@ int solve_t(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hip
         CAP_TRY(launch_prepare<T>(p, sync, x, (long long)p->n * rhs, st));
     }
     return launch_kernel<T>(p, algo, sync, (const T *)p->val, (const T *)b, (T *)x, rhs, st);
+}
+
+template <typename T>
+int reset_t(cap_plan p, int algo, int sync, void *x, int rhs, hipStream_t st)
+{
+    if (algo == CAP_ALGO_CSC) return launch_csc_reset<T>(p, st);
+    return launch_prepare<T>(p, sync, x, (long long)p->n * rhs, st);
 }
 
 int plan_solve_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
@@ -456,13 +489,9 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
                 if (rc != CAP_OK) break;
             }
         }
-        if ((rc = dev_alloc(&p->ticket, kTickets)) != CAP_OK) break;
-        if ((rc = dev_alloc(&p->err, 4)) != CAP_OK) break;
-        if (hipMemset(p->err, 0, 16) != hipSuccess || hipMemset(p->ticket, 0, 16) != hipSuccess) {
-            rc = CAP_ERR_HIP;
-            break;
-        }
+        if ((rc = init_ctrl(p)) != CAP_OK) break;
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
+        if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         p->algo = o.algo;
         p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
         const int border = o.border > 0 ? o.border : kMixBorderDefault;
@@ -516,6 +545,23 @@ int cap_plan_solve(cap_plan p, const void *b, void *x, int rhs, void *stream)
     return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
 }
 
+int cap_plan_reset(cap_plan p, void *x, int rhs, void *stream)
+{
+    if (!p || !x || rhs < 1) return CAP_ERR_ARG;
+    return p->vt == CAP_F64 ? reset_t<double>(p, p->algo, p->sync, x, rhs, S(stream))
+                            : reset_t<float>(p, p->algo, p->sync, x, rhs, S(stream));
+}
+
+int cap_plan_launch(cap_plan p, const void *b, void *x, int rhs, void *stream)
+{
+    if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
+    if (p->vt == CAP_F64)
+        return launch_kernel<double>(p, p->algo, p->sync, (const double *)p->val, (const double *)b, (double *)x, rhs,
+                                     S(stream));
+    return launch_kernel<float>(p, p->algo, p->sync, (const float *)p->val, (const float *)b, (float *)x, rhs,
+                                S(stream));
+}
+
 int cap_plan_info(cap_plan p, int *algo, int *sync, int *border, int *mix_len, double *wocc, double *eocc)
 {
     if (!p) return CAP_ERR_ARG;
@@ -567,7 +613,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
                                                                     p->ticket, kTickets);
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
-    k_adjoint<<<blocks_for(((long long)n + kWave - 1) / kWave * kWave), kBlock, 0, st>>>(
+    k_adjoint<<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
         p->rp, p->ci, (const double *)p->val, (const double *)g, (const double *)x, p->cnt, p->adj_ls,
         (double *)grad_b, (double *)grad_val, n, sc, p->ticket);
     CAP_HIP_TRY(hipGetLastError());
@@ -586,7 +632,6 @@ int ensure_refine(cap_plan p, int rhs, hipStream_t st)
         CAP_TRY(dev_alloc(&p->val32, (size_t)p->nnz));
         k_cast_f32<<<blocks_for(p->nnz), kBlock, 0, st>>>((const double *)p->val, p->val32, p->nnz);
         CAP_HIP_TRY(hipGetLastError());
-        CAP_TRY(dev_alloc(&p->norms, 4));
     }
     if (p->refine_len < len) {
         if (p->r32) CAP_HIP_TRY(hipFree(p->r32));
@@ -926,6 +971,7 @@ int bench_csc(const int *cp, const int *ri, const T *cv, int m, int n, int nnz, 
     int dev = 0;
     CAP_TRY(check_device(&dev));
     cap_plan p = new cap_plan_s();
+    p->device = dev;
     p->n = n;
     p->nnz = nnz;
     p->vt = sizeof(T) == 8 ? CAP_F64 : CAP_F32;
@@ -936,14 +982,12 @@ int bench_csc(const int *cp, const int *ri, const T *cv, int m, int n, int nnz, 
     T *db = nullptr, *dx = nullptr;
     double analyse_ms = 0.0, total_ms = 0.0;
     do {
-        if ((rc = dev_alloc(&p->ticket, kTickets)) || (rc = dev_alloc(&p->err, 4)) ||
-            (rc = dev_alloc(&p->cp, (size_t)n + 1)) || (rc = dev_alloc(&p->cri, (size_t)nnz)) ||
+        if ((rc = init_ctrl(p)) || (rc = dev_alloc(&p->cp, (size_t)n + 1)) || (rc = dev_alloc(&p->cri, (size_t)nnz)) ||
             (rc = dev_alloc((T **)&p->cv, (size_t)nnz)) || (rc = dev_alloc(&p->deg0, (size_t)n)) ||
             (rc = dev_alloc(&p->deg, (size_t)n)) || (rc = dev_alloc((T **)&p->left_sum, (size_t)n)) ||
             (rc = dev_alloc(&db, (size_t)n)) || (rc = dev_alloc(&dx, (size_t)n)))
             break;
-        if (hipMemset(p->err, 0, 16) != hipSuccess ||
-            hipMemcpy(p->cp, cp, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        if (hipMemcpy(p->cp, cp, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->cri, ri, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(p->cv, cv, sizeof(T) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(db, b, sizeof(T) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||

@@ -41,7 +41,8 @@ class Plan:
     """
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
-                 sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0, stream=None):
+                 sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
+                 blocks_per_cu: int = 0, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -52,7 +53,7 @@ class Plan:
         self.nnz = col_idx.numel()
         self.dtype = values.dtype
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
-                              spin_limit=spin_limit)
+                              spin_limit=spin_limit, blocks_per_cu=blocks_per_cu)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -82,6 +83,17 @@ class Plan:
             raise ValueError("b has the wrong number of rows")
         x = out if out is not None else torch.empty_like(b)
         N.check(N.lib().cap_plan_solve(self._h, _ptr(b), _ptr(x), k, _stream(stream)), "cap_plan_solve")
+        return x
+
+    def reset(self, x: torch.Tensor, stream=None):
+        """First half of solve(): per-solve reset of x / tickets (asynchronous)."""
+        k = 1 if x.dim() == 1 else x.shape[1]
+        N.check(N.lib().cap_plan_reset(self._h, _ptr(x), k, _stream(stream)), "cap_plan_reset")
+
+    def launch(self, b: torch.Tensor, x: torch.Tensor, stream=None):
+        """Second half of solve(): the solve kernel alone (after reset(x))."""
+        k = 1 if b.dim() == 1 else b.shape[1]
+        N.check(N.lib().cap_plan_launch(self._h, _ptr(b), _ptr(x), k, _stream(stream)), "cap_plan_launch")
         return x
 
     def adjoint(self, g: torch.Tensor, x: torch.Tensor, want_grad_val: bool = True, stream=None):

@@ -73,7 +73,9 @@ typedef struct cap_opts {
     int spin_limit;   /* polls without progress before a wait gives up
                          (0 = default 2^24); the watchdog never lets a
                          kernel spin forever on a malformed input */
-    int reserved[7];
+    int blocks_per_cu; /* persistent grid size in workgroups per compute unit
+                          (0 = tuned / default 1) */
+    int reserved[6];
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -92,6 +94,12 @@ int cap_plan_create(cap_plan *plan, int n, int nnz, const int *row_ptr,
                     const int *col_idx, const void *val, int value_type,
                     const cap_opts *opts, void *stream);
 int cap_plan_solve(cap_plan plan, const void *b, void *x, int rhs, void *stream);
+/* cap_plan_solve = cap_plan_reset + cap_plan_launch.  The reset (tickets;
+ * the sentinel fill of x in the value protocol, the in-degree restore for
+ * CAP_ALGO_CSC) may be issued early to overlap other work; the launch is the
+ * solve kernel alone.  Both asynchronous on `stream`, same b/x/rhs. */
+int cap_plan_reset(cap_plan plan, void *x, int rhs, void *stream);
+int cap_plan_launch(cap_plan plan, const void *b, void *x, int rhs, void *stream);
 /* Synchronizes `stream`, then reads and clears the plan's watchdog word:
  * CAP_OK, or CAP_ERR_WATCHDOG if any wait gave up since the last call. */
 int cap_plan_status(cap_plan plan, void *stream);

@@ -1,0 +1,86 @@
+"""The benchmark-compatible wrappers and the ./sptrsv_main drop-in driver on the GPU.
+
+They build x_ref = 1, b = L*1 internally and validate every repetition with
+the reference's own tests (elementwise 1 % for the CSR wrappers, relative L1
+< 1e-4 for the CSC one): a passing run reports solve_time > 0.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from capellinisptrsv_amd import api
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit_L(n=30000, seed=0):
+    rp, ci, v = api.make_config(0, seed=seed, scale=n / 100000)
+    nnzL, rpL, ciL, vL = api.change2tran(len(rp) - 1, len(ci), rp, ci, v)
+    return rpL, ciL, vL
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_wrappers(cuda, dtype):
+    rp, ci, v = _unit_L()
+    v = v.astype(dtype)
+    n = len(rp) - 1
+    for fn in (api.spts_syncfree_hip_csr_wrt, api.spts_syncfree_hip_csr):
+        ret, t, gf, bw = fn(rp, ci, v, n, n, len(ci))
+        assert ret == 0 and t > 0 and gf > 0 and bw > 0, fn.__name__
+        assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize) / (1e6 * t)) <= 1e-9 * bw
+        assert abs(gf - 2 * len(ci) / (1e6 * t)) <= 1e-9 * gf
+    ret, t, gf, bw, pre, wo, eo = api.spts_syncfree_hip_mix(rp, ci, v, n, n, len(ci))
+    ln, _, wo_ref, eo_ref = oracle.matrix_warp4(n, len(ci), rp, 16, 64)
+    assert ret == 0 and t > 0 and pre >= 0 and wo == wo_ref and eo == eo_ref
+    assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize, ln) / (1e6 * t)) <= 1e-9 * bw
+
+
+def test_not_square_returns_minus_one(cuda):
+    rp, ci, v = _unit_L(1000)
+    n = len(rp) - 1
+    assert api.spts_syncfree_hip_csr_wrt(rp, ci, v, n, n + 1, len(ci))[0] == -1
+
+
+def test_csc_wrapper(cuda):
+    rp, ci, v = _unit_L()
+    n = len(rp) - 1
+    cr, cp, cv = api.matrix_transposition(n, n, len(ci), rp, ci, v)
+    x_ref, b = api.get_x_b(n, n, rp, ci, v)
+    ret, x, pre, t, gf, bw = api.sptrsv_syncfree_hip_csc(cp, cr, cv, n, n, len(ci), b, x_ref)
+    assert ret == 0 and t > 0 and pre > 0
+    assert np.array_equal(x, x_ref)
+
+
+def test_driver_appends_result_csv(cuda, tmp_path):
+    mtx = tmp_path / "tri.mtx"
+    rng = np.random.default_rng(1)
+    n = 5000
+    lines = ["%%MatrixMarket matrix coordinate real general", "% generated", f"{n} {n} {4 * n}"]
+    for i in range(n):
+        for c in rng.integers(0, n, size=3):
+            lines.append(f"{i + 1} {int(c) + 1} {rng.uniform(-1, 1):.6f}")
+        lines.append(f"{i + 1} {i + 1} 4.0")
+    mtx.write_text("\n".join(lines) + "\n")
+    env = dict(os.environ)
+    for algo, exe in (("wrtfst", "sptrsv_main"), ("mix", "sptrsv_main"), ("2part", "sptrsv_main_f32"),
+                      ("csc", "sptrsv_main")):
+        r = subprocess.run([os.path.join(ROOT, "capellinisptrsv_amd", exe), str(mtx), "-a", algo], cwd=tmp_path,
+                           env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (algo, r.stdout, r.stderr)
+    rows = (tmp_path / "result.csv").read_text().splitlines()
+    assert len(rows) == 8
+    assert rows[0] == "filename,m,nnzL,nnzL/m,layer,parallelism,pre_time,solve_time,gflops,bandwith"
+    assert rows[2].startswith("filename,m,nnzL,nnzL/m,layer,parallelism,pre_time,solve_time,gflops,bandwith,"
+                              "warp_occupy,element_occupy")
+    ret, m, nn, nnzA, rpA, ciA, vA = oracle.read_mtx(str(mtx))
+    nnzL, rpL, ciL, vL = oracle.change2tran(m, nnzA, rpA, ciA, vA)
+    layer, par = oracle.matrix_layer(m, rpL, ciL)
+    for row in rows[1::2]:
+        f = [s.strip() for s in row.split(",")]
+        assert f[0] == str(mtx) and int(f[1]) == m and int(f[2]) == nnzL and int(f[4]) == layer
+        assert f[3] == f"{nnzL / m:4.2f}" and f[5] == f"{par:4.2f}"
+        assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
