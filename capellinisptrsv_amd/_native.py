@@ -30,6 +30,7 @@ CAP_F64, CAP_F32 = 0, 1
 ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4}
 ALGO_NAME = {v: k for k, v in ALGO.items()}
 SYNC = {"auto": 0, "value": 1, "flag": 2}
+ADJOINT = {"auto": 0, "scatter": 1, "gather": 2}
 SYNC_NAME = {v: k for k, v in SYNC.items()}
 
 
@@ -47,7 +48,7 @@ class CapError(RuntimeError):
 class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
-                ("reserved", c_int * 6)]
+                ("adjoint", c_int), ("reserved", c_int * 5)]
 
 
 class cap_gen_params(Structure):
@@ -172,5 +173,7 @@ def default_opts(**kw) -> cap_opts:
             v = ALGO[v]
         if k == "sync" and isinstance(v, str):
             v = SYNC[v]
+        if k == "adjoint" and isinstance(v, str):
+            v = ADJOINT[v]
         setattr(o, k, int(v))
     return o

@@ -178,10 +178,12 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
 // Solves row i (off-diagonal entries [j, d), diagonal at d) for right-hand
 // side q of k (x and b interleaved: x[row*k + q]) and publishes it.  `s`
 // carries a partial sum when the caller already consumed a prefix (two-part).
+// n_rev > 0 stores row r at position n_rev-1-r of b and x (the adjoint solves
+// L^T in reversed row order, which is lower triangular).
 template <typename T, int SYNC>
 __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *__restrict__ val,
                                            const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
-                                           int i, int j, int d, int k, int q, T s)
+                                           int i, int j, int d, int k, int q, T s, int n_rev = 0)
 {
     typedef typename VBits<T>::U U;
     U *xu = reinterpret_cast<U *>(x);
@@ -194,7 +196,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
     for (int t = 0; t < kWin; ++t) {
         rdy[t] = false;
         xw[t] = 0;
-        cw[t] = t < cnt ? ci[j + t] : 0;
+        cw[t] = t < cnt ? (n_rev ? n_rev - 1 - ci[j + t] : ci[j + t]) : 0;
         vw[t] = t < cnt ? val[j + t] : (T)0;
     }
     // The loop exits only when every lane of the wavefront has published
@@ -256,7 +258,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                         for (int t = 0; t < kWin; ++t) {
                             rdy[t] = false;
                             if (t < cnt) {
-                                cw[t] = ci[j + t];
+                                cw[t] = n_rev ? n_rev - 1 - ci[j + t] : ci[j + t];
                                 vw[t] = val[j + t];
                             }
                         }
@@ -264,7 +266,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                 }
             }
             if (used == cnt && j >= d) {  // write first: publish inside the polling loop
-                const long long o = (long long)i * k + q;
+                const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
                 publish<T, SYNC>(x, sc, o, (b[o] - s) / val[d]);
                 asm volatile("" ::: "memory");
                 done = true;
@@ -281,13 +283,13 @@ template <typename T, int SYNC>
 __global__ void __launch_bounds__(kBlock)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
-             unsigned *__restrict__ ticket)
+             unsigned *__restrict__ ticket, int n_rev)
 {
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
     ticket_loop(ticket, kBlock >> lpr_log2, n, [&](int base) {
         const int i = base + r;
-        if (i < n && q < k) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0);
+        if (i < n && q < k) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev);
     });
 }
 
@@ -431,6 +433,18 @@ __global__ void __launch_bounds__(kBlock)
             }
         } while (__any(!done));
     });
+}
+
+// grad_val[j] = -grad_b[row_j] * x[col_j] over the stored pattern (gather
+// adjoint; the scatter adjoint fuses this into k_adjoint)
+__global__ void __launch_bounds__(kBlock)
+    k_grad_val(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ grad_b,
+               const double *__restrict__ x, double *__restrict__ grad_val, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double y = grad_b[i];
+    for (int j = rp[i]; j < rp[i + 1]; ++j) grad_val[j] = -y * x[ci[j]];
 }
 
 // off-diagonal column counts of L (the adjoint's in-degrees)

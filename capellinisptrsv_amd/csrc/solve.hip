@@ -67,9 +67,13 @@ struct cap_plan_s {
     // CSC executor (forward)
     int *cp = nullptr, *cri = nullptr, *deg0 = nullptr, *deg = nullptr;
     void *cv = nullptr, *left_sum = nullptr;
-    // adjoint
+    // adjoint: scatter (in-degree counters + atomic left sums) or gather
+    // (reversed transpose of L, solved with the write-first kernel)
+    int adjoint_mode = CAP_ADJOINT_GATHER;
     int *cnt0 = nullptr, *cnt = nullptr;
     double *adj_ls = nullptr;
+    int *rpT = nullptr, *ciT = nullptr;
+    double *vT = nullptr;
     // refinement
     float *val32 = nullptr, *r32 = nullptr, *d32 = nullptr;
     long long refine_len = 0;
@@ -136,7 +140,7 @@ void plan_free(cap_plan p)
         (void)hipFree(p->val);
     }
     void *bufs[] = {p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
-                    p->cnt0, p->cnt, p->adj_ls, p->val32, p->r32, p->d32};
+                    p->cnt0, p->cnt, p->adj_ls, p->rpT, p->ciT, p->vT, p->val32, p->r32, p->d32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->refine_graph) (void)hipGraphExecDestroy(p->refine_graph);
@@ -212,9 +216,9 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const int rows_per_block = kBlock >> l;
         const unsigned nb = grid_for(p, ((long long)n + rows_per_block - 1) / rows_per_block);
         if (sync == CAP_SYNC_VALUE)
-            k_wrtfst<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
+            k_wrtfst<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
         else
-            k_wrtfst<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket);
+            k_wrtfst<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
     } else if (algo == CAP_ALGO_TWOPART) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         const unsigned nb = grid_for(p, ((long long)n + kBlock - 1) / kBlock);
@@ -492,6 +496,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if ((rc = init_ctrl(p)) != CAP_OK) break;
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
+        if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
         p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
         const int border = o.border > 0 ? o.border : kMixBorderDefault;
@@ -510,6 +515,44 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         return rc;
     }
     *out = p;
+    return CAP_OK;
+}
+
+// Reversed transpose of L for the gather adjoint: row i' = n-1-c holds
+// column c of L with rows r mapped to n-1-r in ascending order, i.e. the
+// column's entries from last to first, diagonal last.  Solving this lower
+// triangular matrix with the write-first kernel accumulates each y_c over
+// rows in descending order: the executor's BACKWARD order
+// (serialref.h:75-100), so the gather adjoint is deterministic.
+int ensure_reversed_transpose(cap_plan p)
+{
+    if (p->rpT) return CAP_OK;
+    const int n = p->n, nnz = p->nnz;
+    std::vector<int> rp((size_t)n + 1), ci((size_t)nnz), cp((size_t)n + 1), cr((size_t)nnz);
+    std::vector<double> v((size_t)nnz), cv((size_t)nnz);
+    CAP_HIP_TRY(hipMemcpy(rp.data(), p->rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
+    CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
+    cap_transpose_d(n, n, nnz, rp.data(), ci.data(), v.data(), cr.data(), cp.data(), cv.data());
+    std::vector<int> rT((size_t)n + 1), cT((size_t)nnz);
+    std::vector<double> vT((size_t)nnz);
+    size_t o = 0;
+    rT[0] = 0;
+    for (int ip = 0; ip < n; ++ip) {
+        const int c = n - 1 - ip;
+        for (int j = cp[c + 1] - 1; j >= cp[c]; --j) {
+            cT[o] = n - 1 - cr[j];
+            vT[o] = cv[j];
+            ++o;
+        }
+        rT[ip + 1] = (int)o;
+    }
+    CAP_TRY(dev_alloc(&p->rpT, (size_t)n + 1));
+    CAP_TRY(dev_alloc(&p->ciT, (size_t)nnz));
+    CAP_TRY(dev_alloc(&p->vT, (size_t)nnz));
+    CAP_HIP_TRY(hipMemcpy(p->rpT, rT.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    CAP_HIP_TRY(hipMemcpy(p->ciT, cT.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
+    CAP_HIP_TRY(hipMemcpy(p->vT, vT.data(), sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
     return CAP_OK;
 }
 
@@ -600,6 +643,19 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
     hipStream_t st = S(stream);
     const int n = p->n;
+    if (n == 0) return CAP_OK;
+    if (p->adjoint_mode == CAP_ADJOINT_GATHER) {
+        CAP_TRY(ensure_reversed_transpose(p));
+        SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
+        CAP_TRY(launch_prepare<double>(p, CAP_SYNC_VALUE, grad_b, n, st));
+        k_wrtfst<double, 1><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+            p->rpT, p->ciT, p->vT, (const double *)g, (double *)grad_b, sc, n, 1, 0, p->ticket, n);
+        if (grad_val)
+            k_grad_val<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)grad_b, (const double *)x,
+                                                          (double *)grad_val, n);
+        CAP_HIP_TRY(hipGetLastError());
+        return CAP_OK;
+    }
     if (!p->cnt0) {
         CAP_TRY(dev_alloc(&p->cnt0, (size_t)n));
         CAP_TRY(dev_alloc(&p->cnt, (size_t)n));
@@ -608,7 +664,6 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
         k_offdiag_colcount<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, n, p->cnt0);
         CAP_HIP_TRY(hipGetLastError());
     }
-    if (n == 0) return CAP_OK;
     CAP_HIP_TRY(hipMemcpyAsync(p->cnt, p->cnt0, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
     k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
                                                                     p->ticket, kTickets);

@@ -57,6 +57,13 @@ extern "C" {
 #define CAP_ALGO_MIX 3     /* per-64-row chunk: thread or wavefront per row (mix) */
 #define CAP_ALGO_CSC 4     /* wavefront-per-column in-degree scatter (SyncFree_csc) */
 
+/* adjoint method (cap_opts.adjoint) */
+#define CAP_ADJOINT_AUTO 0
+#define CAP_ADJOINT_SCATTER 1 /* column-driven on the CSR of L: in-degree counters
+                                 and atomic left sums (SyncFree_csc BACKWARD) */
+#define CAP_ADJOINT_GATHER 2  /* write-first solve on a reversed transpose of L
+                                 built once per plan; deterministic */
+
 /* readiness protocol (cap_opts.sync) */
 #define CAP_SYNC_AUTO 0
 #define CAP_SYNC_VALUE 1 /* x itself is the flag: x starts as a signalling-NaN
@@ -75,7 +82,8 @@ typedef struct cap_opts {
                          kernel spin forever on a malformed input */
     int blocks_per_cu; /* persistent grid size in workgroups per compute unit
                           (0 = tuned / default 1) */
-    int reserved[6];
+    int adjoint;       /* CAP_ADJOINT_* (0 = gather) */
+    int reserved[5];
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -108,10 +116,14 @@ int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
 int cap_plan_destroy(cap_plan plan);
 
-/* Adjoint of x = L^{-1} b: grad_b = L^{-T} g (column-driven sync-free
- * transposed solve on the same CSR; in-degree counters + atomic left sums) and
- * grad_val[k] = -grad_b[row_k] * x[col_k] on the pattern (grad_val may be
- * NULL).  FP64 plans only.  Device pointers, asynchronous on `stream`. */
+/* Adjoint of x = L^{-1} b: grad_b = L^{-T} g and grad_val[k] =
+ * -grad_b[row_k] * x[col_k] on the pattern (grad_val may be NULL).  FP64
+ * plans only.  Device pointers, asynchronous on `stream`.  cap_opts.adjoint
+ * selects the method: SCATTER is the column-driven sync-free transposed
+ * solve on the same CSR (in-degree counters + atomic left sums; the order of
+ * the atomic additions depends on scheduling), GATHER solves a reversed
+ * transpose of L built on first use (bit-reproducible, the executor's
+ * backward accumulation order). */
 int cap_plan_adjoint(cap_plan plan, const void *g, const void *x, void *grad_b,
                      void *grad_val, void *stream);
 

@@ -19,15 +19,16 @@ from conftest import random_lower, rel_maxnorm
 pytestmark = pytest.mark.gpu
 
 
-def _plan(cuda, rp, ci, v, algo="wrtfst"):
+def _plan(cuda, rp, ci, v, algo="wrtfst", adjoint="auto"):
     import torch
     from capellinisptrsv_amd import Plan
     d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
-    return Plan(*d, algo=algo, sync="value"), d
+    return Plan(*d, algo=algo, sync="value", adjoint=adjoint), d
 
 
+@pytest.mark.parametrize("mode", ["gather", "scatter"])
 @pytest.mark.parametrize("mat", ["band", "level", "chain_dup"])
-def test_adjoint_matches_oracle(cuda, mat):
+def test_adjoint_matches_oracle(cuda, mat, mode):
     import torch
     if mat == "band":
         rp, ci, v = random_lower(4000, max_off=6, band=60, seed=21)
@@ -36,7 +37,7 @@ def test_adjoint_matches_oracle(cuda, mat):
     else:
         rp, ci, v = random_lower(3000, max_off=5, band=10, seed=22, dup=True)
     n = len(rp) - 1
-    plan, d = _plan(cuda, rp, ci, v)
+    plan, d = _plan(cuda, rp, ci, v, adjoint=mode)
     b = api.uniform(1, n)
     g = api.uniform(2, n)
     x = plan.solve(torch.from_numpy(b).to(cuda))
@@ -45,6 +46,8 @@ def test_adjoint_matches_oracle(cuda, mat):
     assert plan.status() == 0
     y_ref = oracle.adjoint(rp, ci, v, g, fused=False)
     assert rel_maxnorm(gb.cpu().numpy(), y_ref) <= 1e-12
+    if mode == "gather":  # the executor's backward order, fused: bit-exact
+        assert np.array_equal(gb.cpu().numpy(), oracle.adjoint(rp, ci, v, g, fused=True))
     gv_ref = oracle.grad_val(rp, ci, y_ref, oracle.csr_forward(rp, ci, v, b, fused=True))
     assert rel_maxnorm(gv.cpu().numpy(), gv_ref) <= 1e-12
     # the oracle's adjoint is the executor's BACKWARD mode on L read as CSC of L^T
