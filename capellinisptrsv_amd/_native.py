@@ -48,7 +48,7 @@ class CapError(RuntimeError):
 class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
-                ("adjoint", c_int), ("reserved", c_int * 5)]
+                ("adjoint", c_int), ("lds_window", c_int), ("reserved", c_int * 4)]
 
 
 class cap_gen_params(Structure):

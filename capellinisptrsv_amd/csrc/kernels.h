@@ -91,21 +91,48 @@ __device__ __forceinline__ bool give_up(const SyncCtx &sc, int idle)
 }
 
 // Persistent ticket loop: the workgroup takes `per_block` work items at a time
-// (one atomicAdd, broadcast through LDS) and calls body(base) until `total`
-// items are handed out.  body must not contain workgroup barriers.
-template <typename F>
-__device__ __forceinline__ void ticket_loop(unsigned *ticket, int per_block, int total, F body)
+// (one atomicAdd, broadcast through LDS), runs init(base) on every thread
+// between two barriers (all threads have left the previous body), then
+// body(base), until `total` items are handed out.  body must not contain
+// workgroup barriers.
+template <typename FI, typename F>
+__device__ __forceinline__ void ticket_loop(unsigned *ticket, int per_block, int total, FI init, F body)
 {
     __shared__ int s_base;
     while (true) {
         if (threadIdx.x == 0) s_base = (int)atomicAdd(ticket, (unsigned)per_block);
         __syncthreads();
         const int base = s_base;
+        init(base);
         __syncthreads();
         if (base >= total) return;
         body(base);
     }
 }
+template <typename F>
+__device__ __forceinline__ void ticket_loop(unsigned *ticket, int per_block, int total, F body)
+{
+    ticket_loop(ticket, per_block, total, [](int) {}, body);
+}
+
+// Workgroup-local readiness window: values of the rows [lo, lo + len) the
+// workgroup is solving, kept in LDS next to their global copies.  A
+// dependency inside the window is polled in LDS (one CU, no memory traffic);
+// older rows are polled in global memory.  len == 0 disables it.
+template <typename U>
+struct LdsWin {
+    U *s;
+    int lo, len, k;
+    __device__ __forceinline__ bool has(int c) const { return (unsigned)(c - lo) < (unsigned)len; }
+    __device__ __forceinline__ U load(int c, int q) const
+    {
+        return __hip_atomic_load(s + (c - lo) * k + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ void store(int c, int q, U v) const
+    {
+        __hip_atomic_store(s + (c - lo) * k + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+};
 
 // Publish x[i] in the value protocol: the sentinel can never be published
 // (division results are never signalling NaNs; the guard keeps that true for
@@ -183,7 +210,8 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
 template <typename T, int SYNC>
 __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *__restrict__ val,
                                            const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
-                                           int i, int j, int d, int k, int q, T s, int n_rev = 0)
+                                           int i, int j, int d, int k, int q, T s, int n_rev = 0,
+                                           LdsWin<typename VBits<T>::U> win = {nullptr, 0, 0, 1})
 {
     typedef typename VBits<T>::U U;
     U *xu = reinterpret_cast<U *>(x);
@@ -210,7 +238,8 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                 if (SYNC == 1) {
 #pragma unroll
                     for (int t = 0; t < kWin; ++t)
-                        if (t >= used && t < cnt && !rdy[t]) xw[t] = ld_agent(xu + (long long)cw[t] * k + q);
+                        if (t >= used && t < cnt && !rdy[t])
+                            xw[t] = win.has(cw[t]) ? win.load(cw[t], q) : ld_agent(xu + (long long)cw[t] * k + q);
 #pragma unroll
                     for (int t = 0; t < kWin; ++t)
                         if (t >= used && t < cnt && !rdy[t]) rdy[t] = xw[t] != VBits<T>::kSentinel;
@@ -267,7 +296,13 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
             }
             if (used == cnt && j >= d) {  // write first: publish inside the polling loop
                 const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
-                publish<T, SYNC>(x, sc, o, (b[o] - s) / val[d]);
+                const T xi = (b[o] - s) / val[d];
+                if (SYNC == 1 && win.len) {
+                    U u = as_bits(xi);
+                    if (u == VBits<T>::kSentinel) u = VBits<T>::kQNaN;
+                    win.store(i, q, u);
+                }
+                publish<T, SYNC>(x, sc, o, xi);
                 asm volatile("" ::: "memory");
                 done = true;
             }
@@ -279,18 +314,28 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 // per row and right-hand side; a workgroup ticket covers kBlock >> lpr_log2
 // rows with 1 << lpr_log2 lanes per row, lane q of a row owning right-hand
 // side q (lanes with q >= k idle).
-template <typename T, int SYNC>
+template <typename T, int SYNC, int LDS>
 __global__ void __launch_bounds__(kBlock)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
              unsigned *__restrict__ ticket, int n_rev)
 {
+    typedef typename VBits<T>::U U;
+    __shared__ U s_x[LDS ? kBlock : 1];
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
-    ticket_loop(ticket, kBlock >> lpr_log2, n, [&](int base) {
-        const int i = base + r;
-        if (i < n && q < k) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev);
-    });
+    const int rows = kBlock >> lpr_log2;
+    ticket_loop(
+        ticket, rows, n,
+        [&](int) {
+            if (LDS) s_x[threadIdx.x] = VBits<T>::kSentinel;
+        },
+        [&](int base) {
+            const int i = base + r;
+            LdsWin<U> win{s_x, base, (LDS && !n_rev) ? rows : 0, k};
+            if (i < n && q < k)
+                wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win);
+        });
 }
 
 // our2Part (CUDA/our2Part/spts_syncfree_cuda_csr.h:10-124): dependencies on
@@ -322,30 +367,68 @@ __global__ void __launch_bounds__(kBlock)
 // (lane-strided products, blocking waits -- every dependency is an earlier
 // unit -- and a butterfly reduction); a longer unit is solved thread-per-row
 // with the write-first loop.  Single right-hand side.
-template <typename T, int SYNC>
-__global__ void __launch_bounds__(kBlock)
+template <typename T, int SYNC, int BS>
+__global__ void __launch_bounds__(BS)
     k_mix(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
           const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, const int *__restrict__ unit, int n_units,
           unsigned *__restrict__ ticket)
 {
+    typedef typename VBits<T>::U U;
+    constexpr int kUnits = BS / kWave;     // one unit per wavefront per ticket
+    __shared__ U s_x[BS / kWave * kWave];  // rows of the ticket: <= 64 per unit
+    __shared__ int s_lo, s_len;
     const int lane = threadIdx.x & (kWave - 1);
-    ticket_loop(ticket, kBlock / kWave, n_units, [&](int base) {
-        const int u = base + (int)(threadIdx.x / kWave);
-        if (u >= n_units) return;
-        const int r0 = unit[u], r1 = unit[u + 1];
-        if (r1 - r0 > 1) {
-            const int i = r0 + lane;
-            if (i < r1) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0);
-            return;
-        }
-        const int row = r0;
-        const int d = rp[row + 1] - 1;
-        T s = (T)0;
-        for (int j = rp[row] + lane; j < d; j += kWave) s = fmac(val[j], wait_value<T, SYNC>(x, sc, ci[j]), s);
+    ticket_loop(
+        ticket, kUnits, n_units,
+        [&](int base) {
+            if (SYNC == 1) s_x[threadIdx.x] = VBits<T>::kSentinel;
+            if (threadIdx.x == 0 && base < n_units) {
+                s_lo = unit[base];
+                s_len = unit[min(base + kUnits, n_units)] - s_lo;
+            }
+        },
+        [&](int base) {
+            const int u = base + (int)(threadIdx.x / kWave);
+            if (u >= n_units) return;
+            const LdsWin<U> win{s_x, s_lo, SYNC == 1 ? s_len : 0, 1};
+            const int r0 = unit[u], r1 = unit[u + 1];
+            if (r1 - r0 > 1) {
+                const int i = r0 + lane;
+                if (i < r1) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0, 0, win);
+                return;
+            }
+            const int row = r0;
+            const int d = rp[row + 1] - 1;
+            T s = (T)0;
+            for (int j = rp[row] + lane; j < d; j += kWave) {
+                const int c = ci[j];
+                T xv;
+                if (win.has(c)) {  // blocking wait in LDS: an earlier wavefront of this workgroup
+                    U raw;
+                    int idle = 0;
+                    while ((raw = win.load(c, 0)) == VBits<T>::kSentinel)
+                        if (give_up(sc, ++idle)) {
+                            raw = VBits<T>::kQNaN;
+                            break;
+                        }
+                    xv = as_val(raw);
+                } else {
+                    xv = wait_value<T, SYNC>(x, sc, c);
+                }
+                s = fmac(val[j], xv, s);
+            }
 #pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) publish<T, SYNC>(x, sc, row, (b[row] - s) / val[d]);
-    });
+            for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) {
+                const T xi = (b[row] - s) / val[d];
+                if (win.len) {
+                    U o = as_bits(xi);
+                    if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;
+                    win.store(row, 0, o);
+                }
+                publish<T, SYNC>(x, sc, row, xi);
+            }
+        });
 }
 
 // ------------------------------------------------ SyncFree CSC (K4 / K5)

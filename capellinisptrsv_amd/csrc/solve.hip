@@ -60,6 +60,7 @@ struct cap_plan_s {
     int *err = nullptr;
     int spin_limit = kSpinLimitDefault;
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
+    int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
     // mix
     int *unit = nullptr;
     int n_units = 0, mix_len = 0;
@@ -215,10 +216,12 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const int l = lpr_log2_for(rhs);
         const int rows_per_block = kBlock >> l;
         const unsigned nb = grid_for(p, ((long long)n + rows_per_block - 1) / rows_per_block);
-        if (sync == CAP_SYNC_VALUE)
-            k_wrtfst<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
+        if (sync == CAP_SYNC_VALUE && p->lds)
+            k_wrtfst<T, 1, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
+        else if (sync == CAP_SYNC_VALUE)
+            k_wrtfst<T, 1, 0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
         else
-            k_wrtfst<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
+            k_wrtfst<T, 2, 0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0);
     } else if (algo == CAP_ALGO_TWOPART) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         const unsigned nb = grid_for(p, ((long long)n + kBlock - 1) / kBlock);
@@ -228,11 +231,14 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
             k_twopart<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, p->ticket);
     } else if (algo == CAP_ALGO_MIX) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
-        const unsigned nb = grid_for(p, ((long long)p->n_units * kWave + kBlock - 1) / kBlock);
+        constexpr int kMixBlock = 1024;  // 16 units per ticket share an LDS window
+        const unsigned nb = grid_for(p, ((long long)p->n_units * kWave + kMixBlock - 1) / kMixBlock);
         if (sync == CAP_SYNC_VALUE)
-            k_mix<T, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
+            k_mix<T, 1, kMixBlock><<<nb, kMixBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units,
+                                                             p->ticket);
         else
-            k_mix<T, 2><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units, p->ticket);
+            k_mix<T, 2, kMixBlock><<<nb, kMixBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units,
+                                                             p->ticket);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         const unsigned nb = grid_for(p, ((long long)n * kWave + kBlock - 1) / kBlock);
@@ -496,6 +502,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if ((rc = init_ctrl(p)) != CAP_OK) break;
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
+        p->lds = o.lds_window > 0;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
         p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
@@ -648,7 +655,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
         CAP_TRY(ensure_reversed_transpose(p));
         SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
         CAP_TRY(launch_prepare<double>(p, CAP_SYNC_VALUE, grad_b, n, st));
-        k_wrtfst<double, 1><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        k_wrtfst<double, 1, 0><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
             p->rpT, p->ciT, p->vT, (const double *)g, (double *)grad_b, sc, n, 1, 0, p->ticket, n);
         if (grad_val)
             k_grad_val<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)grad_b, (const double *)x,

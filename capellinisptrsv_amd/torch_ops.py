@@ -42,7 +42,7 @@ class Plan:
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
-                 blocks_per_cu: int = 0, adjoint="auto", stream=None):
+                 blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -53,7 +53,8 @@ class Plan:
         self.nnz = col_idx.numel()
         self.dtype = values.dtype
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
-                              spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint)
+                              spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
+                              lds_window=int(lds_window))
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")

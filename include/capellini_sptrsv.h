@@ -83,7 +83,9 @@ typedef struct cap_opts {
     int blocks_per_cu; /* persistent grid size in workgroups per compute unit
                           (0 = tuned / default 1) */
     int adjoint;       /* CAP_ADJOINT_* (0 = gather) */
-    int reserved[5];
+    int lds_window;    /* 1 = thread-per-row kernels resolve dependencies on
+                          rows of their own workgroup in LDS (value protocol) */
+    int reserved[4];
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */

@@ -19,11 +19,11 @@ FP64_TOL = 1e-12
 FP32_TOL = 1e-5
 
 
-def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0):
+def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0, lds=False, bpc=0):
     import torch
     from capellinisptrsv_amd import Plan
     plan = Plan(torch.from_numpy(rp).to(cuda), torch.from_numpy(ci).to(cuda), torch.from_numpy(v).to(cuda),
-                algo=algo, sync=sync, border=border)
+                algo=algo, sync=sync, border=border, lds_window=lds, blocks_per_cu=bpc)
     bt = torch.from_numpy(np.ascontiguousarray(b.reshape(-1, k) if k > 1 else b)).to(cuda)
     x = plan.solve(bt)
     torch.cuda.synchronize()
@@ -48,15 +48,16 @@ def _matrices():
 MATS = list(_matrices())
 
 
-@pytest.mark.parametrize("algo,sync", [("wrtfst", "value"), ("wrtfst", "flag"), ("twopart", "value"),
-                                       ("twopart", "flag")])
+@pytest.mark.parametrize("algo,sync,lds,bpc", [("wrtfst", "value", False, 1), ("wrtfst", "value", True, 1),
+                                               ("wrtfst", "value", True, 4), ("wrtfst", "flag", False, 2),
+                                               ("twopart", "value", False, 1), ("twopart", "flag", False, 1)])
 @pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
-def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, name, mat):
+def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, lds, bpc, name, mat):
     rp, ci, v = mat
     n = len(rp) - 1
     b = api.uniform(7, n)
     ref = oracle.csr_forward(rp, ci, v, b, fused=True)
-    x, info = _solve(cuda, rp, ci, v, b, algo, sync)
+    x, info = _solve(cuda, rp, ci, v, b, algo, sync, lds=lds, bpc=bpc)
     assert info["algo"] == algo and info["sync"] == sync
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
     # and within tolerance of the reference's unfused serial arithmetic
@@ -104,13 +105,13 @@ def test_csc_syncfree(cuda, name, mat):
 
 
 @pytest.mark.parametrize("k", [2, 3, 8, 16, 32, 64])
-@pytest.mark.parametrize("sync", ["value", "flag"])
-def test_multi_rhs_bit_exact(cuda, k, sync):
+@pytest.mark.parametrize("sync,lds", [("value", False), ("value", True), ("flag", False)])
+def test_multi_rhs_bit_exact(cuda, k, sync, lds):
     rp, ci, v = api.make_config(0, seed=3, scale=0.2)
     n = len(rp) - 1
     b = api.uniform(100 + k, n * k)
     ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
-    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", sync, k=k)
+    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", sync, k=k, lds=lds)
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
 
 
