@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--quick", action="store_true", help="headline workload only (skip the extra workloads)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solve is a single-GPU kernel and this build runs on one MI355X "
@@ -199,7 +200,122 @@ def main():
     out["fp64_err_vs_cpu"] = float(np.max(np.abs(x64.cpu().numpy() - ref)) / den)
     out["refined_err_vs_cpu_fp64"] = float(np.max(np.abs(xr.cpu().numpy() - ref)) / den)
     out["fp32_err_vs_cpu_fp64"] = float(np.max(np.abs(x32.double().cpu().numpy() - ref)) / den)
+    plan.close()
+    plan32.close()
+    del d_rp, d_ci, d_v, b, x, xr, x32, b32
+    if not args.quick:
+        out["extra_workloads"] = extra_workloads(args)
     print(json.dumps(out), flush=True)
+
+
+def _time_gpu(fn, steps, warmup=3):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def extra_workloads(args):
+    """The other benchmark workloads, measured the same way (steps back-to-back)."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from capellinisptrsv_amd import Batch, Plan, api
+
+    dev = torch.device("cuda:0")
+    res = {}
+
+    # band (n = 100,000, 3 off-diagonals in the previous 1,000 rows, b = L*1) and
+    # deep band (n = 500,000, bandwidth 128, p = 0.3): autotuned plans
+    for name, cfg in (("band_100k", 0), ("deep_band_500k", 2)):
+        log(f"extra workload {name}")
+        rp, ci, v = api.make_config(cfg, seed=0)
+        n, nnz = len(rp) - 1, len(ci)
+        d = [torch.from_numpy(a).to(dev) for a in (rp, ci, v)]
+        if cfg == 0:
+            b = torch.from_numpy(api.get_x_b(n, n, rp, ci, v)[1]).to(dev)
+        else:
+            b = torch.from_numpy(api.uniform(1, n)).to(dev)
+        x = torch.empty_like(b)
+        p = Plan(*d)
+        ms = _time_gpu(lambda: p.solve(b, out=x), 20 if cfg == 0 else 5, warmup=2)
+        assert p.status() == 0
+        err = float(np.max(np.abs(x.cpu().numpy() - oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True))))
+        res[name] = {"n": n, "nnzL": nnz, "levels": api.matrix_layer(n, n, nnz, rp, ci, v)[0], "plan": p.info(),
+                     "ms_per_solve": round(ms, 4), "gflops": round(2 * nnz / (ms * 1e6), 3),
+                     "max_abs_err_vs_oracle": err}
+        p.close()
+
+    # differentiable solve: n = 1,000,000 level matrix, forward + adjoint
+    log("extra workload adjoint_1m")
+    rp, ci, v = api.make_config(4, seed=0)
+    n, nnz = len(rp) - 1, len(ci)
+    d = [torch.from_numpy(a).to(dev) for a in (rp, ci, v)]
+    b = torch.from_numpy(api.uniform(1, n)).to(dev)
+    xt = torch.from_numpy(api.uniform(2, n)).to(dev)
+    r = {"n": n, "nnzL": nnz}
+    for mode in ("gather", "scatter"):
+        p = Plan(*d, algo="wrtfst", sync="value", adjoint=mode)
+        x = p.solve(b)
+        g = (x - xt).contiguous()
+        fwd = _time_gpu(lambda: p.solve(b, out=x), 20)
+        adj = _time_gpu(lambda: p.adjoint(g, x), 20)
+        assert p.status() == 0
+        r[mode] = {"forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4), "adjoint_over_forward": round(adj / fwd, 3)}
+        p.close()
+    res["adjoint_1m"] = r
+
+    # batched: 256 matrices, n log-uniform in [2,000, 200,000], nnz/row U{1..16}
+    # (diagonal included), band U[64, 4096], k = 16 interleaved RHS
+    log("extra workload batched_256x16")
+    rng = np.random.default_rng(0)
+    mats = []
+    for m in range(256):
+        nm = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
+        mats.append(api.generate("band_rowcount", n=nm, offdiag=16, band=int(rng.integers(64, 4097)), seed=1000 + m))
+    k = 16
+    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
+    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
+    rp_all = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
+    ci_all = np.concatenate([m[1] for m in mats]).astype(np.int32)
+    v_all = np.concatenate([m[2] for m in mats])
+    N = int(row_off[-1])
+    bb = api.uniform(3, N * k)
+    t = [torch.from_numpy(a).to(dev) for a in (row_off, rp_all, ci_all, v_all)]
+    bt = Batch(*t, k=k)
+    bd = torch.from_numpy(bb.reshape(N, k)).to(dev)
+    xb = torch.empty_like(bd)
+    fp64_ms = _time_gpu(lambda: bt.solve(bd, out=xb), 5, warmup=1)
+    x64 = xb.clone()
+    xr = torch.empty_like(bd)
+    holder = {}
+
+    def refined():
+        holder["r"] = bt.solve_refined(bd, tol=1e-12, out=xr)
+    ref_ms = _time_gpu(refined, 3, warmup=1)
+    _, iters, rel = holder["r"]
+    # error against the FP64 oracle on a sample of matrices
+    worst64, worstr = 0.0, 0.0
+    for m in rng.choice(256, size=6, replace=False):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        o = oracle.csr_forward(*mats[m], bb[lo:hi], rhs=k, fused=True)
+        den = float(np.max(np.abs(o)))
+        worst64 = max(worst64, float(np.max(np.abs(x64.reshape(-1)[lo:hi].cpu().numpy() - o))) / den)
+        worstr = max(worstr, float(np.max(np.abs(xr.reshape(-1)[lo:hi].cpu().numpy() - o))) / den)
+    res["batched_256x16"] = {
+        "matrices": 256, "rows": N, "nnzL": int(len(ci_all)), "k": k,
+        "fp64_ms": round(fp64_ms, 3), "fp64_gflops": round(2 * len(ci_all) * k / (fp64_ms * 1e6), 2),
+        "fp32_refined_ms": round(ref_ms, 3), "refined_iterations": iters, "refined_over_fp64": round(ref_ms / fp64_ms, 3),
+        "fp64_rel_err_sampled": worst64, "refined_rel_err_sampled": worstr,
+    }
+    bt.close()
+    return res
 
 
 if __name__ == "__main__":

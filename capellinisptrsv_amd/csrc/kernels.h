@@ -202,6 +202,94 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
     }
 }
 
+// Non-blocking test of x[o]: true and the value when published.
+template <typename T, int SYNC>
+__device__ __forceinline__ bool try_value(const T *x, const SyncCtx &sc, long long o, T *out)
+{
+    typedef typename VBits<T>::U U;
+    const U *xu = reinterpret_cast<const U *>(x);
+    if (SYNC == 1) {
+        const U raw = ld_agent(xu + o);
+        *out = as_val(raw);
+        return raw != VBits<T>::kSentinel;
+    }
+    if (ld_agent(sc.flags + o) != sc.epoch) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *out = as_val(xu[o]);
+    return true;
+}
+
+__device__ __forceinline__ double bcast(double v, int lane)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float bcast(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// One row solved by a whole wavefront in the sequential order: lane l polls
+// off-diagonal l of the current 64-entry window; the wavefront then adds the
+// ready prefix term by term, s = fma(v_t, x_t, s) with broadcast operands, so
+// the sum is the oracle's fused sequential one and only the last dependency
+// to arrive sits on the critical path (one fma and the division after it).
+// Every dependency is an earlier row, solved by an earlier wavefront.
+template <typename T, int SYNC>
+__device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *__restrict__ ci,
+                                         const T *__restrict__ val, const T *__restrict__ b, T *__restrict__ x,
+                                         const SyncCtx &sc, const LdsWin<typename VBits<T>::U> &win, int row)
+{
+    typedef typename VBits<T>::U U;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int d = rp[row + 1] - 1;
+    T s = (T)0;
+    int idle = 0;
+    bool failed = false;
+    for (int j0 = rp[row]; j0 < d; j0 += kWave) {
+        const int cnt = min(kWave, d - j0);
+        const int c = lane < cnt ? ci[j0 + lane] : 0;
+        const T v = lane < cnt ? val[j0 + lane] : (T)0;
+        T xv = (T)0;
+        bool rdy = lane >= cnt;
+        int used = 0;
+        while (used < cnt) {
+            if (!rdy) {
+                if (win.has(c)) {
+                    const U raw = win.load(c, 0);
+                    rdy = raw != VBits<T>::kSentinel;
+                    xv = as_val(raw);
+                } else {
+                    rdy = try_value<T, SYNC>(x, sc, c, &xv);
+                }
+            }
+            const unsigned long long ready = __ballot(rdy);
+            const int prefix = ~ready ? __builtin_ctzll(~ready) : kWave;
+            const int upto = min(prefix, cnt);
+            for (int t = used; t < upto; ++t) s = fmac(bcast(v, t), bcast(xv, t), s);
+            if (upto > used) {
+                idle = 0;
+            } else if (give_up(sc, ++idle)) {  // uniform: idle is the same in every lane
+                failed = true;
+                break;
+            }
+            used = upto;
+        }
+        if (failed) break;
+    }
+    if (lane == 0) {
+        const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[row] - s) / val[d];
+        if (win.len) {
+            U o = as_bits(xi);
+            if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;
+            win.store(row, 0, o);
+        }
+        publish<T, SYNC>(x, sc, row, xi);
+    }
+}
+
 // Solves row i (off-diagonal entries [j, d), diagonal at d) for right-hand
 // side q of k (x and b interleaved: x[row*k + q]) and publishes it.  `s`
 // carries a partial sum when the caller already consumed a prefix (two-part).
@@ -364,9 +452,9 @@ __global__ void __launch_bounds__(kBlock)
 
 // mix (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:15-137): unit u covers rows
 // [unit[u], unit[u+1]).  A unit of one row is solved by the whole wavefront
-// (lane-strided products, blocking waits -- every dependency is an earlier
-// unit -- and a butterfly reduction); a longer unit is solved thread-per-row
-// with the write-first loop.  Single right-hand side.
+// (wave_row: parallel polls, sequential fused sum); a longer unit is solved
+// thread-per-row with the write-first loop.  The 16 units of a ticket share
+// an LDS window.  Single right-hand side.
 template <typename T, int SYNC, int BS>
 __global__ void __launch_bounds__(BS)
     k_mix(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
@@ -397,37 +485,7 @@ __global__ void __launch_bounds__(BS)
                 if (i < r1) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0, 0, win);
                 return;
             }
-            const int row = r0;
-            const int d = rp[row + 1] - 1;
-            T s = (T)0;
-            for (int j = rp[row] + lane; j < d; j += kWave) {
-                const int c = ci[j];
-                T xv;
-                if (win.has(c)) {  // blocking wait in LDS: an earlier wavefront of this workgroup
-                    U raw;
-                    int idle = 0;
-                    while ((raw = win.load(c, 0)) == VBits<T>::kSentinel)
-                        if (give_up(sc, ++idle)) {
-                            raw = VBits<T>::kQNaN;
-                            break;
-                        }
-                    xv = as_val(raw);
-                } else {
-                    xv = wait_value<T, SYNC>(x, sc, c);
-                }
-                s = fmac(val[j], xv, s);
-            }
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-            if (lane == 0) {
-                const T xi = (b[row] - s) / val[d];
-                if (win.len) {
-                    U o = as_bits(xi);
-                    if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;
-                    win.store(row, 0, o);
-                }
-                publish<T, SYNC>(x, sc, row, xi);
-            }
+            wave_row<T, SYNC>(rp, ci, val, b, x, sc, win, r0);
         });
 }
 

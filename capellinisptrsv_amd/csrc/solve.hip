@@ -350,7 +350,10 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::tuple<int, int, int>> g_tune_cache;
+struct TuneChoice {
+    int algo, sync, border, bpc, lds;
+};
+std::map<TuneKey, TuneChoice> g_tune_cache;
 
 unsigned long long fingerprint(const std::vector<int> &rp)
 {
@@ -387,6 +390,19 @@ int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, hipSt
     return rc;
 }
 
+void apply_choice(cap_plan p, const TuneChoice &c)
+{
+    p->algo = c.algo;
+    p->sync = c.sync;
+    p->bpc = c.bpc;
+    p->lds = c.lds;
+}
+
+// Candidates are timed on the plan's own matrix (b = 0; the dependency
+// structure, not the values, sets the time) and the fastest is cached per
+// matrix shape: thread-per-row with and without the LDS window and at 1 or 2
+// workgroups per CU, the flag protocol, and mix at three thresholds when the
+// rows are long enough for wavefront mode to matter.
 int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
     TuneKey key{p->n, p->nnz, p->vt, 1, fingerprint(rp_host)};
@@ -394,26 +410,31 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            p->algo = std::get<0>(it->second);
-            p->sync = std::get<1>(it->second);
-            if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, std::get<2>(it->second));
+            apply_choice(p, it->second);
+            if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, it->second.border);
             return CAP_OK;
         }
     }
-    // candidates: thread-per-row in both protocols, mix at three thresholds
     void *b = nullptr, *x = nullptr;
     CAP_HIP_TRY(hipMalloc(&b, (size_t)p->n * p->vbytes + 8));
     CAP_HIP_TRY(hipMalloc(&x, (size_t)p->n * p->vbytes + 8));
     CAP_HIP_TRY(hipMemsetAsync(b, 0, (size_t)p->n * p->vbytes, st));
-    struct Cand { int algo, sync, border; };
-    std::vector<Cand> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0}, {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0}};
+    std::vector<TuneChoice> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 0},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 1},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0}};
     const double avg = (double)p->nnz / std::max(1, p->n);
     for (int bd : {8, 16, 32})
-        if (avg * 4 >= bd) cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd});
+        if (avg * 4 >= bd) {
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0});
+        }
     float best = 1e30f;
-    Cand pick = cands[0];
+    TuneChoice pick = cands[0];
     int rc = CAP_OK;
-    for (const Cand &c : cands) {
+    for (const TuneChoice &c : cands) {
+        apply_choice(p, c);
         if (c.algo == CAP_ALGO_MIX && (rc = build_mix(p, rp_host, c.border)) != CAP_OK) break;
         float t = 0;
         if ((rc = time_candidate(p, c.algo, c.sync, b, x, st, &t)) != CAP_OK) break;
@@ -425,11 +446,10 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     (void)hipFree(b);
     (void)hipFree(x);
     if (rc != CAP_OK) return rc;
-    p->algo = pick.algo;
-    p->sync = pick.sync;
+    apply_choice(p, pick);
     if (pick.algo == CAP_ALGO_MIX) CAP_TRY(build_mix(p, rp_host, pick.border));
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune_cache[key] = std::make_tuple(pick.algo, pick.sync, pick.border);
+    g_tune_cache[key] = pick;
     return CAP_OK;
 }
 
@@ -525,35 +545,34 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     return CAP_OK;
 }
 
-// Reversed transpose of L for the gather adjoint: row i' = n-1-c holds
-// column c of L with rows r mapped to n-1-r in ascending order, i.e. the
-// column's entries from last to first, diagonal last.  Solving this lower
-// triangular matrix with the write-first kernel accumulates each y_c over
-// rows in descending order: the executor's BACKWARD order
-// (serialref.h:75-100), so the gather adjoint is deterministic.
+// Reversed transpose of L for the gather adjoint: row i' = n-1-c lists the
+// entries of column c of L, rows r mapped to n-1-r, in the order the
+// executor's BACKWARD mode adds them into left_sum[c] (serialref.h:75-100):
+// rows of L in descending order, entries of a row in stored order (which
+// keeps duplicate columns of one row in their stored order).  The diagonal
+// (row c) comes last.  Solving this lower triangular matrix with the
+// write-first kernel makes the gather adjoint deterministic and bit-equal to
+// that order.
 int ensure_reversed_transpose(cap_plan p)
 {
     if (p->rpT) return CAP_OK;
     const int n = p->n, nnz = p->nnz;
-    std::vector<int> rp((size_t)n + 1), ci((size_t)nnz), cp((size_t)n + 1), cr((size_t)nnz);
-    std::vector<double> v((size_t)nnz), cv((size_t)nnz);
+    std::vector<int> rp((size_t)n + 1), ci((size_t)nnz);
+    std::vector<double> v((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(rp.data(), p->rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
     CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
-    cap_transpose_d(n, n, nnz, rp.data(), ci.data(), v.data(), cr.data(), cp.data(), cv.data());
-    std::vector<int> rT((size_t)n + 1), cT((size_t)nnz);
+    std::vector<int> rT((size_t)n + 1, 0), cT((size_t)nnz);
     std::vector<double> vT((size_t)nnz);
-    size_t o = 0;
-    rT[0] = 0;
-    for (int ip = 0; ip < n; ++ip) {
-        const int c = n - 1 - ip;
-        for (int j = cp[c + 1] - 1; j >= cp[c]; --j) {
-            cT[o] = n - 1 - cr[j];
-            vT[o] = cv[j];
-            ++o;
+    for (int j = 0; j < nnz; ++j) rT[(size_t)(n - 1 - ci[j]) + 1]++;
+    for (int r = 0; r < n; ++r) rT[r + 1] += rT[r];
+    std::vector<int> fill(rT.begin(), rT.end() - 1);
+    for (int i = n - 1; i >= 0; --i)
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int o = fill[n - 1 - ci[j]]++;
+            cT[o] = n - 1 - i;
+            vT[o] = v[j];
         }
-        rT[ip + 1] = (int)o;
-    }
     CAP_TRY(dev_alloc(&p->rpT, (size_t)n + 1));
     CAP_TRY(dev_alloc(&p->ciT, (size_t)nnz));
     CAP_TRY(dev_alloc(&p->vT, (size_t)nnz));
@@ -634,6 +653,15 @@ int cap_plan_status(cap_plan p, void *stream)
     CAP_HIP_TRY(hipMemcpy(&e, p->err, sizeof(int), hipMemcpyDeviceToHost));
     if (e) CAP_HIP_TRY(hipMemset(p->err, 0, sizeof(int)));
     return e ? CAP_ERR_WATCHDOG : CAP_OK;
+}
+
+int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint)
+{
+    if (!p) return CAP_ERR_ARG;
+    if (blocks_per_cu) *blocks_per_cu = p->bpc;
+    if (lds_window) *lds_window = p->lds;
+    if (adjoint) *adjoint = p->adjoint_mode;
+    return CAP_OK;
 }
 
 int cap_plan_destroy(cap_plan p)

@@ -66,8 +66,11 @@ class Plan:
         wo, eo = c_double(), c_double()
         N.check(N.lib().cap_plan_info(self._h, byref(a), byref(s), byref(b), byref(ln), byref(wo), byref(eo)),
                 "cap_plan_info")
+        bpc, lds, adj = c_int(), c_int(), c_int()
+        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj)), "cap_plan_tuning")
         return dict(algo=N.ALGO_NAME[a.value], sync=N.SYNC_NAME[s.value], border=b.value, mix_len=ln.value,
-                    warp_occupy=wo.value, element_occupy=eo.value)
+                    warp_occupy=wo.value, element_occupy=eo.value, blocks_per_cu=bpc.value,
+                    lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"))
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

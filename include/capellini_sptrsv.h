@@ -116,6 +116,8 @@ int cap_plan_status(cap_plan plan, void *stream);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table. */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
+/* Grid and window choices the plan uses (after autotuning). */
+int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint);
 int cap_plan_destroy(cap_plan plan);
 
 /* Adjoint of x = L^{-1} b: grad_b = L^{-T} g and grad_val[k] =

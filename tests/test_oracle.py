@@ -285,3 +285,25 @@ def test_planted_mistakes_are_caught():
         if d > a:
             v_sw[a], v_sw[d] = v_sw[d], v_sw[a]
     assert rel_maxnorm(oracle.csr_forward(rp, ci, v_sw, b), xd) > 1e-6
+
+
+@pytest.mark.parametrize("dup", [False, True])
+def test_reversed_transpose_forward_equals_backward_executor(dup):
+    """The gather adjoint's identity: a forward solve on the reversed transpose
+    (rows of L visited in descending order, entries in stored order) is the
+    executor's BACKWARD mode on L read as CSC of L^T, bit for bit."""
+    rp, ci, v = random_lower(2500, max_off=5, band=12, seed=31, dup=dup)
+    n, nnz = len(rp) - 1, len(ci)
+    cnt = np.bincount(n - 1 - ci, minlength=n)
+    rT = np.r_[0, np.cumsum(cnt)].astype(np.int32)
+    fill = rT[:-1].astype(np.int64).copy()
+    cT, vT = np.zeros(nnz, np.int32), np.zeros(nnz)
+    for i in range(n - 1, -1, -1):
+        for j in range(rp[i], rp[i + 1]):
+            o = fill[n - 1 - ci[j]]
+            fill[n - 1 - ci[j]] += 1
+            cT[o], vT[o] = n - 1 - i, v[j]
+    g = np.random.default_rng(4).uniform(-1, 1, size=n)
+    for fused in (False, True):
+        y = oracle.csr_forward(rT, cT, vT, g[::-1].copy(), fused=fused)[::-1]
+        assert np.array_equal(y, oracle.adjoint(rp, ci, v, g, fused=fused))
