@@ -1,0 +1,56 @@
+"""Small GPU workloads for rocprofv3 (kernel traces and counter runs).
+
+    python -m capellinisptrsv_amd.profile_targets solve [--steps N] [--algo auto]
+        the wide-level n = 2,000,000 FP64 solve of bench.py, N reset+solve steps
+    python -m capellinisptrsv_amd.profile_targets copy [--mib 1024]
+        out = in * 1 over a float64 buffer of the given size: a kernel that
+        reads and writes exactly mib MiB each, to calibrate FETCH_SIZE /
+        WRITE_SIZE against known bytes
+    python -m capellinisptrsv_amd.profile_targets adjoint [--steps N]
+        forward + gather adjoint of the n = 1,000,000 level matrix
+"""
+import argparse
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched"])
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--algo", default="auto")
+    ap.add_argument("--mib", type=int, default=1024)
+    a = ap.parse_args(argv)
+
+    import torch
+
+    from capellinisptrsv_amd import Plan, api
+
+    dev = torch.device("cuda:0")
+    if a.target == "copy":
+        n = a.mib * 1024 * 1024 // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(a.steps):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(f"copy: {a.steps} x {a.mib} MiB read + {a.mib} MiB written", flush=True)
+        return 0
+    cfg = {"solve": 1, "adjoint": 4}[a.target] if a.target != "batched" else None
+    rp, ci, v = api.make_config(cfg, seed=0)
+    n = len(rp) - 1
+    d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
+    b = torch.from_numpy(api.uniform(1, n)).to(dev)
+    x = torch.empty_like(b)
+    plan = Plan(*d, algo=a.algo) if a.target == "solve" else Plan(*d, algo="wrtfst", sync="value")
+    for _ in range(a.steps):
+        plan.solve(b, out=x)
+        if a.target == "adjoint":
+            plan.adjoint(x, x)
+    torch.cuda.synchronize()
+    assert plan.status() == 0
+    print(f"{a.target}: n={n} nnz={len(ci)} plan={plan.info()} steps={a.steps}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
