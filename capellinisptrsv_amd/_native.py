@@ -31,6 +31,8 @@ ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4}
 ALGO_NAME = {v: k for k, v in ALGO.items()}
 SYNC = {"auto": 0, "value": 1, "flag": 2}
 ADJOINT = {"auto": 0, "scatter": 1, "gather": 2}
+READINESS = {"auto": 0, "staged": 1, "in_place": 2}
+READINESS_NAME = {v: k for k, v in READINESS.items()}
 SYNC_NAME = {v: k for k, v in SYNC.items()}
 
 
@@ -48,7 +50,8 @@ class CapError(RuntimeError):
 class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
-                ("adjoint", c_int), ("lds_window", c_int), ("reserved", c_int * 4)]
+                ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
+                ("reserved", c_int * 3)]
 
 
 class cap_gen_params(Structure):
@@ -73,7 +76,7 @@ SIGNATURES = {
     "cap_plan_reset": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
-    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI]),
+    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "cap_plan_solve_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, PI, PD, c_void_p]),
@@ -176,5 +179,7 @@ def default_opts(**kw) -> cap_opts:
             v = SYNC[v]
         if k == "adjoint" and isinstance(v, str):
             v = ADJOINT[v]
+        if k == "readiness" and isinstance(v, str):
+            v = READINESS[v]
         setattr(o, k, int(v))
     return o

@@ -10,18 +10,20 @@
 //    waits for belongs to a workgroup that is running or finished; that is
 //    what makes spinning safe without a grid barrier.
 //  * Readiness of x[c] is observed either through the value itself (SYNC
-//    VALUE: x starts as a signalling-NaN sentinel that arithmetic can never
-//    produce; a relaxed agent-scope 32/64-bit atomic load both tests and
-//    fetches it) or through an int flag per row holding the solve's epoch
-//    (SYNC FLAG: release store / acquire fence, agent scope; the epoch makes
-//    resets between solves unnecessary).
+//    VALUE: the readiness array -- x, or a staged copy in uncached memory --
+//    starts as a signalling-NaN sentinel that arithmetic can never produce; a
+//    relaxed agent-scope 32/64-bit atomic load both tests and fetches it) or
+//    through an int flag per row holding the solve's epoch (SYNC FLAG:
+//    release store / acquire fence, agent scope; the epoch makes resets
+//    between solves unnecessary).
 //  * Thread-per-row kernels use the write-first loop of
 //    spts_syncfree_cuda_csr_wrt.h:37-55: a lane publishes x[i] from inside the
 //    loop in which it polls, so lanes of one wavefront that depend on each
-//    other always make progress.  Each lane polls a window of up to WIN
-//    dependencies at once and consumes the ready prefix in stored order, so
-//    the accumulation order is the sequential one (bit-exact against the
-//    oracle's fused forward substitution).
+//    other always make progress.  Each lane polls a small window of its next
+//    dependencies at once (two in the value protocol, kWin with flags) and
+//    consumes the ready prefix in stored order, so the accumulation order is
+//    the sequential one (bit-exact against the oracle's fused forward
+//    substitution).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -74,12 +76,31 @@ __device__ __forceinline__ void atomic_add_agent(V *p, V v)
 //  flags/epoch: SYNC FLAG only.  spin_limit: polls without progress after
 //  which a lane gives up, publishes NaN (so every dependent row, and with it
 //  the whole grid, still finishes) and sets bit 0 of *err; <= 0 = unbounded.
+//  ready: SYNC VALUE only -- the array that is polled and published (same
+//  shape as x, sentinel-filled before the solve); results are also stored
+//  into x with plain stores.  nullptr = x itself is polled.  The library
+//  places it in uncached device memory, where a poll goes straight to the
+//  memory side instead of through an L2 that agent-scope loads must miss.
 struct SyncCtx {
     int *flags;
     int epoch;
     int spin_limit;
     int *err;
+    void *ready;
 };
+
+template <typename T>
+__device__ __forceinline__ typename VBits<T>::U *ready_array(const SyncCtx &sc, T *x)
+{
+    typedef typename VBits<T>::U U;
+    return sc.ready ? reinterpret_cast<U *>(sc.ready) : reinterpret_cast<U *>(x);
+}
+template <typename T>
+__device__ __forceinline__ const typename VBits<T>::U *ready_array(const SyncCtx &sc, const T *x)
+{
+    typedef typename VBits<T>::U U;
+    return sc.ready ? reinterpret_cast<const U *>(sc.ready) : reinterpret_cast<const U *>(x);
+}
 
 __device__ __forceinline__ bool give_up(const SyncCtx &sc, int idle)
 {
@@ -174,7 +195,12 @@ template <typename T, int SYNC>
 __device__ __forceinline__ void publish(T *x, const SyncCtx &sc, long long o, T xi)
 {
     if (SYNC == 1) {
-        publish_value(x, o, xi);
+        if (sc.ready) {
+            publish_value(reinterpret_cast<T *>(sc.ready), o, xi);
+            x[o] = xi;
+        } else {
+            publish_value(x, o, xi);
+        }
     } else {
         x[o] = xi;
         __hip_atomic_store(sc.flags + o, sc.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -190,8 +216,9 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
     const U *xu = reinterpret_cast<const U *>(x);
     int idle = 0;
     if (SYNC == 1) {
+        const U *xr = ready_array(sc, x);
         U raw;
-        while ((raw = ld_agent(xu + o)) == VBits<T>::kSentinel)
+        while ((raw = ld_agent(xr + o)) == VBits<T>::kSentinel)
             if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
         return as_val(raw);
     } else {
@@ -209,7 +236,7 @@ __device__ __forceinline__ bool try_value(const T *x, const SyncCtx &sc, long lo
     typedef typename VBits<T>::U U;
     const U *xu = reinterpret_cast<const U *>(x);
     if (SYNC == 1) {
-        const U raw = ld_agent(xu + o);
+        const U raw = ld_agent(ready_array(sc, x) + o);
         *out = as_val(raw);
         return raw != VBits<T>::kSentinel;
     }
@@ -290,6 +317,81 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
     }
 }
 
+// Value protocol form of wrtfst_row: a window of the next two dependencies
+// (entries j and j+1) is polled at once and slides by the number consumed,
+// so most rows (one or two off-diagonals) are covered by a single window and
+// the loop body stays short.  Consumption is in stored order (fma chain).
+template <typename T>
+__device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, const T *__restrict__ val,
+                                                 const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
+                                                 int i, int j, int d, int k, int q, T s, int n_rev,
+                                                 const LdsWin<typename VBits<T>::U> &win)
+{
+    typedef typename VBits<T>::U U;
+    constexpr U kS = VBits<T>::kSentinel;
+    U *xr = ready_array(sc, x);
+    int c0 = 0, c1 = 0;
+    T v0 = (T)0, v1 = (T)0;
+    U x0 = kS, x1 = kS;
+    if (j < d) {
+        c0 = n_rev ? n_rev - 1 - ci[j] : ci[j];
+        v0 = val[j];
+    }
+    if (j + 1 < d) {
+        c1 = n_rev ? n_rev - 1 - ci[j + 1] : ci[j + 1];
+        v1 = val[j + 1];
+    }
+    int idle = 0;
+    bool done = false;
+    do {  // wave-uniform exit: see wrtfst_row
+        if (!done) {
+            bool prog = false;
+            if (j < d && x0 == kS) x0 = win.has(c0) ? win.load(c0, q) : ld_agent(xr + (long long)c0 * k + q);
+            if (j + 1 < d && x1 == kS) x1 = win.has(c1) ? win.load(c1, q) : ld_agent(xr + (long long)c1 * k + q);
+            if (j < d && x0 != kS) {
+                s = fmac(v0, as_val(x0), s);
+                ++j;
+                prog = true;
+                if (j < d && x1 != kS) {  // both consumed: a fresh window
+                    s = fmac(v1, as_val(x1), s);
+                    ++j;
+                    x0 = kS;
+                    if (j < d) {
+                        c0 = n_rev ? n_rev - 1 - ci[j] : ci[j];
+                        v0 = val[j];
+                    }
+                } else {  // slide by one
+                    c0 = c1;
+                    v0 = v1;
+                    x0 = x1;
+                }
+                x1 = kS;
+                if (j + 1 < d) {
+                    c1 = n_rev ? n_rev - 1 - ci[j + 1] : ci[j + 1];
+                    v1 = val[j + 1];
+                }
+            }
+            if (prog) {
+                idle = 0;
+            } else if (j < d && give_up(sc, ++idle)) {
+                s = as_val(VBits<T>::kQNaN);  // poison this row; it still publishes below
+                j = d;
+            }
+            if (j >= d) {  // write first: publish inside the polling loop
+                const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
+                const T xi = (b[o] - s) / val[d];
+                U u = as_bits(xi);
+                if (u == kS) u = VBits<T>::kQNaN;
+                if (win.len) win.store(i, q, u);
+                st_agent(xr + o, u);
+                if (sc.ready) x[o] = xi;
+                asm volatile("" ::: "memory");
+                done = true;
+            }
+        }
+    } while (__any(!done));
+}
+
 // Solves row i (off-diagonal entries [j, d), diagonal at d) for right-hand
 // side q of k (x and b interleaved: x[row*k + q]) and publishes it.  `s`
 // carries a partial sum when the caller already consumed a prefix (two-part).
@@ -301,6 +403,11 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                                            int i, int j, int d, int k, int q, T s, int n_rev = 0,
                                            LdsWin<typename VBits<T>::U> win = {nullptr, 0, 0, 1})
 {
+    if constexpr (SYNC == 1) {
+        wrtfst_row_value<T>(ci, val, b, x, sc, i, j, d, k, q, s, n_rev, win);
+        return;
+    }
+    // flag protocol: a fixed window of up to kWin entries, refilled when consumed
     typedef typename VBits<T>::U U;
     U *xu = reinterpret_cast<U *>(x);
     int cw[kWin];
@@ -323,33 +430,22 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
     do {
         if (!done) {
             if (used < cnt) {
-                if (SYNC == 1) {
+                int fl[kWin];
+                bool any = false;
+#pragma unroll
+                for (int t = 0; t < kWin; ++t)
+                    fl[t] = (t >= used && t < cnt && !rdy[t]) ? ld_agent(sc.flags + (long long)cw[t] * k + q) : 0;
+#pragma unroll
+                for (int t = 0; t < kWin; ++t)
+                    if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) any = true;
+                if (any) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
                     for (int t = 0; t < kWin; ++t)
-                        if (t >= used && t < cnt && !rdy[t])
-                            xw[t] = win.has(cw[t]) ? win.load(cw[t], q) : ld_agent(xu + (long long)cw[t] * k + q);
-#pragma unroll
-                    for (int t = 0; t < kWin; ++t)
-                        if (t >= used && t < cnt && !rdy[t]) rdy[t] = xw[t] != VBits<T>::kSentinel;
-                } else {
-                    int fl[kWin];
-                    bool any = false;
-#pragma unroll
-                    for (int t = 0; t < kWin; ++t)
-                        fl[t] = (t >= used && t < cnt && !rdy[t]) ? ld_agent(sc.flags + (long long)cw[t] * k + q)
-                                                                 : 0;
-#pragma unroll
-                    for (int t = 0; t < kWin; ++t)
-                        if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) any = true;
-                    if (any) {
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-#pragma unroll
-                        for (int t = 0; t < kWin; ++t)
-                            if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) {
-                                xw[t] = xu[(long long)cw[t] * k + q];
-                                rdy[t] = true;
-                            }
-                    }
+                        if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) {
+                            xw[t] = xu[(long long)cw[t] * k + q];
+                            rdy[t] = true;
+                        }
                 }
                 // consume the ready prefix in stored order
                 const int before = used;
@@ -384,13 +480,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
             }
             if (used == cnt && j >= d) {  // write first: publish inside the polling loop
                 const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
-                const T xi = (b[o] - s) / val[d];
-                if (SYNC == 1 && win.len) {
-                    U u = as_bits(xi);
-                    if (u == VBits<T>::kSentinel) u = VBits<T>::kQNaN;
-                    win.store(i, q, u);
-                }
-                publish<T, SYNC>(x, sc, o, xi);
+                publish<T, SYNC>(x, sc, o, (b[o] - s) / val[d]);
                 asm volatile("" ::: "memory");
                 done = true;
             }

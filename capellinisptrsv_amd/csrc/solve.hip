@@ -61,6 +61,10 @@ struct cap_plan_s {
     int spin_limit = kSpinLimitDefault;
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
+    // value protocol: staged readiness array (uncached pool block) or x itself
+    int staged = 1;
+    void *ready = nullptr;
+    size_t ready_bytes = 0;
     // mix
     int *unit = nullptr;
     int n_units = 0, mix_len = 0;
@@ -132,9 +136,76 @@ int dev_alloc(P **p, size_t count)
         if (cap_rc_ != CAP_OK) return cap_rc_; \
     } while (0)
 
+// ------------------------------------------------- uncached readiness pool
+// Blocks of uncached device memory for staged readiness arrays.  Uncached
+// memory that a kernel has read is never freed before the process ends
+// (later allocations have been seen to read wrong values after such a free),
+// so blocks return to this pool when a plan lets go of them and are handed to
+// later plans on the same device (best fit).
+struct ReadyBlock {
+    int device;
+    void *ptr;
+    size_t bytes;
+    bool used;
+};
+std::mutex g_ready_mu;
+std::vector<ReadyBlock> g_ready_pool;
+
+int ready_acquire(int device, size_t bytes, void **out, size_t *got)
+{
+    std::lock_guard<std::mutex> g(g_ready_mu);
+    ReadyBlock *best = nullptr;
+    for (ReadyBlock &r : g_ready_pool)
+        if (!r.used && r.device == device && r.bytes >= bytes && (!best || r.bytes < best->bytes)) best = &r;
+    if (!best) {
+        const size_t grain = (size_t)2 << 20;
+        const size_t sz = (bytes + grain - 1) / grain * grain;
+        void *ptr = nullptr;
+        if (hipExtMallocWithFlags(&ptr, sz, hipDeviceMallocUncached) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return CAP_ERR_ALLOC;
+        }
+        g_ready_pool.push_back({device, ptr, sz, false});
+        best = &g_ready_pool.back();
+    }
+    best->used = true;
+    *out = best->ptr;
+    *got = best->bytes;
+    return CAP_OK;
+}
+
+void ready_release(void *ptr)
+{
+    if (!ptr) return;
+    std::lock_guard<std::mutex> g(g_ready_mu);
+    for (ReadyBlock &r : g_ready_pool)
+        if (r.ptr == ptr) r.used = false;
+}
+
+// The plan's staged readiness array holds at least `bytes`.  Growing it
+// waits for the device (kernels in flight may still poll the old block) and
+// drops the captured refinement graph, which has the old block's address.
+int ensure_ready(cap_plan p, size_t bytes)
+{
+    if (p->ready_bytes >= bytes) return CAP_OK;
+    if (p->ready) {
+        CAP_HIP_TRY(hipDeviceSynchronize());
+        if (p->refine_graph) CAP_HIP_TRY(hipGraphExecDestroy(p->refine_graph));
+        p->refine_graph = nullptr;
+        ready_release(p->ready);
+        p->ready = nullptr;
+        p->ready_bytes = 0;
+    }
+    return ready_acquire(p->device, bytes, &p->ready, &p->ready_bytes);
+}
+
 void plan_free(cap_plan p)
 {
     if (!p) return;
+    if (p->ready) {
+        (void)hipDeviceSynchronize();  // no kernel of this plan may still poll the block
+        ready_release(p->ready);
+    }
     if (p->owns_matrix) {
         (void)hipFree(p->rp);
         (void)hipFree(p->ci);
@@ -183,14 +254,20 @@ int ensure_flags(cap_plan p, long long len, hipStream_t st)
     return CAP_OK;
 }
 
-// Per-solve reset (tickets; sentinel fill of x in the value protocol).
+// Per-solve reset (tickets; sentinel fill of the readiness array -- staged
+// or x -- in the value protocol).
 template <typename T>
 int launch_prepare(cap_plan p, int sync, void *x, long long count, hipStream_t st)
 {
     typedef typename VBits<T>::U U;
     if (sync == CAP_SYNC_VALUE) {
+        void *target = x;
+        if (p->staged) {
+            CAP_TRY(ensure_ready(p, sizeof(U) * (size_t)count));
+            target = p->ready;
+        }
         const unsigned nb = std::max(1u, std::min(blocks_for(count), 2048u));
-        k_prepare<U><<<nb, kBlock, 0, st>>>((U *)x, count, VBits<T>::kSentinel, p->ticket, kTickets);
+        k_prepare<U><<<nb, kBlock, 0, st>>>((U *)target, count, VBits<T>::kSentinel, p->ticket, kTickets);
     } else {
         CAP_TRY(ensure_flags(p, count, st));
         if (++p->epoch == 0x7fffffff) {  // wrap: clear the flags once
@@ -205,12 +282,28 @@ int launch_prepare(cap_plan p, int sync, void *x, long long count, hipStream_t s
 }
 
 // The solve kernel itself (after launch_prepare).
+// Protocol state of a launch over `count` values; nullptr-ready unless the
+// value protocol is staged.  A staged array smaller than count means the
+// launch was not preceded by its reset.
+template <typename T>
+int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
+{
+    *sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    if (sync == CAP_SYNC_VALUE && p->staged) {
+        if (!p->ready || p->ready_bytes < sizeof(typename VBits<T>::U) * (size_t)count) return CAP_ERR_ARG;
+        sc->ready = p->ready;
+    }
+    return CAP_OK;
+}
+
 template <typename T>
 int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x, int rhs, hipStream_t st)
 {
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
     const int n = p->n;
     if (n == 0) return CAP_OK;
+    SyncCtx sc;
+    if (algo != CAP_ALGO_CSC) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
+    else sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     if (algo == CAP_ALGO_WRTFST) {
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
@@ -351,7 +444,7 @@ struct TuneKey {
 };
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int algo, sync, border, bpc, lds;
+    int algo, sync, border, bpc, lds, staged;
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 
@@ -366,7 +459,9 @@ unsigned long long fingerprint(const std::vector<int> &rp)
     return h;
 }
 
-int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, hipStream_t st, float *ms)
+// Best of three timed solves after one warm-up; stops early once a run is
+// 1.5x slower than `bound` (the best candidate so far).
+int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, hipStream_t st, float bound, float *ms)
 {
     hipEvent_t e0, e1;
     CAP_HIP_TRY(hipEventCreate(&e0));
@@ -382,7 +477,12 @@ int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, hipSt
         (void)hipEventSynchronize(e1);
         float t = 0;
         (void)hipEventElapsedTime(&t, e0, e1);
-        if (r > 0) best = std::min(best, t);
+        if (r > 0) {
+            best = std::min(best, t);
+        } else if (t > 1.5f * bound) {  // the warm-up alone rules it out
+            best = t;
+            break;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -396,13 +496,16 @@ void apply_choice(cap_plan p, const TuneChoice &c)
     p->sync = c.sync;
     p->bpc = c.bpc;
     p->lds = c.lds;
+    p->staged = c.staged;
 }
 
 // Candidates are timed on the plan's own matrix (b = 0; the dependency
 // structure, not the values, sets the time) and the fastest is cached per
-// matrix shape: thread-per-row with and without the LDS window and at 1 or 2
-// workgroups per CU, the flag protocol, and mix at three thresholds when the
-// rows are long enough for wavefront mode to matter.
+// matrix shape: thread-per-row with and without the LDS window, at 1 or 2
+// workgroups per CU, with the staged or the in-place readiness array, the
+// flag protocol, and mix at three thresholds when the rows are long enough
+// for wavefront mode to matter.  A plan created with readiness in place keeps
+// it in every candidate.
 int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
     TuneKey key{p->n, p->nnz, p->vt, 1, fingerprint(rp_host)};
@@ -419,16 +522,18 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     CAP_HIP_TRY(hipMalloc(&b, (size_t)p->n * p->vbytes + 8));
     CAP_HIP_TRY(hipMalloc(&x, (size_t)p->n * p->vbytes + 8));
     CAP_HIP_TRY(hipMemsetAsync(b, 0, (size_t)p->n * p->vbytes, st));
-    std::vector<TuneChoice> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 0},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 1},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0}};
+    const int stg = p->staged;
+    std::vector<TuneChoice> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, stg},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 0, stg},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1, stg},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 1, stg},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0, 0}};
+    if (stg) cands.push_back({CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, 0});
     const double avg = (double)p->nnz / std::max(1, p->n);
     for (int bd : {8, 16, 32})
         if (avg * 4 >= bd) {
-            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0});
-            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg});
         }
     float best = 1e30f;
     TuneChoice pick = cands[0];
@@ -437,7 +542,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         apply_choice(p, c);
         if (c.algo == CAP_ALGO_MIX && (rc = build_mix(p, rp_host, c.border)) != CAP_OK) break;
         float t = 0;
-        if ((rc = time_candidate(p, c.algo, c.sync, b, x, st, &t)) != CAP_OK) break;
+        if ((rc = time_candidate(p, c.algo, c.sync, b, x, st, best, &t)) != CAP_OK) break;
         if (t < best) {
             best = t;
             pick = c;
@@ -523,6 +628,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         p->lds = o.lds_window > 0;
+        p->staged = o.readiness != CAP_READY_IN_PLACE;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
         p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
@@ -655,12 +761,13 @@ int cap_plan_status(cap_plan p, void *stream)
     return e ? CAP_ERR_WATCHDOG : CAP_OK;
 }
 
-int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint)
+int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint, int *readiness)
 {
     if (!p) return CAP_ERR_ARG;
     if (blocks_per_cu) *blocks_per_cu = p->bpc;
     if (lds_window) *lds_window = p->lds;
     if (adjoint) *adjoint = p->adjoint_mode;
+    if (readiness) *readiness = p->staged ? CAP_READY_STAGED : CAP_READY_IN_PLACE;
     return CAP_OK;
 }
 
@@ -681,8 +788,9 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     if (n == 0) return CAP_OK;
     if (p->adjoint_mode == CAP_ADJOINT_GATHER) {
         CAP_TRY(ensure_reversed_transpose(p));
-        SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
         CAP_TRY(launch_prepare<double>(p, CAP_SYNC_VALUE, grad_b, n, st));
+        SyncCtx sc;
+        CAP_TRY(sync_ctx<double>(p, CAP_SYNC_VALUE, n, &sc));
         k_wrtfst<double, 1, 0><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
             p->rpT, p->ciT, p->vT, (const double *)g, (double *)grad_b, sc, n, 1, 0, p->ticket, n);
         if (grad_val)
@@ -702,7 +810,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     CAP_HIP_TRY(hipMemcpyAsync(p->cnt, p->cnt0, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
     k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
                                                                     p->ticket, kTickets);
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err};
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     k_adjoint<<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
         p->rp, p->ci, (const double *)p->val, (const double *)g, (const double *)x, p->cnt, p->adj_ls,
         (double *)grad_b, (double *)grad_val, n, sc, p->ticket);
@@ -759,6 +867,9 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
     if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
     CAP_TRY(ensure_refine(p, rhs, st));
+    // the staged readiness array must exist before capture (no allocation
+    // inside the captured steps, whose kernels bake in its address)
+    if (p->staged) CAP_TRY(ensure_ready(p, sizeof(float) * (size_t)p->n * rhs));
     // The first two steps (solve from b, one correction) are one graph:
     // captured once per (b, x, rhs), replayed on later calls.
     if (!p->refine_graph || p->graph_b != b || p->graph_x != x || p->graph_rhs != rhs) {

@@ -42,7 +42,8 @@ class Plan:
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
-                 blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, stream=None):
+                 blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
+                 stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -54,7 +55,7 @@ class Plan:
         self.dtype = values.dtype
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
-                              lds_window=int(lds_window))
+                              lds_window=int(lds_window), readiness=readiness)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -66,11 +67,13 @@ class Plan:
         wo, eo = c_double(), c_double()
         N.check(N.lib().cap_plan_info(self._h, byref(a), byref(s), byref(b), byref(ln), byref(wo), byref(eo)),
                 "cap_plan_info")
-        bpc, lds, adj = c_int(), c_int(), c_int()
-        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj)), "cap_plan_tuning")
+        bpc, lds, adj, rdy = c_int(), c_int(), c_int(), c_int()
+        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj), byref(rdy)),
+                "cap_plan_tuning")
         return dict(algo=N.ALGO_NAME[a.value], sync=N.SYNC_NAME[s.value], border=b.value, mix_len=ln.value,
                     warp_occupy=wo.value, element_occupy=eo.value, blocks_per_cu=bpc.value,
-                    lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"))
+                    lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"),
+                    readiness=N.READINESS_NAME.get(rdy.value, "?"))
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

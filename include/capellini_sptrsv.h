@@ -64,10 +64,21 @@ extern "C" {
 #define CAP_ADJOINT_GATHER 2  /* write-first solve on a reversed transpose of L
                                  built once per plan; deterministic */
 
+/* readiness array of the value protocol (cap_opts.readiness).
+ * CAP_READY_STAGED polls a plan-owned array in uncached device memory and
+ * also stores each result into x; the array comes from a per-process pool
+ * that is never freed (uncached memory read by a kernel stays allocated
+ * until the process ends) and is reused by later plans.
+ * CAP_READY_IN_PLACE polls x itself. */
+#define CAP_READY_AUTO 0
+#define CAP_READY_STAGED 1
+#define CAP_READY_IN_PLACE 2
+
 /* readiness protocol (cap_opts.sync) */
 #define CAP_SYNC_AUTO 0
-#define CAP_SYNC_VALUE 1 /* x itself is the flag: x starts as a signalling-NaN
-                            sentinel; relaxed agent-scope 64/32-bit atomics */
+#define CAP_SYNC_VALUE 1 /* the value is the flag: x (or the staged readiness
+                            array) starts as a signalling-NaN sentinel;
+                            relaxed agent-scope 64/32-bit atomics */
 #define CAP_SYNC_FLAG 2  /* separate int flags published with a release store
                             and read with acquire loads (agent scope) */
 
@@ -85,7 +96,9 @@ typedef struct cap_opts {
     int adjoint;       /* CAP_ADJOINT_* (0 = gather) */
     int lds_window;    /* 1 = thread-per-row kernels resolve dependencies on
                           rows of their own workgroup in LDS (value protocol) */
-    int reserved[4];
+    int readiness;     /* CAP_READY_* (value protocol): where readiness is
+                          polled; 0 = tuned / default staged */
+    int reserved[3];
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -116,8 +129,10 @@ int cap_plan_status(cap_plan plan, void *stream);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table. */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
-/* Grid and window choices the plan uses (after autotuning). */
-int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint);
+/* Grid, window and readiness choices the plan uses (after autotuning);
+ * *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE. */
+int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint,
+                    int *readiness);
 int cap_plan_destroy(cap_plan plan);
 
 /* Adjoint of x = L^{-1} b: grad_b = L^{-T} g and grad_val[k] =

@@ -19,11 +19,11 @@ FP64_TOL = 1e-12
 FP32_TOL = 1e-5
 
 
-def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0, lds=False, bpc=0):
+def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0, lds=False, bpc=0, readiness="auto"):
     import torch
     from capellinisptrsv_amd import Plan
     plan = Plan(torch.from_numpy(rp).to(cuda), torch.from_numpy(ci).to(cuda), torch.from_numpy(v).to(cuda),
-                algo=algo, sync=sync, border=border, lds_window=lds, blocks_per_cu=bpc)
+                algo=algo, sync=sync, border=border, lds_window=lds, blocks_per_cu=bpc, readiness=readiness)
     bt = torch.from_numpy(np.ascontiguousarray(b.reshape(-1, k) if k > 1 else b)).to(cuda)
     x = plan.solve(bt)
     torch.cuda.synchronize()
@@ -210,3 +210,40 @@ def test_autotuner_picks_a_correct_path(cuda):
     x, info = _solve(cuda, rp, ci, v, b, "auto", "auto")
     assert info["algo"] in ("wrtfst", "mix", "twopart")
     assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=True)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("algo,lds,k", [("wrtfst", False, 1), ("wrtfst", True, 1), ("wrtfst", False, 8),
+                                        ("twopart", False, 1), ("mix", False, 1)])
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_staged_and_in_place_readiness_bit_exact(cuda, algo, lds, k, name, mat):
+    """The staged readiness array (uncached memory, results also stored into
+    x) and polling x in place give the oracle's bits, and the plan reports
+    the choice it was given."""
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(11, n * k)
+    ref = np.stack([oracle.csr_forward(rp, ci, v, b.reshape(n, k)[:, q].copy(), fused=True)
+                    for q in range(k)], axis=1).reshape(-1)
+    for rd in ("staged", "in_place"):
+        x, info = _solve(cuda, rp, ci, v, b, algo, "value", k=k, border=16, lds=lds, readiness=rd)
+        assert info["readiness"] == rd
+        assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), (rd, name)
+
+
+def test_staged_readiness_reused_across_plans_and_sizes(cuda):
+    """Plans of growing and shrinking size share pool blocks one after the
+    other; every solve is still the oracle's, and x is fully written."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    for n, seed in ((5000, 1), (20000, 2), (3000, 3), (20000, 4), (40000, 5)):
+        rp, ci, v = random_lower(n, max_off=4, band=300, seed=seed)
+        b = api.uniform(seed, n)
+        plan = Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, v)), algo="wrtfst", sync="value",
+                    readiness="staged")
+        out = torch.full((n,), 12345.0, dtype=torch.float64, device=cuda)
+        for _ in range(2):
+            plan.solve(torch.from_numpy(b).to(cuda), out=out)
+        torch.cuda.synchronize()
+        assert plan.status() == 0
+        assert np.array_equal(out.cpu().numpy(), oracle.csr_forward(rp, ci, v, b, fused=True))
+        plan.close()
