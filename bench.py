@@ -275,16 +275,8 @@ def extra_workloads(args):
     # (diagonal included), band U[64, 4096], k = 16 interleaved RHS
     log("extra workload batched_256x16")
     rng = np.random.default_rng(0)
-    mats = []
-    for m in range(256):
-        nm = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
-        mats.append(api.generate("band_rowcount", n=nm, offdiag=16, band=int(rng.integers(64, 4097)), seed=1000 + m))
+    mats, row_off, rp_all, ci_all, v_all = api.make_batch(256, seed=0)
     k = 16
-    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
-    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
-    rp_all = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
-    ci_all = np.concatenate([m[1] for m in mats]).astype(np.int32)
-    v_all = np.concatenate([m[2] for m in mats])
     N = int(row_off[-1])
     bb = api.uniform(3, N * k)
     t = [torch.from_numpy(a).to(dev) for a in (row_off, rp_all, ci_all, v_all)]

@@ -27,7 +27,7 @@ CAP_ERR_NOT_CONVERGED = -12
 CAP_ERR_WATCHDOG = -13
 
 CAP_F64, CAP_F32 = 0, 1
-ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4}
+ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4, "local": 5}
 ALGO_NAME = {v: k for k, v in ALGO.items()}
 SYNC = {"auto": 0, "value": 1, "flag": 2}
 ADJOINT = {"auto": 0, "scatter": 1, "gather": 2}
@@ -77,6 +77,7 @@ SIGNATURES = {
     "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
     "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI]),
+    "cap_batch_info": (c_int, [c_void_p, PI]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "cap_plan_solve_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, PI, PD, c_void_p]),

@@ -32,6 +32,7 @@ __all__ = [
     "read_mtx", "change2tran", "get_x_b", "matrix_layer", "matrix_transposition", "matrix_warp4",
     "spts_syncfree_hip_csr_wrt", "spts_syncfree_hip_csr", "spts_syncfree_hip_mix", "sptrsv_syncfree_hip_csc",
     "sptrsv_csr_solve", "validate_csr", "generate", "uniform", "datasize", "make_config", "CONFIGS",
+    "make_batch",
 ]
 
 
@@ -254,3 +255,26 @@ def make_config(cfg: int, seed: int = 0, dtype=np.float64, scale: float = 1.0):
         else:
             c["n"] = max(2, int(c["n"] * scale))
     return generate(seed=seed, dtype=dtype, **c)
+
+
+def make_batch(n_mat: int = 256, seed: int = 0, n_lo: int = 2000, n_hi: int = 200_000, band_lo: int = 64,
+               band_hi: int = 4096, max_row_nnz: int = 16, dtype=np.float64):
+    """The batched workload: n_mat matrices with n log-uniform in [n_lo, n_hi],
+    nnz/row (diagonal included) uniform in {1..max_row_nnz} and the band
+    uniform in [band_lo, band_hi], each by the band rule of workload 0.
+
+    Returns (mats, row_off, row_ptr, col_idx, values): the list of per-matrix
+    CSR triples and their concatenation in the batch layout (global row_ptr,
+    column indices local to each matrix)."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for m in range(n_mat):
+        nm = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        band = int(rng.integers(band_lo, band_hi + 1))
+        mats.append(generate("band_rowcount", n=nm, offdiag=max_row_nnz, band=band, seed=1000 + m, dtype=dtype))
+    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
+    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
+    rp = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
+    ci = np.concatenate([m[1] for m in mats]).astype(np.int32)
+    v = np.concatenate([m[2] for m in mats])
+    return mats, row_off, rp, ci, v

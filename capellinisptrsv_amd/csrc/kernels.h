@@ -492,11 +492,16 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 // per row and right-hand side; a workgroup ticket covers kBlock >> lpr_log2
 // rows with 1 << lpr_log2 lanes per row, lane q of a row owning right-hand
 // side q (lanes with q >= k idle).
+// sched (optional, n_sched blocks): ticket t solves rows [sched[t].x,
+// sched[t].x + sched[t].y) instead of the t-th run of consecutive rows.  The
+// host orders the blocks so that every row comes after the rows it depends
+// on (each matrix of a batch keeps its row order), which keeps the
+// forward-progress argument of the ticket loop.
 template <typename T, int SYNC, int LDS>
 __global__ void __launch_bounds__(kBlock)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
-             unsigned *__restrict__ ticket, int n_rev)
+             unsigned *__restrict__ ticket, int n_rev, const int2 *__restrict__ sched, int n_sched)
 {
     typedef typename VBits<T>::U U;
     __shared__ U s_x[LDS ? kBlock : 1];
@@ -504,14 +509,20 @@ __global__ void __launch_bounds__(kBlock)
     const int r = threadIdx.x >> lpr_log2;
     const int rows = kBlock >> lpr_log2;
     ticket_loop(
-        ticket, rows, n,
+        ticket, sched ? 1 : rows, sched ? n_sched : n,
         [&](int) {
             if (LDS) s_x[threadIdx.x] = VBits<T>::kSentinel;
         },
         [&](int base) {
-            const int i = base + r;
-            LdsWin<U> win{s_x, base, (LDS && !n_rev) ? rows : 0, k};
-            if (i < n && q < k)
+            int lo = base, len = rows;
+            if (sched) {
+                const int2 blk = sched[base];
+                lo = blk.x;
+                len = blk.y;
+            }
+            const int i = lo + r;
+            LdsWin<U> win{s_x, lo, (LDS && !n_rev) ? len : 0, k};
+            if (r < len && i < n && q < k)
                 wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win);
         });
 }
@@ -576,6 +587,332 @@ __global__ void __launch_bounds__(BS)
                 return;
             }
             wave_row<T, SYNC>(rp, ci, val, b, x, sc, win, r0);
+        });
+}
+
+// ---------------------------------------- local: one matrix per workgroup
+// For deep matrices (few rows per level) the cross-CU publish -> observe
+// round trip (~0.5-2 us) is paid once per level.  Here one workgroup of
+// kLocalBlock threads solves a whole matrix -- a segment [lo, hi) of rows
+// whose dependencies never leave it -- so every hop stays inside one
+// compute unit.  Wavefront w solves the rows lo + w, lo + w + W, ... (W
+// wavefronts) one at a time: a row's entries are loaded 64 at a time into
+// the lanes (coalesced; the next row's first 64 are prefetched while the
+// current row is solved) and broadcast one by one, lane q < k accumulating
+// right-hand side q in stored order (fused, bit-exact against the oracle).
+//
+// Readiness, per segment:
+//  * ring (bandwidth + 2 W <= R slots): results are published into an LDS
+//    ring -- slot r mod R holds the k values of row r, then a tag r written
+//    after them -- and polled there (tag read before the value).  A
+//    wavefront publishes row p only once every row <= p - R + bandwidth is
+//    done, so no reader of the slot's previous row p - R is left; the
+//    frontier is the lowest row any wavefront is still solving.  The
+//    lowest unfinished row can always publish (R > bandwidth), so the
+//    workgroup always makes progress.
+//  * global: x itself (sentinel-filled) is polled with workgroup-scope
+//    atomics.
+// Results always go to x.  Segments are taken by ticket, largest first.
+constexpr int kLocalBlock = 1024;
+constexpr int kLocalGroup = 8;  // dependencies polled together
+constexpr int kLocalPre = 8;    // rows of prefetch per wavefront
+constexpr int kLocalStaticLds = 20 * 1024;  // k_local's static LDS (staging, frontier, ticket), rounded up
+
+// segment {lo, hi, bandwidth, longest row's off-diagonal count}: ring mode
+// needs the bandwidth plus two rounds of rows in flight within the ring and
+// rows whose entries and diagonal fit one 64-lane load
+__device__ __forceinline__ bool local_ring_fits(int4 sg, int R)
+{
+    return sg.z + 2 * (kLocalBlock / kWave) <= R && sg.w < kWave;
+}
+
+// The rows of one segment for one wavefront of k_local, ring mode (rows of
+// at most 64 off-diagonal entries).  The wavefront's rows m = 0, 1, ... (row
+// lo + w + m W) go in batches of kLocalPre: at the start of a batch the
+// registers prefetched during the previous batch (each row's first 64
+// entries with the diagonal in lanes, its right-hand sides in lanes < k) are
+// copied -- the only wait for global memory, on loads issued a whole batch
+// earlier -- and only then are the next batch's loads issued, into other
+// registers, so that solving the batch never waits on a fresh load.  Row
+// bounds (lanes 0-7: rp[row], lanes 8-15: rp[row + 1]) run one batch ahead
+// of the entries.  The watchdog only sets a flag, reported once at the end.
+template <typename T, bool K1>
+__device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                const T *__restrict__ val, const T *__restrict__ b,
+                                                typename VBits<T>::U *__restrict__ xu, const SyncCtx &sc, int k,
+                                                int4 sg, int R, typename VBits<T>::U *rv, int *rt, int *s_cur,
+                                                typename VBits<T>::U *xs, T *vs)
+{
+    typedef typename VBits<T>::U U;
+    constexpr U kS = VBits<T>::kSentinel;
+    constexpr int kWaves = kLocalBlock / kWave;
+    constexpr int B = kLocalPre;
+    static_assert(2 * B <= kWave, "bounds of a batch fit one register");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int qk = min(lane, k - 1);
+    const int lo = sg.x, hi = sg.y, bw = sg.z;
+    auto row_of = [&](int m) { return lo + w + m * kWaves; };
+    auto poll = [&](int c) -> U {
+        const int sl = c & (R - 1);
+        const int tg = __hip_atomic_load(rt + sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        asm volatile("" ::: "memory");  // tag before value (the writer stores the value first)
+        const U v = __hip_atomic_load(rv + (size_t)sl * k + qk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return tg == c ? v : kS;
+    };
+    auto load_bounds = [&](int mb) {
+        const int r = row_of(mb + (lane & (B - 1)));
+        return (lane < 2 * B && r < hi) ? rp[r + (lane >= B)] : 0;
+    };
+    // loads of batch mb from its bounds register: entries into c/v, right-hand
+    // sides into bq; row bounds kept as scalars in a/e (e = diagonal's index)
+    int nc[B], na[B], ne[B];
+    T nv[B], nbq[B];
+    auto issue_batch = [&](int mb, int bnd) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int r = row_of(mb + u);
+            na[u] = __builtin_amdgcn_readlane(bnd, u);
+            ne[u] = __builtin_amdgcn_readlane(bnd, u + B) - 1;
+            if (r < hi) {
+                if (lane <= ne[u] - na[u]) {
+                    nc[u] = ci[na[u] + lane];
+                    nv[u] = val[na[u] + lane];
+                }
+                nbq[u] = b[(long long)r * k + qk];
+            }
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        nc[u] = 0;
+        nv[u] = (T)0;
+        nbq[u] = (T)0;
+    }
+    int bnd_next = load_bounds(0);
+    {
+        const int bnd = bnd_next;  // waits once, at the start
+        issue_batch(0, bnd);
+        bnd_next = load_bounds(B);
+    }
+    int front = lo;
+    bool failed = false;
+    for (int mb = 0; row_of(mb) < hi; mb += B) {
+        // the previous batch's loads (and this batch's bounds) are complete: copy
+        int cc[B], ca[B], ce[B];
+        T cv[B], cbq[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            cc[u] = nc[u];
+            cv[u] = nv[u];
+            cbq[u] = nbq[u];
+            ca[u] = na[u];
+            ce[u] = ne[u];
+        }
+        const int bnd = bnd_next;
+        if (row_of(mb + B) < hi) {
+            issue_batch(mb + B, bnd);
+            bnd_next = load_bounds(mb + 2 * B);
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int row = row_of(mb + u);
+            if (row < hi) {
+                const int a = ca[u], d = ce[u];
+                const int cnt = d - a;
+                int idle = 0;
+                T s = (T)0;
+                if constexpr (K1) {
+                    // one right-hand side: lane t polls entry t; values that
+                    // arrive go to this wavefront's staging array, and the
+                    // ready prefix is added in stored order from there (LDS
+                    // reads at a wave-uniform address: a broadcast)
+                    if (lane < cnt) vs[lane] = cv[u];
+                    U xt = kS;
+                    bool rdy = lane >= cnt;
+                    int used = 0;
+                    while (used < cnt) {
+                        if (!rdy) {
+                            xt = poll(cc[u]);
+                            rdy = xt != kS;
+                            if (rdy) xs[lane] = xt;
+                        }
+                        const unsigned long long ready = __ballot(rdy);
+                        const int upto = min(cnt, ~ready ? (int)__builtin_ctzll(~ready) : kWave);
+                        if (upto == used) {
+                            if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                                failed = true;
+                                s = as_val(VBits<T>::kQNaN);
+                                break;
+                            }
+                            continue;
+                        }
+                        idle = 0;
+                        int t = used;
+                        for (; t + 4 <= upto; t += 4) {
+                            const T x0 = as_val(xs[t]), x1 = as_val(xs[t + 1]), x2 = as_val(xs[t + 2]),
+                                    x3 = as_val(xs[t + 3]);
+                            const T v0 = vs[t], v1 = vs[t + 1], v2 = vs[t + 2], v3 = vs[t + 3];
+                            s = fmac(v0, x0, s);
+                            s = fmac(v1, x1, s);
+                            s = fmac(v2, x2, s);
+                            s = fmac(v3, x3, s);
+                        }
+                        for (; t < upto; ++t) s = fmac(vs[t], as_val(xs[t]), s);
+                        used = upto;
+                    }
+                } else
+                for (int u0 = 0; u0 < cnt; u0 += kLocalGroup) {
+                    int cu[kLocalGroup];
+                    U xr[kLocalGroup];
+#pragma unroll
+                    for (int g = 0; g < kLocalGroup; ++g) {
+                        cu[g] = __builtin_amdgcn_readlane(cc[u], min(u0 + g, cnt - 1));
+                        xr[g] = u0 + g < cnt ? poll(cu[g]) : (U)0;
+                    }
+#pragma unroll
+                    for (int g = 0; g < kLocalGroup; ++g) {
+                        if (u0 + g < cnt) {
+                            while (xr[g] == kS) {
+                                if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                                    failed = true;
+                                    xr[g] = VBits<T>::kQNaN;
+                                    break;
+                                }
+                                xr[g] = poll(cu[g]);
+                            }
+                            s = fmac(bcast(cv[u], u0 + g), as_val(xr[g]), s);
+                        }
+                    }
+                }
+                // the slot's previous row and all its readers must be done
+                idle = 0;
+                while (row - R + bw >= front) {
+                    int f = hi;
+#pragma unroll
+                    for (int v = 0; v < kWaves; ++v)
+                        f = min(f, __hip_atomic_load(s_cur + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                    front = f;
+                    if (row - R + bw < front) break;
+                    if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                        failed = true;
+                        break;
+                    }
+                }
+                if (lane < k) {
+                    const long long o = (long long)row * k + lane;
+                    U uo = as_bits((cbq[u] - s) / bcast(cv[u], cnt));
+                    if (uo == kS) uo = VBits<T>::kQNaN;
+                    const int sl = row & (R - 1);
+                    __hip_atomic_store(rv + (size_t)sl * k + lane, uo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    asm volatile("" ::: "memory");  // value before tag
+                    if (lane == 0) __hip_atomic_store(rt + sl, row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    xu[o] = uo;
+                }
+                asm volatile("" ::: "memory");
+                if (lane == 0) __hip_atomic_store(s_cur + w, row + kWaves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    if (__any(failed) && lane == 0) atomicOr(sc.err, 1);
+}
+
+// The rows of one segment for one wavefront of k_local, global mode (any
+// row length): x itself is polled; each row's entries are loaded when the
+// row starts.
+template <typename T>
+__device__ __forceinline__ void local_rows_global(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const T *__restrict__ val, const T *__restrict__ b,
+                                                  typename VBits<T>::U *__restrict__ xu, const SyncCtx &sc, int k,
+                                                  int4 sg, int *s_cur)
+{
+    typedef typename VBits<T>::U U;
+    constexpr U kS = VBits<T>::kSentinel;
+    constexpr int kWaves = kLocalBlock / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int qk = min(lane, k - 1);
+    for (int row = sg.x + w; row < sg.y; row += kWaves) {
+        const int a = rp[row], d = rp[row + 1] - 1;
+        int idle = 0;
+        T s = (T)0;
+        for (int j0 = a; j0 < d; j0 += kWave) {
+            const int cnt = min(kWave, d - j0);
+            const int cl = lane < cnt ? ci[j0 + lane] : 0;
+            const T vl = lane < cnt ? val[j0 + lane] : (T)0;
+            for (int u0 = 0; u0 < cnt; u0 += kLocalGroup) {
+                int cu[kLocalGroup];
+                U xr[kLocalGroup];
+#pragma unroll
+                for (int g = 0; g < kLocalGroup; ++g) {
+                    cu[g] = __builtin_amdgcn_readlane(cl, min(u0 + g, cnt - 1));
+                    xr[g] = u0 + g < cnt ? __hip_atomic_load(xu + (long long)cu[g] * k + qk, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP)
+                                         : (U)0;
+                }
+#pragma unroll
+                for (int g = 0; g < kLocalGroup; ++g) {
+                    if (u0 + g < cnt) {
+                        while (xr[g] == kS) {
+                            if (give_up(sc, ++idle)) {
+                                xr[g] = VBits<T>::kQNaN;
+                                break;
+                            }
+                            xr[g] = __hip_atomic_load(xu + (long long)cu[g] * k + qk, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        s = fmac(bcast(vl, u0 + g), as_val(xr[g]), s);
+                    }
+                }
+            }
+        }
+        if (lane < k) {
+            const long long o = (long long)row * k + lane;
+            U uo = as_bits((b[o] - s) / val[d]);
+            if (uo == kS) uo = VBits<T>::kQNaN;
+            __hip_atomic_store(xu + o, uo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        asm volatile("" ::: "memory");
+        if (lane == 0) __hip_atomic_store(s_cur + w, row + kWaves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLocalBlock)
+    k_local(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
+            const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int k, const int4 *__restrict__ seg, int n_seg,
+            unsigned *__restrict__ ticket, int ring_log2)
+{
+    typedef typename VBits<T>::U U;
+    constexpr int kWaves = kLocalBlock / kWave;
+    extern __shared__ unsigned long long smem_local[];
+    const int R = 1 << ring_log2;
+    U *rv = reinterpret_cast<U *>(smem_local);              // R * k values
+    int *rt = reinterpret_cast<int *>(rv + (size_t)R * k);  // R tags
+    __shared__ int s_cur[kWaves];                            // row each wavefront is solving
+    __shared__ U s_xs[kWaves][kWave];                        // k = 1: per-wavefront staging of x
+    __shared__ T s_vs[kWaves][kWave];                        // and of the row's coefficients
+    U *xu = reinterpret_cast<U *>(x);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    ticket_loop(
+        ticket, 1, n_seg,
+        [&](int t) {
+            if (t >= n_seg) return;
+            const int4 sg = seg[t];
+            if (local_ring_fits(sg, R))
+                for (int i = threadIdx.x; i < R; i += blockDim.x) rt[i] = -1;
+            if (lane == 0) s_cur[w] = sg.x + w;
+        },
+        [&](int t) {
+            const int4 sg = seg[t];
+            if (local_ring_fits(sg, R) && k == 1)
+                local_rows_ring<T, true>(rp, ci, val, b, xu, sc, k, sg, R, rv, rt, s_cur, s_xs[w], s_vs[w]);
+            else if (local_ring_fits(sg, R))
+                local_rows_ring<T, false>(rp, ci, val, b, xu, sc, k, sg, R, rv, rt, s_cur, s_xs[w], s_vs[w]);
+            else
+                local_rows_global<T>(rp, ci, val, b, xu, sc, k, sg, s_cur);
+            if (lane == 0) __hip_atomic_store(s_cur + w, sg.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         });
 }
 
@@ -689,33 +1026,62 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ------------------------------------------------------------ refinement
+// Maxima of non-negative doubles are taken on their bit patterns (which
+// order like the values): per thread over a grid-stride loop, then across
+// the workgroup in LDS, then one atomicMax per workgroup -- an atomic per
+// element on one address serialises in the memory side.
+__device__ __forceinline__ void block_max_to(unsigned long long v, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s_max[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const unsigned long long w = (unsigned long long)__shfl_xor((long long)v, o, kWave);
+        v = w > v ? w : v;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_max[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) v = s_max[w] > v ? s_max[w] : v;
+        atomicMax(out, v);
+    }
+}
+
 // r = b - L x in FP64 (rows in CSR order, fused multiply-add chain), written
-// as FP32 for the next correction solve; also accumulates max|x| for the
-// convergence test (non-negative doubles order like their bit patterns).
+// as FP32 for the next correction solve; also max|x| for the convergence
+// test.  Grid-stride over the n*k (row, right-hand side) pairs.
 __global__ void __launch_bounds__(kBlock)
     k_residual32(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
                  const double *__restrict__ b, const double *__restrict__ x, float *__restrict__ r32, int n, int k,
                  unsigned long long *__restrict__ xmax_bits)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)n * k) return;
-    const int i = (int)(g / k), q = (int)(g - (long long)i * k);
-    double s = 0.0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[(long long)ci[j] * k + q], s);
-    r32[g] = (float)(b[g] - s);
-    atomicMax(xmax_bits, (unsigned long long)__double_as_longlong(fabs(x[g])));
+    const long long count = (long long)n * k;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long m = 0;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < count; g += stride) {
+        const int i = (int)(g / k), q = (int)(g - (long long)i * k);
+        double s = 0.0;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[(long long)ci[j] * k + q], s);
+        r32[g] = (float)(b[g] - s);
+        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(x[g]));
+        m = a > m ? a : m;
+    }
+    block_max_to(m, xmax_bits);
 }
 
-// x += d (FP32 correction) and max|d| for the convergence test
+// x += d (FP32 correction) and max|d| for the convergence test (grid-stride)
 __global__ void __launch_bounds__(kBlock)
     k_accumulate(double *__restrict__ x, const float *__restrict__ dcor, long long count,
                  unsigned long long *__restrict__ dmax_bits, int first)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= count) return;
-    const double dv = (double)dcor[g];
-    x[g] = first ? dv : x[g] + dv;
-    atomicMax(dmax_bits, (unsigned long long)__double_as_longlong(fabs(dv)));
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long m = 0;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < count; g += stride) {
+        const double dv = (double)dcor[g];
+        x[g] = first ? dv : x[g] + dv;
+        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(dv));
+        m = a > m ? a : m;
+    }
+    block_max_to(m, dmax_bits);
 }
 
 __global__ void k_cast_f32(const double *__restrict__ a, float *__restrict__ o, long long count)

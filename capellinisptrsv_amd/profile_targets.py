@@ -8,6 +8,11 @@
         WRITE_SIZE against known bytes
     python -m capellinisptrsv_amd.profile_targets adjoint [--steps N]
         forward + gather adjoint of the n = 1,000,000 level matrix
+    python -m capellinisptrsv_amd.profile_targets batched [--steps N]
+        the 256-matrix batch with 16 right-hand sides: N FP64 solves and N
+        FP32 + FP64 refined solves
+    python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto]
+        the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels)
 """
 import argparse
 import sys
@@ -15,7 +20,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched"])
+    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched", "deep"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--algo", default="auto")
     ap.add_argument("--mib", type=int, default=1024)
@@ -35,13 +40,28 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(f"copy: {a.steps} x {a.mib} MiB read + {a.mib} MiB written", flush=True)
         return 0
-    cfg = {"solve": 1, "adjoint": 4}[a.target] if a.target != "batched" else None
+    if a.target == "batched":
+        from capellinisptrsv_amd import Batch
+        _, row_off, rp, ci, v = api.make_batch(256, seed=0)
+        k, n = 16, int(row_off[-1])
+        bt = Batch(*(torch.from_numpy(t).to(dev) for t in (row_off, rp, ci, v)), k=k)
+        b = torch.from_numpy(api.uniform(3, n * k).reshape(n, k)).to(dev)
+        x = torch.empty_like(b)
+        for _ in range(a.steps):
+            bt.solve(b, out=x)
+        for _ in range(a.steps):
+            _, it, rel = bt.solve_refined(b, tol=1e-12, out=x)
+        torch.cuda.synchronize()
+        print(f"batched: n={n} nnz={len(ci)} k={k} steps={a.steps} refined iterations={it}", flush=True)
+        bt.close()
+        return 0
+    cfg = {"solve": 1, "adjoint": 4, "deep": 2}[a.target]
     rp, ci, v = api.make_config(cfg, seed=0)
     n = len(rp) - 1
     d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
     b = torch.from_numpy(api.uniform(1, n)).to(dev)
     x = torch.empty_like(b)
-    plan = Plan(*d, algo=a.algo) if a.target == "solve" else Plan(*d, algo="wrtfst", sync="value")
+    plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep") else Plan(*d, algo="wrtfst", sync="value")
     for _ in range(a.steps):
         plan.solve(b, out=x)
         if a.target == "adjoint":

@@ -56,6 +56,9 @@ extern "C" {
 #define CAP_ALGO_TWOPART 2 /* thread-per-row, blocking for earlier waves (our2Part) */
 #define CAP_ALGO_MIX 3     /* per-64-row chunk: thread or wavefront per row (mix) */
 #define CAP_ALGO_CSC 4     /* wavefront-per-column in-degree scatter (SyncFree_csc) */
+#define CAP_ALGO_LOCAL 5   /* one workgroup per matrix, wavefront per row, dependencies
+                              resolved inside one compute unit: deep matrices
+                              (few rows per level); value protocol on x itself */
 
 /* adjoint method (cap_opts.adjoint) */
 #define CAP_ADJOINT_AUTO 0
@@ -162,11 +165,17 @@ int cap_plan_solve_refined(cap_plan plan64, const double *b, double *x, int rhs,
  * the readiness state.  Matrices are concatenated: matrix m owns rows
  * [row_off[m], row_off[m+1]); row_ptr (n_total+1 entries) indexes the
  * concatenated col_idx/val; column indices are LOCAL to their matrix.  b/x
- * are concatenated the same way, k interleaved values per row. */
+ * are concatenated the same way, k interleaved values per row.
+ * opts->algo: CAP_ALGO_WRTFST (thread-per-row over all matrices, row blocks
+ * handed out in level order so that every matrix advances at once),
+ * CAP_ALGO_LOCAL (one workgroup per matrix, largest first) or
+ * CAP_ALGO_AUTO (both timed at creation, the faster kept). */
 typedef struct cap_batch_s *cap_batch;
 int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
                      const int *row_ptr, const int *col_idx, const void *val,
                      int value_type, int k, const cap_opts *opts, void *stream);
+/* The algorithm the batch resolved to (CAP_ALGO_WRTFST or CAP_ALGO_LOCAL). */
+int cap_batch_info(cap_batch batch, int *algo);
 int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
 int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
                             double tol, int max_iter, int *iters,

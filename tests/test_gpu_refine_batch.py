@@ -15,14 +15,15 @@ from conftest import rel_maxnorm
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("cfg,scale,k", [(1, 0.05, 1), (0, 1.0, 1), (0, 0.2, 16), (4, 0.1, 8)])
-def test_refined_reaches_1e12(cuda, cfg, scale, k):
+@pytest.mark.parametrize("cfg,scale,k,algo", [(1, 0.05, 1, "wrtfst"), (0, 1.0, 1, "wrtfst"), (0, 0.2, 16, "wrtfst"),
+                                              (4, 0.1, 8, "wrtfst"), (2, 0.02, 1, "local"), (0, 0.2, 16, "local")])
+def test_refined_reaches_1e12(cuda, cfg, scale, k, algo):
     import torch
     from capellinisptrsv_amd import Plan
     rp, ci, v = api.make_config(cfg, seed=1, scale=scale)
     n = len(rp) - 1
     d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
-    plan = Plan(*d, algo="wrtfst", sync="value")
+    plan = Plan(*d, algo=algo, sync="value")
     b = api.uniform(9, n * k)
     bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
     for rep in range(2):  # second call replays the captured graph
@@ -49,15 +50,18 @@ def _batch(seed, n_mat, k, nmin=200, nmax=5000):
     return mats, row_off, rp_all, ci_all, v_all
 
 
+@pytest.mark.parametrize("algo", ["wrtfst", "local", "auto"])
 @pytest.mark.parametrize("k", [1, 8, 16, 32])
-def test_batch_bit_exact_per_matrix(cuda, k):
+def test_batch_bit_exact_per_matrix(cuda, k, algo):
     import torch
     from capellinisptrsv_amd import Batch
     mats, row_off, rp, ci, v = _batch(3, 24, k)
     N = row_off[-1]
     b = api.uniform(11, N * k)
     t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
-    bt = Batch(*t, k=k)
+    bt = Batch(*t, k=k, algo=algo)
+    if algo != "auto":
+        assert bt.algo == algo
     x = bt.solve(torch.from_numpy(b.reshape(N, k)).to(cuda)).cpu().numpy().reshape(-1)
     for m, (mrp, mci, mv) in enumerate(mats):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
@@ -65,7 +69,8 @@ def test_batch_bit_exact_per_matrix(cuda, k):
         assert np.array_equal(x[lo:hi], ref), m
 
 
-def test_batch_refined(cuda):
+@pytest.mark.parametrize("algo", ["wrtfst", "local"])
+def test_batch_refined(cuda, algo):
     import torch
     from capellinisptrsv_amd import Batch
     k = 16
@@ -73,7 +78,7 @@ def test_batch_refined(cuda):
     N = row_off[-1]
     b = api.uniform(12, N * k)
     t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
-    bt = Batch(*t, k=k)
+    bt = Batch(*t, k=k, algo=algo)
     x, iters, rel = bt.solve_refined(torch.from_numpy(b.reshape(N, k)).to(cuda), tol=1e-12)
     x = x.cpu().numpy().reshape(-1)
     for m, (mrp, mci, mv) in enumerate(mats):

@@ -50,7 +50,8 @@ MATS = list(_matrices())
 
 @pytest.mark.parametrize("algo,sync,lds,bpc", [("wrtfst", "value", False, 1), ("wrtfst", "value", True, 1),
                                                ("wrtfst", "value", True, 4), ("wrtfst", "flag", False, 2),
-                                               ("twopart", "value", False, 1), ("twopart", "flag", False, 1)])
+                                               ("twopart", "value", False, 1), ("twopart", "flag", False, 1),
+                                               ("local", "value", False, 1)])
 @pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
 def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, lds, bpc, name, mat):
     rp, ci, v = mat
@@ -64,7 +65,8 @@ def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, lds, bpc, name, mat):
     assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=False)) <= FP64_TOL
 
 
-@pytest.mark.parametrize("algo,sync", [("wrtfst", "value"), ("wrtfst", "flag"), ("twopart", "value")])
+@pytest.mark.parametrize("algo,sync", [("wrtfst", "value"), ("wrtfst", "flag"), ("twopart", "value"),
+                                       ("local", "value")])
 @pytest.mark.parametrize("name,mat", MATS[:4], ids=[m[0] for m in MATS[:4]])
 def test_fp32_bit_exact_and_within_1e5(cuda, algo, sync, name, mat):
     rp, ci, v = mat
@@ -103,14 +105,26 @@ def test_csc_syncfree(cuda, name, mat):
 
 
 @pytest.mark.parametrize("k", [2, 3, 8, 16, 32, 64])
-@pytest.mark.parametrize("sync,lds", [("value", False), ("value", True), ("flag", False)])
-def test_multi_rhs_bit_exact(cuda, k, sync, lds):
+@pytest.mark.parametrize("algo,sync,lds", [("wrtfst", "value", False), ("wrtfst", "value", True),
+                                           ("wrtfst", "flag", False), ("local", "value", False)])
+def test_multi_rhs_bit_exact(cuda, k, algo, sync, lds):
     rp, ci, v = api.make_config(0, seed=3, scale=0.2)
     n = len(rp) - 1
     b = api.uniform(100 + k, n * k)
     ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
-    x, _ = _solve(cuda, rp, ci, v, b, "wrtfst", sync, k=k, lds=lds)
+    x, _ = _solve(cuda, rp, ci, v, b, algo, sync, k=k, lds=lds)
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+def test_local_deep_band_bit_exact(cuda):
+    """One workgroup walks a deep banded matrix (n = 20,000, bandwidth 128,
+    ~38 nnz/row, thousands of levels) and matches the oracle bit for bit."""
+    rp, ci, v = api.make_config(2, seed=5, scale=0.04)
+    n = len(rp) - 1
+    b = api.uniform(21, n)
+    x, info = _solve(cuda, rp, ci, v, b, "local", "value")
+    assert info["algo"] == "local"
+    assert np.array_equal(x.view(np.uint64), oracle.csr_forward(rp, ci, v, b, fused=True).view(np.uint64))
 
 
 def test_unit_lower_known_answer(cuda):
@@ -205,11 +219,18 @@ def test_watchdog_forced_from_host(cuda):
 
 
 def test_autotuner_picks_a_correct_path(cuda):
+    # deep band (few rows per level): the one-workgroup kernel is a candidate
     rp, ci, v = api.make_config(2, seed=1, scale=0.01)
     b = api.uniform(9, len(rp) - 1)
     x, info = _solve(cuda, rp, ci, v, b, "auto", "auto")
-    assert info["algo"] in ("wrtfst", "mix", "twopart")
-    assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=True)) <= FP64_TOL
+    assert info["algo"] in ("wrtfst", "mix", "twopart", "local")
+    assert np.array_equal(x.view(np.uint64), oracle.csr_forward(rp, ci, v, b, fused=True).view(np.uint64))
+    # wide levels: it is not (n / levels far above the threshold)
+    rp, ci, v = api.make_config(1, seed=1, scale=0.02)
+    b = api.uniform(9, len(rp) - 1)
+    x, info = _solve(cuda, rp, ci, v, b, "auto", "auto")
+    assert info["algo"] in ("wrtfst", "mix")
+    assert np.array_equal(x.view(np.uint64), oracle.csr_forward(rp, ci, v, b, fused=True).view(np.uint64))
 
 
 @pytest.mark.parametrize("algo,lds,k", [("wrtfst", False, 1), ("wrtfst", True, 1), ("wrtfst", False, 8),
