@@ -143,7 +143,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        if t.get("algo") == info["algo"] and t.get("n") == n:
+        # only for the kernel configuration those counters were taken on
+        if all(t.get("plan", {}).get(key) == info.get(key) for key in ("algo", "sync", "lds_window",
+                                                                       "blocks_per_cu", "readiness")) \
+                and t.get("n") == n:
             traffic = t.get("hbm_bytes_per_launch")
 
     out = {

@@ -638,6 +638,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         }
     float best = 1e30f;
     TuneChoice pick = cands[0];
+    bool pick_set = false;
     int rc = CAP_OK;
     for (const TuneChoice &c : cands) {
         apply_choice(p, c);
@@ -645,9 +646,12 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         if (c.algo == CAP_ALGO_LOCAL && (rc = ensure_segments(p, nullptr)) != CAP_OK) break;
         float t = 0;
         if ((rc = time_candidate(p, c.algo, c.sync, b, x, 1, st, best, &t)) != CAP_OK) break;
-        if (t < best) {
+        // a later (less preferred) candidate must be 2% faster to win, so
+        // near-ties resolve the same way from run to run
+        if (t < (pick_set ? 0.98f : 1.0f) * best) {
             best = t;
             pick = c;
+            pick_set = true;
         }
     }
     (void)hipFree(b);
