@@ -96,3 +96,28 @@ def test_batch_rejects_bad_matrix(cuda):
     t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
     with pytest.raises(CapError):
         Batch(*t, k=1)
+
+
+@pytest.mark.parametrize("algo", ["wrtfst", "local"])
+def test_batch_with_empty_and_single_row_matrices(cuda, algo):
+    """Empty matrices (n = 0) and 1-row matrices inside a batch."""
+    import torch
+    from capellinisptrsv_amd import Batch
+    mats = [api.generate("band_rowcount", n=n, offdiag=8, band=64, seed=70 + i) if n else
+            (np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+            for i, n in enumerate([300, 0, 1, 0, 2000, 1])]
+    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
+    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
+    rp = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
+    ci = np.concatenate([m[1] for m in mats]).astype(np.int32)
+    v = np.concatenate([m[2] for m in mats])
+    k = 2
+    N = int(row_off[-1])
+    b = api.uniform(5, N * k)
+    t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
+    bt = Batch(*t, k=k, algo=algo)
+    x = bt.solve(torch.from_numpy(b.reshape(N, k)).to(cuda)).cpu().numpy().reshape(-1)
+    for m, (mrp, mci, mv) in enumerate(mats):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        if hi > lo:
+            assert np.array_equal(x[lo:hi], oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)), m

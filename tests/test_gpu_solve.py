@@ -268,3 +268,17 @@ def test_staged_readiness_reused_across_plans_and_sizes(cuda):
         assert plan.status() == 0
         assert np.array_equal(out.cpu().numpy(), oracle.csr_forward(rp, ci, v, b, fused=True))
         plan.close()
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_local_long_rows_use_global_mode_bit_exact(cuda, k):
+    """Rows with more than 63 off-diagonal entries do not fit the one-load row
+    staging of the ring mode; such a matrix runs in the global mode."""
+    rp, ci, v = random_lower(3000, max_off=150, band=400, seed=11)
+    assert int(np.max(np.diff(rp))) - 1 > 63
+    n = len(rp) - 1
+    b = api.uniform(31, n * k)
+    ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
+    x, info = _solve(cuda, rp, ci, v, b, "local", "value", k=k)
+    assert info["algo"] == "local"
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
