@@ -310,7 +310,52 @@ def extra_workloads(args):
         "fp64_rel_err_sampled": worst64, "refined_rel_err_sampled": worstr,
     }
     bt.close()
+    res["rocsparse_csrsv"] = rocsparse_comparison()
     return res
+
+
+def rocsparse_comparison():
+    """rocSPARSE csrsv (analysis once, then solves) on three of the matrices,
+    next to this library's autotuned plan: context for the numbers above."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from capellinisptrsv_amd import Plan, api
+    from capellinisptrsv_amd.rocsparse_cmp import Csrsv
+
+    dev = torch.device("cuda:0")
+    out = {}
+    for name, cfg in (("wide_level_2m", 1), ("band_100k", 0), ("deep_band_500k", 2)):
+        log(f"rocsparse comparison {name}")
+        rp, ci, v = api.make_config(cfg, seed=0)
+        n = len(rp) - 1
+        d = [torch.from_numpy(a).to(dev) for a in (rp, ci, v)]
+        b = torch.from_numpy(api.uniform(1, n)).to(dev)
+        x = torch.empty_like(b)
+        r = {}
+        try:
+            cs = Csrsv(*d)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cs.analysis()
+            torch.cuda.synchronize()
+            r["rocsparse_analysis_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+            steps = 20 if cfg != 2 else 3
+            r["rocsparse_ms_per_solve"] = round(_time_gpu(lambda: cs.solve(b, x), steps, warmup=2), 4)
+            ref = oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True)
+            r["rocsparse_rel_err_vs_oracle"] = float(np.max(np.abs(x.cpu().numpy() - ref)) / np.max(np.abs(ref)))
+            cs.close()
+        except Exception as e:  # a comparator that cannot run is reported, not fatal
+            r["rocsparse_error"] = f"{type(e).__name__}: {e}"
+        p = Plan(*d)
+        r["this_library_ms_per_solve"] = round(_time_gpu(lambda: p.solve(b, out=x), 20 if cfg != 2 else 3, warmup=2), 4)
+        r["this_library_algo"] = p.info()["algo"]
+        if "rocsparse_ms_per_solve" in r:
+            r["speedup_over_rocsparse"] = round(r["rocsparse_ms_per_solve"] / r["this_library_ms_per_solve"], 2)
+        p.close()
+        out[name] = r
+    return out
 
 
 if __name__ == "__main__":
