@@ -51,7 +51,7 @@ class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
                 ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
-                ("reserved", c_int * 3)]
+                ("upper", c_int), ("unit_diag", c_int), ("reserved", c_int * 1)]
 
 
 class cap_gen_params(Structure):

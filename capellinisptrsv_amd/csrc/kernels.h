@@ -1090,6 +1090,17 @@ __global__ void k_cast_f32(const double *__restrict__ a, float *__restrict__ o, 
     if (g < count) o[g] = (float)a[g];
 }
 
+// dst row n-1-i = src row i (k interleaved values per row): an upper
+// triangular plan solves the row- and column-reversed (lower) matrix
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_reverse_rows(const T *__restrict__ src, T *__restrict__ dst, int n, int k)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)n * k) return;
+    const int i = (int)(g / k), q = (int)(g - (long long)i * k);
+    dst[(long long)(n - 1 - i) * k + q] = src[g];
+}
+
 // Batch: local column indices -> columns of the block-diagonal matrix
 // (matrix of each row found by binary search over row_off).
 __global__ void __launch_bounds__(kBlock)

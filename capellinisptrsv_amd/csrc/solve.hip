@@ -72,6 +72,11 @@ struct cap_plan_s {
     int4 *seg = nullptr;
     int n_seg = 0;
     int lds_max = 65536;
+    // upper-triangular plans solve the reversed matrix (b and x reversed
+    // around the solve through rev_b / rev_x); unit_diag: the diagonal was implicit
+    bool reversed = false, unit_diag = false;
+    void *rev_b = nullptr, *rev_x = nullptr;
+    size_t rev_bytes = 0;
     // value protocol: staged readiness array (uncached pool block) or x itself
     int staged = 1;
     void *ready = nullptr;
@@ -222,7 +227,7 @@ void plan_free(cap_plan p)
         (void)hipFree(p->ci);
         (void)hipFree(p->val);
     }
-    void *bufs[] = {p->sched, p->seg, p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
+    void *bufs[] = {p->rev_b, p->rev_x, p->sched, p->seg, p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
                     p->cnt0, p->cnt, p->adj_ls, p->rpT, p->ciT, p->vT, p->val32, p->r32, p->d32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -679,15 +684,93 @@ int validate_device_matrix(cap_plan p, std::vector<int> *rp_host)
     return cap_validate_csr_s(n, nnz, rp_host->data(), ci.data(), (const float *)(const void *)v.data(), &bad);
 }
 
+// Upper-triangular and implicit-unit-diagonal matrices become the lower,
+// diagonal-last CSR every kernel solves: row t of the result is row
+// i = n-1-t of an upper matrix (columns c -> n-1-c), or row t of a lower
+// one; its off-diagonal entries keep their stored order (so the
+// accumulation order is the caller's) and the diagonal -- stored, or 1 --
+// comes last.  Rejects a stored diagonal that is not where the layout puts
+// it and, with unit_diag, any diagonal entry.
+template <typename T>
+int convert_triangular(int n, int nnz, const int *rp, const int *ci, const T *val, bool upper, bool unit,
+                       std::vector<int> *rp2, std::vector<int> *ci2, std::vector<T> *v2)
+{
+    rp2->assign((size_t)n + 1, 0);
+    ci2->clear();
+    v2->clear();
+    ci2->reserve((size_t)nnz + (unit ? n : 0));
+    v2->reserve((size_t)nnz + (unit ? n : 0));
+    for (int t = 0; t < n; ++t) {
+        const int i = upper ? n - 1 - t : t;
+        int lo = rp[i], hi = rp[i + 1];
+        if (hi < lo || lo < 0 || hi > nnz) return CAP_ERR_ROWPTR;
+        T d = (T)1;
+        if (!unit) {
+            if (hi == lo) return CAP_ERR_NO_DIAG;
+            const int dj = upper ? lo++ : --hi;
+            if (ci[dj] != i) return CAP_ERR_NO_DIAG;
+            d = val[dj];
+        }
+        for (int j = lo; j < hi; ++j) {
+            const int c = ci[j];
+            if (c < 0 || c >= n || (upper ? c <= i : c >= i)) return CAP_ERR_UPPER;
+            ci2->push_back(upper ? n - 1 - c : c);
+            v2->push_back(val[j]);
+        }
+        ci2->push_back(t);
+        v2->push_back(d);
+        (*rp2)[(size_t)t + 1] = (int)ci2->size();
+    }
+    return CAP_OK;
+}
+
 int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci, const void *val, int vt,
                      const cap_opts *opts, hipStream_t st)
 {
     cap_opts o;
     cap_opts_default(&o);
     if (opts) o = *opts;
-    if (!out || n < 0 || nnz < n || (vt != CAP_F64 && vt != CAP_F32) || !rp || (nnz > 0 && (!ci || !val)))
+    if (!out || n < 0 || nnz < 0 || (vt != CAP_F64 && vt != CAP_F32) || !rp || (nnz > 0 && (!ci || !val)))
         return CAP_ERR_ARG;
+    if (!o.unit_diag && nnz < n) return CAP_ERR_ARG;
     *out = nullptr;
+    if (o.upper || o.unit_diag) {
+        // convert on the host, then create a plan over the owned lower matrix
+        std::vector<int> rph((size_t)n + 1), cih((size_t)nnz);
+        std::vector<unsigned char> vh((size_t)nnz * (vt == CAP_F64 ? 8 : 4));
+        const hipMemcpyKind kind = o.host_inputs ? hipMemcpyHostToHost : hipMemcpyDeviceToHost;
+        CAP_HIP_TRY(hipMemcpy(rph.data(), rp, sizeof(int) * rph.size(), kind));
+        if (nnz) {
+            CAP_HIP_TRY(hipMemcpy(cih.data(), ci, sizeof(int) * cih.size(), kind));
+            CAP_HIP_TRY(hipMemcpy(vh.data(), val, vh.size(), kind));
+        }
+        if (rph[0] != 0 || rph[n] != nnz) return CAP_ERR_ROWPTR;
+        std::vector<int> rp2, ci2;
+        std::vector<unsigned char> v2;
+        int rc;
+        if (vt == CAP_F64) {
+            std::vector<double> t;
+            rc = convert_triangular<double>(n, nnz, rph.data(), cih.data(), (const double *)(const void *)vh.data(),
+                                            o.upper != 0, o.unit_diag != 0, &rp2, &ci2, &t);
+            v2.assign((const unsigned char *)t.data(), (const unsigned char *)t.data() + t.size() * 8);
+        } else {
+            std::vector<float> t;
+            rc = convert_triangular<float>(n, nnz, rph.data(), cih.data(), (const float *)(const void *)vh.data(),
+                                           o.upper != 0, o.unit_diag != 0, &rp2, &ci2, &t);
+            v2.assign((const unsigned char *)t.data(), (const unsigned char *)t.data() + t.size() * 4);
+        }
+        if (rc != CAP_OK) return rc;
+        cap_opts o2 = o;
+        o2.upper = 0;
+        o2.unit_diag = 0;
+        o2.host_inputs = 1;
+        cap_plan p = nullptr;
+        CAP_TRY(plan_create_impl(&p, n, (int)ci2.size(), rp2.data(), ci2.data(), v2.data(), vt, &o2, st));
+        p->reversed = o.upper != 0;
+        p->unit_diag = o.unit_diag != 0;
+        *out = p;
+        return CAP_OK;
+    }
     int dev = 0;
     CAP_TRY(check_device(&dev));
     cap_plan p = new cap_plan_s();
@@ -823,15 +906,66 @@ int cap_plan_create(cap_plan *plan, int n, int nnz, const int *row_ptr, const in
     return plan_create_impl(plan, n, nnz, row_ptr, col_idx, val, value_type, opts, S(stream));
 }
 
+}  // extern "C"
+
+namespace {
+
+// Upper-triangular plans: reversed copies of b and x around the solve.
+int ensure_reversal(cap_plan p, int rhs)
+{
+    const size_t bytes = (size_t)p->n * rhs * p->vbytes + 8;
+    if (p->rev_bytes >= bytes) return CAP_OK;
+    CAP_HIP_TRY(hipDeviceSynchronize());  // kernels in flight may use the old buffers
+    if (p->rev_b) CAP_HIP_TRY(hipFree(p->rev_b));
+    if (p->rev_x) CAP_HIP_TRY(hipFree(p->rev_x));
+    p->rev_b = p->rev_x = nullptr;
+    p->rev_bytes = 0;
+    if (hipMalloc(&p->rev_b, bytes) != hipSuccess || hipMalloc(&p->rev_x, bytes) != hipSuccess) return CAP_ERR_ALLOC;
+    p->rev_bytes = bytes;
+    return CAP_OK;
+}
+
+int reverse_rows(cap_plan p, const void *src, void *dst, int rhs, hipStream_t st)
+{
+    const long long cnt = (long long)p->n * rhs;
+    if (cnt == 0) return CAP_OK;
+    if (p->vt == CAP_F64)
+        k_reverse_rows<double><<<blocks_for(cnt), kBlock, 0, st>>>((const double *)src, (double *)dst, p->n, rhs);
+    else
+        k_reverse_rows<float><<<blocks_for(cnt), kBlock, 0, st>>>((const float *)src, (float *)dst, p->n, rhs);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
+int launch_plain(cap_plan p, const void *b, void *x, int rhs, hipStream_t st)
+{
+    if (p->vt == CAP_F64)
+        return launch_kernel<double>(p, p->algo, p->sync, (const double *)p->val, (const double *)b, (double *)x, rhs,
+                                     st);
+    return launch_kernel<float>(p, p->algo, p->sync, (const float *)p->val, (const float *)b, (float *)x, rhs, st);
+}
+
+}  // namespace
+
+extern "C" {
+
 int cap_plan_solve(cap_plan p, const void *b, void *x, int rhs, void *stream)
 {
     if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
-    return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
+    if (!p->reversed) return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
+    CAP_TRY(ensure_reversal(p, rhs));
+    CAP_TRY(reverse_rows(p, b, p->rev_b, rhs, S(stream)));
+    CAP_TRY(plan_solve_with(p, p->algo, p->sync, p->rev_b, p->rev_x, rhs, S(stream)));
+    return reverse_rows(p, p->rev_x, x, rhs, S(stream));
 }
 
 int cap_plan_reset(cap_plan p, void *x, int rhs, void *stream)
 {
     if (!p || !x || rhs < 1) return CAP_ERR_ARG;
+    if (p->reversed) {
+        CAP_TRY(ensure_reversal(p, rhs));
+        x = p->rev_x;
+    }
     return p->vt == CAP_F64 ? reset_t<double>(p, p->algo, p->sync, x, rhs, S(stream))
                             : reset_t<float>(p, p->algo, p->sync, x, rhs, S(stream));
 }
@@ -839,11 +973,11 @@ int cap_plan_reset(cap_plan p, void *x, int rhs, void *stream)
 int cap_plan_launch(cap_plan p, const void *b, void *x, int rhs, void *stream)
 {
     if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
-    if (p->vt == CAP_F64)
-        return launch_kernel<double>(p, p->algo, p->sync, (const double *)p->val, (const double *)b, (double *)x, rhs,
-                                     S(stream));
-    return launch_kernel<float>(p, p->algo, p->sync, (const float *)p->val, (const float *)b, (float *)x, rhs,
-                                S(stream));
+    if (!p->reversed) return launch_plain(p, b, x, rhs, S(stream));
+    if (p->rev_bytes < (size_t)p->n * rhs * p->vbytes) return CAP_ERR_ARG;  // launch without its reset
+    CAP_TRY(reverse_rows(p, b, p->rev_b, rhs, S(stream)));
+    CAP_TRY(launch_plain(p, p->rev_b, p->rev_x, rhs, S(stream)));
+    return reverse_rows(p, p->rev_x, x, rhs, S(stream));
 }
 
 int cap_plan_info(cap_plan p, int *algo, int *sync, int *border, int *mix_len, double *wocc, double *eocc)
@@ -891,7 +1025,7 @@ int cap_plan_destroy(cap_plan p)
 int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, void *grad_val, void *stream)
 {
     if (!p || !g || !x || !grad_b) return CAP_ERR_ARG;
-    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
+    if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
     hipStream_t st = S(stream);
     const int n = p->n;
     if (n == 0) return CAP_OK;
@@ -983,7 +1117,7 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
                  double *last_rel, hipStream_t st)
 {
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
-    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
+    if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
     CAP_TRY(ensure_refine(p, rhs, st));
     // the staged readiness array must exist before capture (no allocation
     // inside the captured steps, whose kernels bake in its address)
@@ -1132,6 +1266,7 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     cap_opts o;
     cap_opts_default(&o);
     if (opts) o = *opts;
+    if (o.upper || o.unit_diag) return CAP_ERR_UNSUPPORTED;
     hipStream_t st = S(stream);
     int dev = 0;
     CAP_TRY(check_device(&dev));
@@ -1252,7 +1387,7 @@ int oneshot(int n, int nnz, const int *rp, const int *ci, const T *val, const T 
     do {
         if ((rc = dev_alloc(&db, (size_t)n)) || (rc = dev_alloc(&dx, (size_t)n))) break;
         if (hipMemcpy(db, b, sizeof(T) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) { rc = CAP_ERR_HIP; break; }
-        if ((rc = plan_solve_with(p, p->algo, p->sync, db, dx, 1, nullptr)) != CAP_OK) break;
+        if ((rc = cap_plan_solve(p, db, dx, 1, nullptr)) != CAP_OK) break;
         if (hipMemcpy(x, dx, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) { rc = CAP_ERR_HIP; break; }
         rc = cap_plan_status(p, nullptr);
     } while (0);

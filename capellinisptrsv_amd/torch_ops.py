@@ -37,13 +37,16 @@ class Plan:
 
     row_ptr/col_idx: int32 CUDA tensors; values: float64/float32 CUDA tensor.
     The plan keeps references to the three tensors (their memory is used in
-    place).  algo: auto|wrtfst|twopart|mix|csc; sync: auto|value|flag.
+    place).  algo: auto|wrtfst|twopart|mix|csc|local; sync: auto|value|flag.
+    upper=True: upper triangular, diagonal first in each row; unit_diagonal=
+    True: no diagonal stored (taken as 1).  Both convert the matrix into a
+    plan-owned copy; such plans solve, but have no adjoint or refinement.
     """
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
-                 stream=None):
+                 upper: bool = False, unit_diagonal: bool = False, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -55,7 +58,8 @@ class Plan:
         self.dtype = values.dtype
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
-                              lds_window=int(lds_window), readiness=readiness)
+                              lds_window=int(lds_window), readiness=readiness, upper=int(upper),
+                              unit_diag=int(unit_diagonal))
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")

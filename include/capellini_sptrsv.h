@@ -101,7 +101,10 @@ typedef struct cap_opts {
                           rows of their own workgroup in LDS (value protocol) */
     int readiness;     /* CAP_READY_* (value protocol): where readiness is
                           polled; 0 = tuned / default staged */
-    int reserved[3];
+    int upper;         /* 1 = the matrix is upper triangular, diagonal first
+                          in each row (backward substitution) */
+    int unit_diag;     /* 1 = implicit unit diagonal: no diagonal is stored */
+    int reserved[1];
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */

@@ -142,6 +142,17 @@ def csc_executor(cp, ri, cv, indeg, m, n, substitution, rhs, b):
     return x
 
 
+def csr_solve(rp, ci, v, b, rhs=1, upper=False, unit=False, fused=True):
+    """Lower/upper, stored/implicit-unit-diagonal CSR solve in stored order (oracle.c)."""
+    v = np.ascontiguousarray(v)
+    b = np.ascontiguousarray(b, dtype=v.dtype).reshape(-1)
+    n = len(rp) - 1
+    x = np.zeros(n * rhs, dtype=v.dtype)
+    getattr(lib(), f"orc_csr_solve_{_sfx(v.dtype)}")(n, _pi(_i32(rp)), _pi(_i32(ci)), _pv(v), rhs, _pv(b), _pv(x),
+                                                     int(upper), int(unit), int(fused))
+    return x
+
+
 def csr_forward(rp, ci, v, b, rhs=1, fused=True):
     v = np.ascontiguousarray(v)
     n = len(rp) - 1

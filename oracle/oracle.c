@@ -324,6 +324,36 @@ void orc_csr_forward_##SFX(int n, const int *rp, const int *ci, const T *v,     
 ORC_DEFINE_FWD(double, d, fma)
 ORC_DEFINE_FWD(float, s, fmaf)
 
+/* General CSR triangular solve in stored order (see oracle.h): lower rows
+ * visited 0..n-1 with the diagonal last, upper rows n-1..0 with the diagonal
+ * first; with unit = 1 no diagonal is stored and x = b - s. */
+#define ORC_DEFINE_TRI(T, SFX, FMA)                                                  \
+void orc_csr_solve_##SFX(int n, const int *rp, const int *ci, const T *v, int rhs,  \
+                         const T *b, T *x, int upper, int unit, int fused)          \
+{                                                                                    \
+    int t, j, k;                                                                     \
+    for (t = 0; t < n; ++t) {                                                        \
+        int i = upper ? n - 1 - t : t;                                               \
+        int lo = rp[i], hi = rp[i + 1], dg = -1;                                     \
+        if (!unit) {                                                                 \
+            if (upper) dg = lo++;                                                    \
+            else dg = --hi;                                                          \
+        }                                                                            \
+        for (k = 0; k < rhs; ++k) {                                                  \
+            T s = (T)0;                                                              \
+            for (j = lo; j < hi; ++j) {                                              \
+                T xv = x[(size_t)ci[j] * rhs + k];                                   \
+                if (fused) s = FMA(xv, v[j], s);                                     \
+                else { T p = xv * v[j]; s = s + p; }                                 \
+            }                                                                        \
+            T r = b[(size_t)i * rhs + k] - s;                                        \
+            x[(size_t)i * rhs + k] = unit ? r : r / v[dg];                           \
+        }                                                                            \
+    }                                                                                \
+}
+ORC_DEFINE_TRI(double, d, fma)
+ORC_DEFINE_TRI(float, s, fmaf)
+
 /* Adjoint (see oracle.h): executor BACKWARD on the CSR of L read as CSC of L^T */
 void orc_adjoint_d(int n, const int *rp, const int *ci, const double *v,
                    const double *g, double *y, int fused)

@@ -98,6 +98,16 @@ void orc_csr_forward_d(int n, const int *rp, const int *ci, const double *v,
 void orc_csr_forward_s(int n, const int *rp, const int *ci, const float *v,
                        int rhs, const float *b, float *x, int fused);
 
+/* Lower or upper CSR triangular solve in stored order, the upper-triangular
+ * and implicit-unit-diagonal generalisation of orc_csr_forward (the
+ * reference's SUBSTITUTION_BACKWARD mode, serialref.h:73-102, is its
+ * column-driven form).  Lower: diagonal last, rows 0..n-1.  Upper: diagonal
+ * first, rows n-1..0.  unit = 1: no diagonal stored, x_i = b_i - s. */
+void orc_csr_solve_d(int n, const int *rp, const int *ci, const double *v, int rhs,
+                     const double *b, double *x, int upper, int unit, int fused);
+void orc_csr_solve_s(int n, const int *rp, const int *ci, const float *v, int rhs,
+                     const float *b, float *x, int upper, int unit, int fused);
+
 /* ---- Adjoint: y = L^{-T} g computed column-driven on the CSR of L, i.e. the
  *      executor's BACKWARD mode (serialref.h:73-102) reading row i of L as
  *      column i of L^T (diagonal last).  Then grad_val[k] = -y[row_k]*x[col_k]

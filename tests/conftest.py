@@ -49,3 +49,27 @@ def random_lower(n, max_off=4, band=50, seed=0, dtype=np.float64, dup=False, shu
         v += vals.tolist() + [1.0 + float(np.abs(vals).sum())]
         rp.append(len(ci))
     return np.array(rp, np.int32), np.array(ci, np.int32), np.array(v, dtype)
+
+
+def upper_from_lower(rp, ci, v):
+    """U[i][j] = L[n-1-i][n-1-j]: an upper CSR with the diagonal first and
+    ascending columns (rows of L reversed, entries reversed)."""
+    n = len(rp) - 1
+    urp, uci, uv = [0], [], []
+    for i in range(n):
+        r = n - 1 - i
+        cols = [n - 1 - int(c) for c in ci[rp[r]:rp[r + 1]]][::-1]
+        vals = [float(x) for x in v[rp[r]:rp[r + 1]]][::-1]
+        uci += cols
+        uv += vals
+        urp.append(len(uci))
+    return np.array(urp, np.int32), np.array(uci, np.int32), np.array(uv)
+
+
+def drop_diagonal(rp, ci, v, upper=False):
+    n = len(rp) - 1
+    keep = np.ones(len(ci), bool)
+    for i in range(n):
+        keep[rp[i] if upper else rp[i + 1] - 1] = False
+    counts = np.diff(rp) - 1
+    return np.r_[0, np.cumsum(counts)].astype(np.int32), ci[keep].astype(np.int32), v[keep]

@@ -27,7 +27,7 @@ import pytest
 import scipy.linalg
 
 import oracle
-from conftest import ROOT, random_lower, rel_maxnorm
+from conftest import ROOT, random_lower, rel_maxnorm, upper_from_lower, drop_diagonal
 
 GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))["cases"]
 MTX = os.path.join(ROOT, "tests", "golden", "mtx")
@@ -307,3 +307,70 @@ def test_reversed_transpose_forward_equals_backward_executor(dup):
     for fused in (False, True):
         y = oracle.csr_forward(rT, cT, vT, g[::-1].copy(), fused=fused)[::-1]
         assert np.array_equal(y, oracle.adjoint(rp, ci, v, g, fused=fused))
+
+
+# ------------------------------------------------ upper / unit-diagonal solves
+def _dense(n, rp, ci, v):
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(rp[i], rp[i + 1]):
+            A[i, ci[j]] += v[j]
+    return A
+
+
+def test_csr_solve_lower_is_the_forward_substitution():
+    rp, ci, v = random_lower(500, max_off=6, band=50, seed=21, dup=True)
+    b = np.random.default_rng(1).uniform(-1, 1, 500)
+    for fused in (True, False):
+        assert np.array_equal(oracle.csr_solve(rp, ci, v, b, fused=fused), oracle.csr_forward(rp, ci, v, b, fused=fused))
+
+
+@pytest.mark.parametrize("unit", [False, True])
+@pytest.mark.parametrize("upper", [False, True])
+def test_csr_solve_against_dense_triangular_solve(upper, unit):
+    from scipy.linalg import solve_triangular
+    n = 400
+    rp, ci, v = random_lower(n, max_off=5, band=60, seed=22)
+    if upper:
+        rp, ci, v = upper_from_lower(rp, ci, v)
+    if unit:
+        rp, ci, v = drop_diagonal(rp, ci, v, upper=upper)
+    b = np.random.default_rng(2).uniform(-1, 1, (n, 3))
+    x = oracle.csr_solve(rp, ci, v, b.reshape(-1), rhs=3, upper=upper, unit=unit).reshape(n, 3)
+    A = _dense(n, rp, ci, v)
+    if unit:
+        A += np.eye(n)
+    ref = solve_triangular(A, b, lower=not upper, unit_diagonal=unit)
+    assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) <= 1e-12
+
+
+def test_csr_solve_closed_forms():
+    n = 200
+    # unit upper bidiagonal with -1 on the superdiagonal, b = 1: x_i = n - i
+    rp = np.r_[np.arange(0, n), n - 1].astype(np.int32)  # rows 0..n-2 one entry, row n-1 none
+    ci = np.arange(1, n).astype(np.int32)
+    v = -np.ones(n - 1)
+    x = oracle.csr_solve(rp, ci, v, np.ones(n), upper=True, unit=True)
+    assert np.array_equal(x, (n - np.arange(n)).astype(float))
+    # lower with stored diagonal 2 and -1 below, b = 1: x_i = 1 - 2^-(i+1)
+    rp = np.r_[0, 1, np.arange(3, 2 * n, 2)].astype(np.int32)
+    ci = np.r_[0, np.c_[np.arange(0, n - 1), np.arange(1, n)].reshape(-1)].astype(np.int32)
+    v = np.r_[2.0, np.tile([-1.0, 2.0], n - 1)]
+    x = oracle.csr_solve(rp, ci, v, np.ones(n))
+    assert np.allclose(x, 1 - 0.5 ** (np.arange(n) + 1), rtol=0, atol=1e-15)
+
+
+def test_upper_solve_planted_diagonal_mistake_is_caught():
+    """A restatement that takes the upper row's diagonal from its last entry
+    (the lower convention) fails the dense check."""
+    from scipy.linalg import solve_triangular
+    n = 120
+    rp, ci, v = upper_from_lower(*random_lower(n, max_off=4, band=20, seed=23))
+    b = np.random.default_rng(3).uniform(-1, 1, n)
+    x = np.zeros(n)
+    for i in range(n - 1, -1, -1):
+        d = rp[i + 1] - 1  # wrong: the diagonal is rp[i]
+        s = sum(v[j] * x[ci[j]] for j in range(rp[i], rp[i + 1]) if j != d)
+        x[i] = (b[i] - s) / v[d]
+    ref = solve_triangular(_dense(n, rp, ci, v), b, lower=False)
+    assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) > 1e-6
