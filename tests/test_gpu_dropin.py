@@ -84,3 +84,23 @@ def test_driver_appends_result_csv(cuda, tmp_path):
         assert f[0] == str(mtx) and int(f[1]) == m and int(f[2]) == nnzL and int(f[4]) == layer
         assert f[3] == f"{nnzL / m:4.2f}" and f[5] == f"{par:4.2f}"
         assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
+
+
+def test_sweep_over_a_matrix_directory(cuda, tmp_path):
+    """The run.sh-style sweep: every valid fixture under two algorithms, one
+    result.csv row (after its header line) per run, none failing validation."""
+    from capellinisptrsv_amd import sweep
+    src = os.path.join(ROOT, "tests", "golden", "mtx")
+    mats = tmp_path / "matrices"
+    mats.mkdir()
+    good = [f for f in sorted(os.listdir(src)) if f.endswith(".mtx") and not f.startswith("err_")]
+    for f in good:
+        (mats / f).write_bytes(open(os.path.join(src, f), "rb").read())
+    out = tmp_path / "out"
+    assert sweep.main([str(mats), "-a", "wrtfst,mix", "--out-dir", str(out)]) == 0
+    rows = (out / "result.csv").read_text().splitlines()
+    data = rows[1::2]
+    assert len(data) == 2 * len(good)
+    for row in data:
+        f = [s.strip() for s in row.split(",")]
+        assert os.path.basename(f[0]) in good and f[7] != "-1.00"

@@ -211,3 +211,20 @@ def test_error_strings():
     for code in range(0, -14, -1):
         assert L.cap_strerror(code) not in (b"", b"unknown error"), code
     assert L.cap_version().startswith(b"capellini")
+
+
+def test_sweep_lists_matrices_and_commands(tmp_path):
+    """The sweep driver (run.sh equivalent): directory and list-file sources,
+    lookup by name under --root, one command per (matrix, algorithm)."""
+    from capellinisptrsv_amd import sweep
+    d = os.path.join(ROOT, "tests", "golden", "mtx")
+    mats = sweep.matrices(d)
+    assert mats == sorted(mats) and len(mats) == len([f for f in os.listdir(d) if f.endswith(".mtx")])
+    lst = tmp_path / "matrix_name.txt"
+    lst.write_text("/nowhere/general_real/general_real.mtx\n\n# comment\n" + mats[0] + "\n")
+    got = sweep.matrices(str(lst), root=d)
+    assert got == [os.path.join(d, "general_real.mtx"), mats[0]]
+    cmds = sweep.commands(got, ["wrtfst", "mix"], f32=True, values=True)
+    assert len(cmds) == 4 and cmds[0][0].endswith("sptrsv_main_f32")
+    assert cmds[1][1:] == [got[0], "-a", "mix", "--values"]
+    assert sweep.main([d, "-a", "wrtfst,csc", "--dry-run"]) == 0
