@@ -194,7 +194,8 @@ def main():
         out["cpu_baseline"] = {"value": round(2.0 * nnz / (cpu_ms * 1e6), 4), "unit": "GFLOPS", "cores": 1,
                                "kind": kind, "ms_per_solve": round(cpu_ms, 3),
                                "sample": f"{len(times)} full solves of the same n=2,000,000 matrix (median), "
-                                         f"serial column-driven executor, host {os.cpu_count()} cores visible"}
+                                         f"serial column-driven executor, one core of a {_cpu_model()} "
+                                         f"({os.cpu_count()} cores visible)"}
         ref = xc
     else:
         import oracle
@@ -209,6 +210,17 @@ def main():
     if not args.quick:
         out["extra_workloads"] = extra_workloads(args)
     print(json.dumps(out), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "host CPU"
 
 
 def _time_gpu(fn, steps, warmup=3):
