@@ -32,7 +32,8 @@ namespace cap {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;  // 4 wavefronts per workgroup
-constexpr int kWin = 4;      // dependencies polled concurrently per lane
+constexpr int kWin = 4;      // dependencies polled concurrently per lane (flag protocol)
+
 
 template <typename T>
 struct VBits;
