@@ -212,6 +212,26 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _cpu_serial_ms(rp, ci, v, b, seconds):
+    """Median time of the reference's serial executor (or the oracle's port of
+    it when the reference build is absent) on one host core."""
+    import numpy as np
+
+    import oracle
+    n, nnz = len(rp) - 1, len(ci)
+    cr, cp, cv = oracle.transpose(n, n, nnz, rp, ci, v)
+    indeg = oracle.csc_analyser(cr, n, nnz)
+    ref = oracle.ref_available("d64")
+    run = oracle.ref_csc_executor if ref else oracle.csc_executor
+    times, start = [], time.perf_counter()
+    run(cp, cr, cv, indeg, n, n, 0, 1, b)
+    while time.perf_counter() - start < seconds and len(times) < 1000:
+        t = time.perf_counter()
+        run(cp, cr, cv, indeg, n, n, 0, 1, b)
+        times.append(time.perf_counter() - t)
+    return ("reference" if ref else "port"), float(np.median(times)) * 1e3, len(times)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -265,6 +285,10 @@ def extra_workloads(args):
         res[name] = {"n": n, "nnzL": nnz, "levels": api.matrix_layer(n, n, nnz, rp, ci, v)[0], "plan": p.info(),
                      "ms_per_solve": round(ms, 4), "gflops": round(2 * nnz / (ms * 1e6), 3),
                      "max_abs_err_vs_oracle": err}
+        if not args.no_cpu:
+            kind, cpu_ms, reps = _cpu_serial_ms(rp, ci, v, b.cpu().numpy(), seconds=3.0)
+            res[name]["cpu_serial"] = {"ms_per_solve": round(cpu_ms, 3), "kind": kind, "cores": 1, "solves": reps,
+                                       "gpu_speedup": round(cpu_ms / ms, 2)}
         p.close()
 
     # differentiable solve: n = 1,000,000 level matrix, forward + adjoint
