@@ -740,6 +740,10 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                         }
                         const unsigned long long ready = __ballot(rdy);
                         const int upto = min(cnt, ~ready ? (int)__builtin_ctzll(~ready) : kWave);
+                        // a row with at most two entries left is about to be on
+                        // the critical path: raise its issue priority early
+                        // (measured 110 -> 104 ms on the deep band)
+                        if (cnt - upto <= 2) __builtin_amdgcn_s_setprio(1);
                         if (upto == used) {
                             if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
                                 failed = true;
