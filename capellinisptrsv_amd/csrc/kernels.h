@@ -936,40 +936,63 @@ __global__ void __launch_bounds__(kBlock) k_indegree(const int *__restrict__ idx
     if (g < nnz) atomic_add_agent(deg + idx[g], 1);
 }
 
-// sptrsv_syncfree_cuda_executor_update (spts_syncfree_cuda2.h:24-86) at
-// wave64: one wavefront per column; wait until only the diagonal's own
-// count remains, x_i = (b_i - left_sum_i) / diag, then every lane scatters
-// left_sum[r] += x_i * v and releases one count of row r.  FORWARD takes
-// the diagonal from the first entry of the column, BACKWARD from the last.
-template <typename T>
+// sptrsv_syncfree_cuda_executor_update (spts_syncfree_cuda2.h:24-86),
+// redesigned for short columns: a group of G lanes per column (G chosen by
+// the host from the mean column length; the reference uses one warp per
+// column), kBlock / G columns per workgroup ticket.  A column waits until
+// only its diagonal's own count remains, computes x_i = (b_i - left_sum_i) /
+// diag, then its lanes scatter left_sum[r] += x_i * v and release one count
+// of each row r.  Columns of one wavefront can depend on each other, so the
+// loop is write-first with a wave-uniform exit (as in wrtfst_row).
+// Ordering without cache-wide fences: the left-sum atomics return their old
+// values and the in-degree decrements are issued only after those returned
+// (a data dependency), so a reader that sees the count drop to 1 finds every
+// contribution performed; it then reads left_sum with an agent-scope atomic
+// load.  FORWARD takes the diagonal from the first entry of the column,
+// BACKWARD from the last.
+template <typename T, int G>
 __global__ void __launch_bounds__(kBlock)
     k_csc(const int *__restrict__ cp, const int *__restrict__ ri, const T *__restrict__ cv,
           int *__restrict__ deg, T *__restrict__ left_sum, const T *__restrict__ b, T *__restrict__ x, int m,
           int backward, SyncCtx sc, unsigned *__restrict__ ticket)
 {
     typedef typename VBits<T>::U U;
-    const int lane = threadIdx.x & (kWave - 1);
-    ticket_loop(ticket, kBlock / kWave, m, [&](int base) {
-        int col = base + (int)(threadIdx.x / kWave);
-        if (col >= m) return;
-        if (backward) col = m - 1 - col;
-        const int dpos = backward ? cp[col + 1] - 1 : cp[col];
+    constexpr int kCols = kBlock / G;
+    const int g = threadIdx.x & (G - 1);
+    ticket_loop(ticket, kCols, m, [&](int base) {
+        int col = base + (int)(threadIdx.x / G);
+        bool done = col >= m;
+        if (!done && backward) col = m - 1 - col;
+        int dpos = 0, start = 0, stop = 0;
+        if (!done) {
+            dpos = backward ? cp[col + 1] - 1 : cp[col];
+            start = backward ? cp[col] : cp[col] + 1;
+            stop = backward ? cp[col + 1] - 1 : cp[col + 1];
+        }
         int idle = 0;
-        bool failed = false;
-        while (ld_agent(deg + col) != 1)
-            if (give_up(sc, ++idle)) {
-                failed = true;
-                break;
+        do {  // wave-uniform exit
+            if (!done) {
+                const bool ready = ld_agent(deg + col) == 1;
+                const bool failed = !ready && give_up(sc, ++idle);
+                if (ready || failed) {
+                    const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
+                    const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
+                    T token = (T)0;
+                    for (int j = start + g; j < stop; j += G)
+                        token += __hip_atomic_fetch_add(left_sum + ri[j], xi * cv[j], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    // the decrements depend on the returned sums: `one` passes
+                    // through an asm that consumes token, so the compiler waits
+                    // for the returning atomics before issuing the decrements
+                    int one = 1;
+                    asm volatile("" : "+v"(one) : "v"(token));
+                    for (int j = start + g; j < stop; j += G) atomic_add_agent(deg + ri[j], -one);
+                    if (g == 0) x[col] = xi;
+                    asm volatile("" ::: "memory");
+                    done = true;
+                }
             }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
-        const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
-        const int start = backward ? cp[col] : cp[col] + 1;
-        const int stop = backward ? cp[col + 1] - 1 : cp[col + 1];
-        for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(left_sum + ri[j], xi * cv[j]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        for (int j = start + lane; j < stop; j += kWave) atomic_add_agent(deg + ri[j], -1);
-        if (lane == 0) x[col] = xi;
+        } while (__any(!done));
     });
 }
 

@@ -428,9 +428,18 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
                                                     rl);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
-        const unsigned nb = grid_for(p, ((long long)n * kWave + kBlock - 1) / kBlock);
-        k_csc<T><<<nb, kBlock, 0, st>>>(p->cp, p->cri, (const T *)p->cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc,
-                                        p->ticket);
+        // lanes per column from the mean column length (diagonal included)
+        const double avg = (double)p->nnz / std::max(1, n);
+        const int G = avg <= 4.0 ? 4 : avg <= 16.0 ? 16 : kWave;
+        const unsigned nb = grid_for(p, ((long long)n * G + kBlock - 1) / kBlock);
+        const int *cp = p->cp, *cr = p->cri;
+        const T *cv = (const T *)p->cv;
+        if (G == 4)
+            k_csc<T, 4><<<nb, kBlock, 0, st>>>(cp, cr, cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc, p->ticket);
+        else if (G == 16)
+            k_csc<T, 16><<<nb, kBlock, 0, st>>>(cp, cr, cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc, p->ticket);
+        else
+            k_csc<T, kWave><<<nb, kBlock, 0, st>>>(cp, cr, cv, p->deg, (T *)p->left_sum, b, x, n, 0, sc, p->ticket);
     } else {
         return CAP_ERR_UNSUPPORTED;
     }
