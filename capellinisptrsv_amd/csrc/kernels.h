@@ -13,9 +13,11 @@
 //    VALUE: the readiness array -- x, or a staged copy in uncached memory --
 //    starts as a signalling-NaN sentinel that arithmetic can never produce; a
 //    relaxed agent-scope 32/64-bit atomic load both tests and fetches it) or
-//    through an int flag per row holding the solve's epoch (SYNC FLAG:
-//    release store / acquire fence, agent scope; the epoch makes resets
-//    between solves unnecessary).
+//    through an int flag per row holding the solve's epoch (SYNC FLAG: the
+//    value is published with a returning agent-scope exchange and the flag
+//    store depends on it; readers load the value with an agent-scope atomic
+//    after seeing the flag -- no cache-wide release/acquire fences; the
+//    epoch makes resets between solves unnecessary).
 //  * Thread-per-row kernels use the write-first loop of
 //    spts_syncfree_cuda_csr_wrt.h:37-55: a lane publishes x[i] from inside the
 //    loop in which it polls, so lanes of one wavefront that depend on each
@@ -203,8 +205,15 @@ __device__ __forceinline__ void publish(T *x, const SyncCtx &sc, long long o, T 
             publish_value(x, o, xi);
         }
     } else {
-        x[o] = xi;
-        __hip_atomic_store(sc.flags + o, sc.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // flag protocol without cache-wide fences: the value goes out as a
+        // returning agent-scope exchange, and the flag store depends on the
+        // returned word, so the flag is issued only once the value is in place
+        typedef typename VBits<T>::U U;
+        const U old = __hip_atomic_exchange(reinterpret_cast<U *>(x) + o, as_bits(xi), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+        int ep = sc.epoch;
+        asm volatile("" : "+v"(ep) : "v"(old));
+        __hip_atomic_store(sc.flags + o, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -225,8 +234,7 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
     } else {
         while (ld_agent(sc.flags + o) != sc.epoch)
             if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        return as_val(xu[o]);
+        return as_val(ld_agent(xu + o));  // issued after the flag was seen
     }
 }
 
@@ -242,8 +250,7 @@ __device__ __forceinline__ bool try_value(const T *x, const SyncCtx &sc, long lo
         return raw != VBits<T>::kSentinel;
     }
     if (ld_agent(sc.flags + o) != sc.epoch) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *out = as_val(xu[o]);
+    *out = as_val(ld_agent(xu + o));  // issued after the flag was seen
     return true;
 }
 
@@ -439,12 +446,11 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 #pragma unroll
                 for (int t = 0; t < kWin; ++t)
                     if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) any = true;
-                if (any) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (any) {  // values read after their flags were seen (agent-scope loads)
 #pragma unroll
                     for (int t = 0; t < kWin; ++t)
                         if (t >= used && t < cnt && !rdy[t] && fl[t] == sc.epoch) {
-                            xw[t] = xu[(long long)cw[t] * k + q];
+                            xw[t] = ld_agent(xu + (long long)cw[t] * k + q);
                             rdy[t] = true;
                         }
                 }
@@ -1021,13 +1027,19 @@ __global__ void __launch_bounds__(kBlock)
                 const bool ready = ld_agent(cnt + i) == 0;
                 const bool failed = !ready && give_up(sc, ++idle);
                 if (ready || failed) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    // ordered as in k_csc: left_sum read with an agent-scope
+                    // load after the count was seen; the count decrements
+                    // depend on the returned left-sum atomics
                     const double ls = as_val(ld_agent(reinterpret_cast<const unsigned long long *>(left_sum) + i));
                     const double y = failed ? as_val(VBits<double>::kQNaN) : (g[i] - ls) / val[d];
                     grad_b[i] = y;
-                    for (int j = a; j < d; ++j) atomic_add_agent(left_sum + ci[j], y * val[j]);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    for (int j = a; j < d; ++j) atomic_add_agent(cnt + ci[j], -1);
+                    double token = 0.0;
+                    for (int j = a; j < d; ++j)
+                        token += __hip_atomic_fetch_add(left_sum + ci[j], y * val[j], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    int one = 1;
+                    asm volatile("" : "+v"(one) : "v"(token));
+                    for (int j = a; j < d; ++j) atomic_add_agent(cnt + ci[j], -one);
                     if (grad_val)
                         for (int j = a; j <= d; ++j) grad_val[j] = -y * xfwd[ci[j]];
                     asm volatile("" ::: "memory");

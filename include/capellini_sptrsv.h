@@ -82,8 +82,10 @@ extern "C" {
 #define CAP_SYNC_VALUE 1 /* the value is the flag: x (or the staged readiness
                             array) starts as a signalling-NaN sentinel;
                             relaxed agent-scope 64/32-bit atomics */
-#define CAP_SYNC_FLAG 2  /* separate int flags published with a release store
-                            and read with acquire loads (agent scope) */
+#define CAP_SYNC_FLAG 2  /* separate int flags (the solve's epoch): the value is
+                            stored first (a returning agent-scope exchange),
+                            then the flag; readers load the value after the
+                            flag */
 
 typedef struct cap_opts {
     int algo;         /* CAP_ALGO_* (0 = auto) */
