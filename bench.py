@@ -162,6 +162,8 @@ def main():
                      "kernel": "k_wrtfst" if info["algo"] == "wrtfst" else "k_" + info["algo"],
                      "kernel_ms": round(kernel_ms, 5), "algorithmic_bytes": alg_bytes},
         "per_level_ns": round(kernel_ms * 1e6 / levels, 1),
+        # latency check: the per-level time 40 % of the HBM roofline would need
+        "per_level_ns_for_40pct_roofline": round(alg_bytes / (0.4 * HBM_PEAK_GBS * 1e9) / levels * 1e9, 1),
         "cold_ms_per_step": round(float(np.median(cold)), 5),
         "fp32_ms_per_step": round(fp32_ms, 5),
         "refined_ms_per_solve": round(refined_ms, 5), "refined_iterations": iters,
@@ -347,7 +349,43 @@ def extra_workloads(args):
     }
     bt.close()
     res["rocsparse_csrsv"] = rocsparse_comparison()
+    res["algorithms_wide_level_2m"] = algorithm_table()
     return res
+
+
+def algorithm_table():
+    """Every solve algorithm of the library on the wide-level matrix (the
+    reference's four drop-in paths and the one-workgroup kernel), each with
+    its error against the oracle."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from capellinisptrsv_amd import Plan, api
+
+    log("algorithm table")
+    dev = torch.device("cuda:0")
+    rp, ci, v = api.make_config(1, seed=0)
+    n = len(rp) - 1
+    d = [torch.from_numpy(a).to(dev) for a in (rp, ci, v)]
+    b = torch.from_numpy(api.uniform(1, n)).to(dev)
+    x = torch.empty_like(b)
+    ref = oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True)
+    out = {}
+    for name, kw in (("wrtfst_value", dict(algo="wrtfst", sync="value")),
+                     ("wrtfst_value_lds", dict(algo="wrtfst", sync="value", lds_window=True)),
+                     ("wrtfst_flag", dict(algo="wrtfst", sync="flag")),
+                     ("twopart_value", dict(algo="twopart", sync="value")),
+                     ("mix_value", dict(algo="mix", sync="value")),
+                     ("csc", dict(algo="csc")),
+                     ("local", dict(algo="local"))):
+        p = Plan(*d, **kw)
+        ms = _time_gpu(lambda: p.solve(b, out=x), 5 if name == "local" else 20, warmup=2)
+        err = float(np.max(np.abs(x.cpu().numpy() - ref)) / np.max(np.abs(ref)))
+        out[name] = {"ms_per_solve": round(ms, 4), "gflops": round(2 * len(ci) / (ms * 1e6), 3),
+                     "rel_err_vs_oracle": err, "status": p.status()}
+        p.close()
+    return out
 
 
 def rocsparse_comparison():
