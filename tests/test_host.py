@@ -228,3 +228,16 @@ def test_sweep_lists_matrices_and_commands(tmp_path):
     assert len(cmds) == 4 and cmds[0][0].endswith("sptrsv_main_f32")
     assert cmds[1][1:] == [got[0], "-a", "mix", "--values"]
     assert sweep.main([d, "-a", "wrtfst,csc", "--dry-run"]) == 0
+
+
+@pytest.mark.parametrize("argv,env", [(["--gpus", "2"], {}), (["--gpus", "8"], {}), ([], {"WORLD_SIZE": "4"})])
+def test_bench_refuses_more_than_one_gpu(argv, env):
+    """bench.py is a one-GPU benchmark: N > 1 exits with a message before any
+    GPU work (exit status 2)."""
+    import subprocess
+    import sys
+    e = dict(os.environ, **env)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *argv], capture_output=True, text=True,
+                       env=e, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
