@@ -77,7 +77,7 @@ SIGNATURES = {
     "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
     "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI]),
-    "cap_batch_info": (c_int, [c_void_p, PI]),
+    "cap_batch_info": (c_int, [c_void_p, PI, PI]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "cap_plan_solve_refined": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, PI, PD, c_void_p]),

@@ -29,6 +29,10 @@ namespace {
 // default is the value measured best on MI355X for the banded workloads.
 constexpr int kMixBorderDefault = 16;
 constexpr int kSpinLimitDefault = 1 << 24;  // polls without progress before the watchdog fires
+// Persistent grids hold at most 4 workgroups per compute unit.  A batch
+// solve with 8 (2048 resident workgroups, one right-hand side) stopped
+// making progress, so larger values are refused.
+constexpr int kMaxBlocksPerCu = 4;
 constexpr int kTickets = 4;
 // The autotuner times CAP_ALGO_LOCAL when rows per level (n / levels) are at
 // most this: one compute unit then keeps up with the level-by-level pace.
@@ -64,7 +68,7 @@ struct cap_plan_s {
     int spin_limit = kSpinLimitDefault;
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
-    // batch: level-ordered block schedule for blocks of sched_rows rows
+    // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
     int n_sched = 0, sched_rows = 0;
     // CAP_ALGO_LOCAL: row segments [x, y) with bandwidth z, one per matrix,
@@ -386,7 +390,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
         const int rows_per_block = kBlock >> l;
-        // a batch's level-ordered block schedule, when built for this block size
+        // a batch's block schedule, when built for this block size
         const bool use_sched = p->sched && p->sched_rows == rows_per_block;
         const int2 *sched = use_sched ? p->sched : nullptr;
         const int n_sched = use_sched ? p->n_sched : 0;
@@ -742,6 +746,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     if (!out || n < 0 || nnz < 0 || (vt != CAP_F64 && vt != CAP_F32) || !rp || (nnz > 0 && (!ci || !val)))
         return CAP_ERR_ARG;
     if (!o.unit_diag && nnz < n) return CAP_ERR_ARG;
+    if (o.blocks_per_cu < 0 || o.blocks_per_cu > kMaxBlocksPerCu) return CAP_ERR_ARG;
     *out = nullptr;
     if (o.upper || o.unit_diag) {
         // convert on the host, then create a plan over the owned lower matrix
@@ -1195,12 +1200,9 @@ extern "C" int cap_plan_solve_refined(cap_plan p, const double *b, double *x, in
 // ---------------------------------------------------------------- batches
 namespace {
 
-// Level-ordered schedule of a batch: every matrix is cut into blocks of
-// `rows` consecutive rows; block keys are the running maximum of the row
-// levels of their matrix at the block's first row (non-decreasing along a
-// matrix), and a stable sort by key interleaves the matrices so that all of
-// them advance level by level in one launch instead of one after another.
-// Within a matrix the order of the blocks is kept.
+// Block schedule of a batch: every matrix is cut into blocks of `rows`
+// consecutive rows, and a stable sort by key interleaves the matrices so
+// that all of them advance in one launch instead of one after another.
 int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipStream_t st)
 {
     const int n = p->n, nnz = p->nnz, n_mat = (int)ro.size() - 1;
@@ -1209,15 +1211,25 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipSt
     CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
     cap_matrix_layer(n, rp.data(), ci.data(), nullptr, nullptr, lv.data());
     struct Blk {
-        int key, lo, len;
+        double key;
+        int lo, len;
     };
+    // Blocks are handed out by relative progress: a block's key is the
+    // running maximum level of its rows divided by its matrix's depth, so
+    // every matrix advances through its levels at the same pace and the
+    // deepest one does not wait behind the early levels of all the others.
+    // Keys never decrease along a matrix's rows and the sort is stable, so
+    // each matrix keeps its row order.
     std::vector<Blk> blks;
     blks.reserve((size_t)n / rows + n_mat + 1);
     for (int m = 0; m < n_mat; ++m) {
+        int depth = 1;
+        for (int r = ro[m]; r < ro[m + 1]; ++r) depth = std::max(depth, lv[r] + 1);
         int run = 0;
         for (int lo = ro[m]; lo < ro[m + 1]; lo += rows) {
             const int hi = std::min(lo + rows, ro[m + 1]);
-            blks.push_back({std::max(run, lv[lo]), lo, hi - lo});
+            run = std::max(run, lv[lo]);
+            blks.push_back({(double)run / depth, lo, hi - lo});
             for (int r = lo; r < hi; ++r) run = std::max(run, lv[r]);
         }
     }
@@ -1232,16 +1244,19 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipSt
     return CAP_OK;
 }
 
-// A batch runs either the level-scheduled thread-per-row kernel over all
-// matrices or one workgroup per matrix; unless the caller named one, both are
-// timed on the batch itself (b = 0) and the faster is kept.
-int choose_batch_algo(cap_plan p, int requested, int k, hipStream_t st)
+// A batch runs either the scheduled thread-per-row kernel over all matrices
+// or one workgroup per matrix; unless the caller named one, both are timed
+// on the batch itself (b = 0) and the faster is kept.  Unless the caller set
+// blocks_per_cu, the thread-per-row grid is timed at 1, 2 and 4 workgroups
+// per compute unit (the schedule's pace depends on how many blocks are in
+// flight); a size must be 2% faster than a smaller one to win.
+int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, hipStream_t st)
 {
-    if (requested == CAP_ALGO_LOCAL || requested == CAP_ALGO_WRTFST) {
+    if (requested == CAP_ALGO_LOCAL || (requested == CAP_ALGO_WRTFST && bpc_fixed)) {
         p->algo = requested;
         return CAP_OK;
     }
-    if (requested != CAP_ALGO_AUTO) return CAP_ERR_ARG;
+    if (requested != CAP_ALGO_AUTO && requested != CAP_ALGO_WRTFST) return CAP_ERR_ARG;
     const size_t bytes = (size_t)p->n * k * p->vbytes + 8;
     void *b = nullptr, *x = nullptr;
     CAP_HIP_TRY(hipMalloc(&b, bytes));
@@ -1251,8 +1266,24 @@ int choose_batch_algo(cap_plan p, int requested, int k, hipStream_t st)
     }
     int rc = hipMemsetAsync(b, 0, bytes, st) == hipSuccess ? CAP_OK : CAP_ERR_HIP;
     float t_w = 1e30f, t_l = 1e30f;
-    if (rc == CAP_OK) rc = time_candidate(p, CAP_ALGO_WRTFST, p->sync, b, x, k, st, 1e30f, &t_w);
-    if (rc == CAP_OK) rc = time_candidate(p, CAP_ALGO_LOCAL, CAP_SYNC_VALUE, b, x, k, st, t_w, &t_l);
+    int best_bpc = p->bpc;
+    std::vector<int> sizes = {1, 2, 4};
+    if (bpc_fixed) sizes = {p->bpc};
+    for (int bpc : sizes) {
+        if (rc != CAP_OK) break;
+        const int keep = p->bpc;
+        p->bpc = bpc;
+        float t = 1e30f;
+        rc = time_candidate(p, CAP_ALGO_WRTFST, p->sync, b, x, k, st, t_w, &t);
+        p->bpc = keep;
+        if (t < (t_w < 1e30f ? 0.98f : 1.0f) * t_w) {
+            t_w = t;
+            best_bpc = bpc;
+        }
+    }
+    p->bpc = best_bpc;
+    if (rc == CAP_OK && requested == CAP_ALGO_AUTO)
+        rc = time_candidate(p, CAP_ALGO_LOCAL, CAP_SYNC_VALUE, b, x, k, st, t_w, &t_l);
     (void)hipStreamSynchronize(st);
     (void)hipFree(b);
     (void)hipFree(x);
@@ -1335,7 +1366,7 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     }
     p->owns_matrix = true;
     if ((rc = build_batch_schedule(p, ro, kBlock >> lpr_log2_for(k), st)) != CAP_OK ||
-        (rc = ensure_segments(p, &ro)) != CAP_OK || (rc = choose_batch_algo(p, o.algo, k, st)) != CAP_OK) {
+        (rc = ensure_segments(p, &ro)) != CAP_OK || (rc = choose_batch_algo(p, o.algo, o.blocks_per_cu > 0, k, st)) != CAP_OK) {
         cap_plan_destroy(p);
         return rc;
     }
@@ -1347,10 +1378,11 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     return CAP_OK;
 }
 
-int cap_batch_info(cap_batch bt, int *algo)
+int cap_batch_info(cap_batch bt, int *algo, int *blocks_per_cu)
 {
     if (!bt) return CAP_ERR_ARG;
     if (algo) *algo = bt->plan->algo;
+    if (blocks_per_cu) *blocks_per_cu = bt->plan->bpc;
     return CAP_OK;
 }
 

@@ -147,22 +147,26 @@ class Batch:
     """Many matrices (concatenated, local column indices) with k interleaved RHS."""
 
     def __init__(self, row_off: torch.Tensor, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor,
-                 k: int, sync="value", validate: bool = True, algo="auto", stream=None):
+                 k: int, sync="value", validate: bool = True, algo="auto", blocks_per_cu: int = 0, stream=None):
         """algo: auto (both timed at creation) | wrtfst (level-scheduled
-        thread-per-row over all matrices) | local (one workgroup per matrix)."""
+        thread-per-row over all matrices) | local (one workgroup per matrix).
+        blocks_per_cu: persistent workgroups per compute unit (0: chosen
+        at creation)."""
         self.row_off, self.row_ptr, self.col_idx, self.values = row_off, row_ptr, col_idx, values
         self.k = k
         self.n = row_ptr.numel() - 1
         self.dtype = values.dtype
-        opts = N.default_opts(algo=algo, sync=sync, validate=int(validate), host_inputs=0)
+        opts = N.default_opts(algo=algo, sync=sync, validate=int(validate), host_inputs=0,
+                              blocks_per_cu=blocks_per_cu)
         h = c_void_p()
         N.check(N.lib().cap_batch_create(byref(h), row_off.numel() - 1, _ptr(row_off), _ptr(row_ptr),
                                          _ptr(col_idx), _ptr(values), _vt(values.dtype), k, byref(opts),
                                          _stream(stream)), "cap_batch_create")
         self._h = h
-        a = c_int()
-        N.check(N.lib().cap_batch_info(h, byref(a)), "cap_batch_info")
+        a, bpc = c_int(), c_int()
+        N.check(N.lib().cap_batch_info(h, byref(a), byref(bpc)), "cap_batch_info")
         self.algo = N.ALGO_NAME[a.value]
+        self.blocks_per_cu = bpc.value
 
     def solve(self, b: torch.Tensor, out=None, stream=None) -> torch.Tensor:
         b = b.contiguous()

@@ -97,7 +97,7 @@ typedef struct cap_opts {
                          (0 = default 2^24); the watchdog never lets a
                          kernel spin forever on a malformed input */
     int blocks_per_cu; /* persistent grid size in workgroups per compute unit
-                          (0 = tuned / default 1) */
+                          (0 = tuned / default 1; at most 4) */
     int adjoint;       /* CAP_ADJOINT_* (0 = gather) */
     int lds_window;    /* 1 = thread-per-row kernels resolve dependencies on
                           rows of their own workgroup in LDS (value protocol) */
@@ -172,15 +172,18 @@ int cap_plan_solve_refined(cap_plan plan64, const double *b, double *x, int rhs,
  * concatenated col_idx/val; column indices are LOCAL to their matrix.  b/x
  * are concatenated the same way, k interleaved values per row.
  * opts->algo: CAP_ALGO_WRTFST (thread-per-row over all matrices, row blocks
- * handed out in level order so that every matrix advances at once),
- * CAP_ALGO_LOCAL (one workgroup per matrix, largest first) or
- * CAP_ALGO_AUTO (both timed at creation, the faster kept). */
+ * handed out by relative level -- level / depth of their matrix -- so that
+ * every matrix advances at the same pace), CAP_ALGO_LOCAL (one workgroup per
+ * matrix, largest first) or CAP_ALGO_AUTO (both timed at creation, the
+ * faster kept).  opts->blocks_per_cu = 0 lets creation time 1, 2 and 4
+ * workgroups per compute unit for the thread-per-row schedule. */
 typedef struct cap_batch_s *cap_batch;
 int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
                      const int *row_ptr, const int *col_idx, const void *val,
                      int value_type, int k, const cap_opts *opts, void *stream);
-/* The algorithm the batch resolved to (CAP_ALGO_WRTFST or CAP_ALGO_LOCAL). */
-int cap_batch_info(cap_batch batch, int *algo);
+/* The algorithm the batch resolved to (CAP_ALGO_WRTFST or CAP_ALGO_LOCAL)
+ * and its persistent grid size in workgroups per compute unit. */
+int cap_batch_info(cap_batch batch, int *algo, int *blocks_per_cu);
 int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
 int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
                             double tol, int max_iter, int *iters,

@@ -69,6 +69,48 @@ def test_batch_bit_exact_per_matrix(cuda, k, algo):
         assert np.array_equal(x[lo:hi], ref), m
 
 
+def _chain(n):
+    """Bidiagonal matrix: one row per level (depth n)."""
+    rp = np.r_[0, 1, np.arange(3, 2 * n, 2)].astype(np.int32)
+    ci = np.r_[0, np.c_[np.arange(0, n - 1), np.arange(1, n)].reshape(-1)].astype(np.int32)
+    return rp, ci, np.r_[1.0, np.tile([0.5, 2.0], n - 1)]
+
+
+@pytest.mark.parametrize("bpc", [1, 2, 4])
+@pytest.mark.parametrize("k", [1, 16])
+def test_batch_grid_sizes_with_mixed_depths(cuda, k, bpc):
+    # a chain of depth 3000 next to wide band matrices: the schedule orders
+    # blocks by level / depth, so the chain's blocks are interleaved early
+    import torch
+    from capellinisptrsv_amd import Batch
+    mats, _, _, _, _ = _batch(7, 12, k)
+    mats.insert(5, _chain(3000))
+    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
+    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
+    rp = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
+    ci = np.concatenate([m[1] for m in mats]).astype(np.int32)
+    v = np.concatenate([m[2] for m in mats])
+    N = row_off[-1]
+    b = api.uniform(13, N * k)
+    t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
+    bt = Batch(*t, k=k, algo="wrtfst", blocks_per_cu=bpc)
+    assert (bt.algo, bt.blocks_per_cu) == ("wrtfst", bpc)
+    x = bt.solve(torch.from_numpy(b.reshape(N, k) if k > 1 else b).to(cuda)).cpu().numpy().reshape(-1)
+    for m, (mrp, mci, mv) in enumerate(mats):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        assert np.array_equal(x[lo:hi], oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)), m
+    bt.close()
+
+
+def test_batch_refuses_more_than_four_blocks_per_cu(cuda):
+    import torch
+    from capellinisptrsv_amd import Batch
+    _, row_off, rp, ci, v = _batch(3, 4, 1)
+    t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
+    with pytest.raises(RuntimeError, match="cap_batch_create"):
+        Batch(*t, k=1, algo="wrtfst", blocks_per_cu=8)
+
+
 @pytest.mark.parametrize("algo", ["wrtfst", "local"])
 def test_batch_refined(cuda, algo):
     import torch

@@ -213,6 +213,19 @@ def test_error_strings():
     assert L.cap_version().startswith(b"capellini")
 
 
+def test_plan_create_refuses_more_than_four_blocks_per_cu():
+    # argument checks run before any device call
+    from ctypes import byref, c_void_p
+    rp, ci, v = random_lower(50, seed=3)
+    h = c_void_p()
+    for bpc, want in ((5, _native.CAP_ERR_ARG), (8, _native.CAP_ERR_ARG), (-1, _native.CAP_ERR_ARG)):
+        o = _native.default_opts(host_inputs=1, blocks_per_cu=bpc)
+        rc = _native.lib().cap_plan_create(byref(h), 50, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                           _native.CAP_F64, byref(o), None)
+        assert rc == want, (bpc, rc)
+        assert not h.value
+
+
 def test_sweep_lists_matrices_and_commands(tmp_path):
     """The sweep driver (run.sh equivalent): directory and list-file sources,
     lookup by name under --root, one command per (matrix, algorithm)."""
