@@ -83,6 +83,13 @@ def main():
     assert plan.status() == 0, "a dependency wait gave up"
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     x64 = x.clone()
+    # the same solve with the host waiting for each one, as a refined solve
+    # does (it reads its convergence test back before returning)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+    synced_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     # the solve kernel alone, HIP events on the stream it is launched on
     stream = torch.cuda.current_stream()
@@ -168,6 +175,8 @@ def main():
         "fp32_ms_per_step": round(fp32_ms, 5),
         "refined_ms_per_solve": round(refined_ms, 5), "refined_iterations": iters,
         "refined_over_fp64": round(refined_ms / ms_per_step, 3),
+        "fp64_ms_per_solve_synced": round(synced_ms, 5),
+        "refined_over_fp64_synced": round(refined_ms / synced_ms, 3),
     }
 
     # CPU baseline: the reference's serial executor on the same matrix (1 core)

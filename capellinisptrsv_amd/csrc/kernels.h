@@ -192,6 +192,20 @@ __global__ void __launch_bounds__(kBlock)
     for (; i < count; i += stride) p[i] = value;
 }
 
+// k_prepare plus zeroed convergence norms (two words): the fused refinement
+// sweep's whole reset in one launch.
+template <typename U>
+__global__ void __launch_bounds__(kBlock)
+    k_prepare_norms(U *__restrict__ p, long long count, U value, unsigned *__restrict__ ticket, int n_ticket,
+                    unsigned long long *__restrict__ norms)
+{
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_ticket) ticket[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 2) norms[threadIdx.x] = 0ULL;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = value;
+}
+
 // ------------------------------------------------- write-first row solver
 // Publishes x[o] (o = row*k + q) in the protocol SYNC (1 value, 2 flag).
 template <typename T, int SYNC>
@@ -532,6 +546,173 @@ __global__ void __launch_bounds__(kBlock)
             if (r < len && i < n && q < k)
                 wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win);
         });
+}
+
+// ------------------------------------------------------------ refinement
+// Maxima of non-negative doubles are taken on their bit patterns (which
+// order like the values): per thread over a grid-stride loop, then across
+// the workgroup in LDS, then one atomicMax per workgroup -- an atomic per
+// element on one address serialises in the memory side.
+__device__ __forceinline__ void block_max_to(unsigned long long v, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s_max[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const unsigned long long w = (unsigned long long)__shfl_xor((long long)v, o, kWave);
+        v = w > v ? w : v;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) s_max[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) v = s_max[w] > v ? s_max[w] : v;
+        atomicMax(out, v);
+    }
+}
+
+// ------------------------------------------------ fused refinement sweep
+// The first two steps of mixed-precision refinement in one dependency sweep.
+// Row i computes, from the pairs (x0_j, d_j) of its dependencies j:
+//   x0_i = ((float)b_i - sum v32_j x0_j) / v32_ii          FP32 solve of b
+//   r_i  = b_i - (sum v_j x0_j + v_ii x0_i)                FP64 residual
+//   d_i  = ((float)r_i - sum v32_j d_j) / v32_ii           FP32 correction
+//   x_i  = (double)x0_i + (double)d_i
+// (v32 = (float)v; every sum an fma chain in stored order), which is bit for
+// bit what the two steps give when run one after the other: FP32 solve,
+// residual, FP32 solve of the residual, accumulation.  Row i publishes
+// (x0_i, d_i) as one 64-bit word (x0 in the low half) in the value protocol;
+// a dependency's word is read once and feeds all three chains, so the sweep
+// has the dependency depth of a single solve.  The polling window is the one
+// of wrtfst_row_value (two entries, sliding by the number consumed).
+constexpr unsigned long long kPairQNaN = 0x7FC000007FC00000ULL;
+
+__device__ __forceinline__ void refine_pair_take(double v, unsigned long long p, float &s0, double &s, float &s1)
+{
+    const float xa = __uint_as_float((unsigned)p), da = __uint_as_float((unsigned)(p >> 32));
+    const float v32 = (float)v;
+    s0 = __builtin_fmaf(v32, xa, s0);
+    s = __builtin_fma(v, (double)xa, s);
+    s1 = __builtin_fmaf(v32, da, s1);
+}
+
+__device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, const double *__restrict__ val,
+                                                const double *__restrict__ b, double *__restrict__ x,
+                                                unsigned long long *__restrict__ pr, const SyncCtx &sc, int i,
+                                                int j, int d, int k, int q,
+                                                const LdsWin<unsigned long long> &win, unsigned long long &dmax,
+                                                unsigned long long &xmax)
+{
+    typedef unsigned long long U;
+    constexpr U kS = VBits<double>::kSentinel;
+    int c0 = 0, c1 = 0;
+    double v0 = 0.0, v1 = 0.0;
+    U p0 = kS, p1 = kS;
+    float s0 = 0.0f, s1 = 0.0f;
+    double s = 0.0;
+    if (j < d) {
+        c0 = ci[j];
+        v0 = val[j];
+    }
+    if (j + 1 < d) {
+        c1 = ci[j + 1];
+        v1 = val[j + 1];
+    }
+    int idle = 0;
+    bool done = false;
+    do {  // wave-uniform exit: see wrtfst_row
+        if (!done) {
+            bool prog = false;
+            if (j < d && p0 == kS) p0 = win.has(c0) ? win.load(c0, q) : ld_agent(pr + (long long)c0 * k + q);
+            if (j + 1 < d && p1 == kS) p1 = win.has(c1) ? win.load(c1, q) : ld_agent(pr + (long long)c1 * k + q);
+            if (j < d && p0 != kS) {
+                refine_pair_take(v0, p0, s0, s, s1);
+                ++j;
+                prog = true;
+                if (j < d && p1 != kS) {  // both consumed: a fresh window
+                    refine_pair_take(v1, p1, s0, s, s1);
+                    ++j;
+                    p0 = kS;
+                    if (j < d) {
+                        c0 = ci[j];
+                        v0 = val[j];
+                    }
+                } else {  // slide by one
+                    c0 = c1;
+                    v0 = v1;
+                    p0 = p1;
+                }
+                p1 = kS;
+                if (j + 1 < d) {
+                    c1 = ci[j + 1];
+                    v1 = val[j + 1];
+                }
+            }
+            if (prog) {
+                idle = 0;
+            } else if (j < d && give_up(sc, ++idle)) {
+                s0 = s1 = __builtin_nanf("");  // poison this row; it still publishes below
+                s = __builtin_nan("");
+                j = d;
+            }
+            if (j >= d) {  // write first: publish inside the polling loop
+                const long long o = (long long)i * k + q;
+                const double vd = val[d], bo = b[o];
+                const float vd32 = (float)vd;
+                const float x0 = ((float)bo - s0) / vd32;
+                s = __builtin_fma(vd, (double)x0, s);
+                const float dc = ((float)(bo - s) - s1) / vd32;
+                U u = ((U)__float_as_uint(dc) << 32) | (U)__float_as_uint(x0);
+                if (u == kS) u = kPairQNaN;
+                if (win.len) win.store(i, q, u);
+                st_agent(pr + o, u);
+                const double xd = (double)x0, dd = (double)dc;
+                x[o] = xd + dd;
+                const U ad = (U)__double_as_longlong(fabs(dd)), ax = (U)__double_as_longlong(fabs(xd));
+                dmax = ad > dmax ? ad : dmax;
+                xmax = ax > xmax ? ax : xmax;
+                asm volatile("" ::: "memory");
+                done = true;
+            }
+        }
+    } while (__any(!done));
+}
+
+// Thread-per-row launch of the fused sweep (lane layout, tickets, batch
+// schedule and LDS window as in k_wrtfst); pr holds the pairs and starts
+// filled with the sentinel.  norms[0] = max|d|, norms[1] = max|x0| (bits of
+// non-negative doubles, atomicMax per workgroup).
+template <int LDS>
+__global__ void __launch_bounds__(kBlock)
+    k_refine2(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
+              const double *__restrict__ b, double *__restrict__ x, unsigned long long *__restrict__ pr, SyncCtx sc,
+              int n, int k, int lpr_log2, unsigned *__restrict__ ticket, const int2 *__restrict__ sched, int n_sched,
+              unsigned long long *__restrict__ norms)
+{
+    typedef unsigned long long U;
+    __shared__ U s_x[LDS ? kBlock : 1];
+    const int q = threadIdx.x & ((1 << lpr_log2) - 1);
+    const int r = threadIdx.x >> lpr_log2;
+    const int rows = kBlock >> lpr_log2;
+    U dmax = 0, xmax = 0;
+    ticket_loop(
+        ticket, sched ? 1 : rows, sched ? n_sched : n,
+        [&](int) {
+            if (LDS) s_x[threadIdx.x] = VBits<double>::kSentinel;
+        },
+        [&](int base) {
+            int lo = base, len = rows;
+            if (sched) {
+                const int2 blk = sched[base];
+                lo = blk.x;
+                len = blk.y;
+            }
+            const int i = lo + r;
+            LdsWin<U> win{s_x, lo, LDS ? len : 0, k};
+            if (r < len && i < n && q < k)
+                refine_pair_row(ci, val, b, x, pr, sc, i, rp[i], rp[i + 1] - 1, k, q, win, dmax, xmax);
+        });
+    block_max_to(dmax, norms);
+    __syncthreads();  // block_max_to's LDS slots are reused by the second reduction
+    block_max_to(xmax, norms + 1);
 }
 
 // our2Part (CUDA/our2Part/spts_syncfree_cuda_csr.h:10-124): dependencies on
@@ -1070,27 +1251,6 @@ __global__ void __launch_bounds__(kBlock)
     if (i >= n) return;
     const int d = rp[i + 1] - 1;
     for (int j = rp[i]; j < d; ++j) atomic_add_agent(cnt + ci[j], 1);
-}
-
-// ------------------------------------------------------------ refinement
-// Maxima of non-negative doubles are taken on their bit patterns (which
-// order like the values): per thread over a grid-stride loop, then across
-// the workgroup in LDS, then one atomicMax per workgroup -- an atomic per
-// element on one address serialises in the memory side.
-__device__ __forceinline__ void block_max_to(unsigned long long v, unsigned long long *__restrict__ out)
-{
-    __shared__ unsigned long long s_max[kBlock / kWave];
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const unsigned long long w = (unsigned long long)__shfl_xor((long long)v, o, kWave);
-        v = w > v ? w : v;
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0) s_max[threadIdx.x / kWave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) v = s_max[w] > v ? s_max[w] : v;
-        atomicMax(out, v);
-    }
 }
 
 // r = b - L x in FP64 (rows in CSR order, fused multiply-add chain), written

@@ -102,6 +102,8 @@ struct cap_plan_s {
     // refinement
     float *val32 = nullptr, *r32 = nullptr, *d32 = nullptr;
     long long refine_len = 0;
+    unsigned long long *pair = nullptr;  // fused refinement pairs when readiness is in place
+    long long pair_len = 0;
     unsigned long long *norms = nullptr;  // inside ctrl
     hipGraphExec_t refine_graph = nullptr;
     const void *graph_b = nullptr;
@@ -232,7 +234,7 @@ void plan_free(cap_plan p)
         (void)hipFree(p->val);
     }
     void *bufs[] = {p->rev_b, p->rev_x, p->sched, p->seg, p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
-                    p->cnt0, p->cnt, p->adj_ls, p->rpT, p->ciT, p->vT, p->val32, p->r32, p->d32};
+                    p->cnt0, p->cnt, p->adj_ls, p->rpT, p->ciT, p->vT, p->val32, p->r32, p->d32, p->pair};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->refine_graph) (void)hipGraphExecDestroy(p->refine_graph);
@@ -1087,6 +1089,10 @@ inline unsigned stream_blocks(const cap_plan p, long long count)
     return std::max(1u, std::min(blocks_for(count), (unsigned)p->ncu * 8u));
 }
 
+// Plans whose solve is thread-per-row run the first two refinement steps as
+// one fused sweep; one-workgroup (LOCAL) plans run them as separate solves.
+inline bool refine_fuses(const cap_plan p) { return p->algo != CAP_ALGO_LOCAL; }
+
 int ensure_refine(cap_plan p, int rhs, hipStream_t st)
 {
     const long long len = (long long)p->n * rhs;
@@ -1103,6 +1109,45 @@ int ensure_refine(cap_plan p, int rhs, hipStream_t st)
         CAP_TRY(dev_alloc(&p->d32, (size_t)len));
         p->refine_len = len;
     }
+    if (refine_fuses(p) && !p->staged && p->pair_len < len) {
+        if (p->pair) CAP_HIP_TRY(hipFree(p->pair));
+        p->pair = nullptr;
+        CAP_TRY(dev_alloc(&p->pair, (size_t)len));
+        p->pair_len = len;
+    }
+    return CAP_OK;
+}
+
+// The first two refinement steps as one sweep (k_refine2): the norms and
+// the pair array (staged: the readiness array; else the plan's own) reset
+// with the tickets, then the thread-per-row launch with the plan's grid,
+// LDS window and batch schedule.
+int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st)
+{
+    typedef unsigned long long U;
+    const long long len = (long long)p->n * rhs;
+    U *pr = p->staged ? reinterpret_cast<U *>(p->ready) : p->pair;
+    if (!pr || (p->staged && p->ready_bytes < sizeof(U) * (size_t)len) || (!p->staged && p->pair_len < len))
+        return CAP_ERR_ARG;
+    const unsigned nbp = std::max(1u, std::min(blocks_for(len), 2048u));
+    k_prepare_norms<U><<<nbp, kBlock, 0, st>>>(pr, len, VBits<double>::kSentinel, p->ticket, kTickets, p->norms);
+    CAP_HIP_TRY(hipGetLastError());
+    if (p->n == 0) return CAP_OK;
+    const int l = lpr_log2_for(rhs);
+    const int rows_per_block = kBlock >> l;
+    const bool use_sched = p->sched && p->sched_rows == rows_per_block;
+    const int2 *sched = use_sched ? p->sched : nullptr;
+    const int n_sched = use_sched ? p->n_sched : 0;
+    const unsigned nb = grid_for(p, use_sched ? n_sched : ((long long)p->n + rows_per_block - 1) / rows_per_block);
+    const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    const double *val = (const double *)p->val;
+    if (p->lds)
+        k_refine2<1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
+                                             p->norms);
+    else
+        k_refine2<0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
+                                             p->norms);
+    CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }
 
@@ -1134,8 +1179,10 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
     if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
     CAP_TRY(ensure_refine(p, rhs, st));
     // the staged readiness array must exist before capture (no allocation
-    // inside the captured steps, whose kernels bake in its address)
-    if (p->staged) CAP_TRY(ensure_ready(p, sizeof(float) * (size_t)p->n * rhs));
+    // inside the captured steps, whose kernels bake in its address); the
+    // fused sweep publishes 8-byte pairs, the correction solves 4-byte values
+    const bool fused = refine_fuses(p);
+    if (p->staged) CAP_TRY(ensure_ready(p, (fused ? 8 : 4) * (size_t)p->n * rhs));
     // The first two steps (solve from b, one correction) are one graph:
     // captured once per (b, x, rhs), replayed on later calls.
     if (!p->refine_graph || p->graph_b != b || p->graph_x != x || p->graph_rhs != rhs) {
@@ -1145,8 +1192,13 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
         CAP_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t graph;
         CAP_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        int rc = refine_step(p, b, x, rhs, true, cs);
-        if (rc == CAP_OK) rc = refine_step(p, b, x, rhs, false, cs);
+        int rc = CAP_OK;
+        if (fused) {
+            rc = refine_fused(p, b, x, rhs, cs);
+        } else {
+            rc = refine_step(p, b, x, rhs, true, cs);
+            if (rc == CAP_OK) rc = refine_step(p, b, x, rhs, false, cs);
+        }
         hipError_t ec = hipStreamEndCapture(cs, &graph);
         if (rc != CAP_OK || ec != hipSuccess) {
             (void)hipStreamDestroy(cs);

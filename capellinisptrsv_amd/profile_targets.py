@@ -11,6 +11,9 @@
     python -m capellinisptrsv_amd.profile_targets batched [--steps N]
         the 256-matrix batch with 16 right-hand sides: N FP64 solves and N
         FP32 + FP64 refined solves
+    python -m capellinisptrsv_amd.profile_targets refine [--steps N]
+        the wide-level matrix: N FP64 solves, then N FP32 + FP64 refined
+        solves (fused first two steps) to 1e-12
     python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto]
         the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels)
 """
@@ -20,7 +23,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched", "deep"])
+    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched", "deep", "refine"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--algo", default="auto")
     ap.add_argument("--mib", type=int, default=1024)
@@ -55,17 +58,22 @@ def main(argv=None):
         print(f"batched: n={n} nnz={len(ci)} k={k} steps={a.steps} refined iterations={it}", flush=True)
         bt.close()
         return 0
-    cfg = {"solve": 1, "adjoint": 4, "deep": 2}[a.target]
+    cfg = {"solve": 1, "refine": 1, "adjoint": 4, "deep": 2}[a.target]
     rp, ci, v = api.make_config(cfg, seed=0)
     n = len(rp) - 1
     d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
     b = torch.from_numpy(api.uniform(1, n)).to(dev)
     x = torch.empty_like(b)
-    plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep") else Plan(*d, algo="wrtfst", sync="value")
+    plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep", "refine") else Plan(*d, algo="wrtfst", sync="value")
     for _ in range(a.steps):
         plan.solve(b, out=x)
         if a.target == "adjoint":
             plan.adjoint(x, x)
+    if a.target == "refine":
+        xr = torch.empty_like(b)
+        for _ in range(a.steps):
+            _, it, rel = plan.solve_refined(b, tol=1e-12, out=xr)
+        print(f"refined: iterations={it} last correction={rel:.3e}", flush=True)
     torch.cuda.synchronize()
     assert plan.status() == 0
     print(f"{a.target}: n={n} nnz={len(ci)} plan={plan.info()} steps={a.steps}", flush=True)

@@ -391,6 +391,28 @@ void orc_residual_d(int n, const int *rp, const int *ci, const double *v, int rh
         }
 }
 
+void orc_refine2_d(int n, const int *rp, const int *ci, const double *v, int rhs,
+                   const double *b, float *x0, float *d, double *x)
+{
+    size_t nnz = (size_t)rp[n], len = (size_t)n * rhs, t;
+    float *v32 = (float *)calloc(nnz + 1, sizeof(float));
+    float *b32 = (float *)calloc(len + 1, sizeof(float));
+    int i, j, k;
+    for (t = 0; t < nnz; ++t) v32[t] = (float)v[t];
+    for (t = 0; t < len; ++t) b32[t] = (float)b[t];
+    orc_csr_forward_s(n, rp, ci, v32, rhs, b32, x0, 1);
+    for (i = 0; i < n; ++i)
+        for (k = 0; k < rhs; ++k) {
+            double s = 0.0;
+            for (j = rp[i]; j < rp[i + 1]; ++j) s = fma(v[j], (double)x0[(size_t)ci[j] * rhs + k], s);
+            b32[(size_t)i * rhs + k] = (float)(b[(size_t)i * rhs + k] - s);
+        }
+    orc_csr_forward_s(n, rp, ci, v32, rhs, b32, d, 1);
+    for (t = 0; t < len; ++t) x[t] = (double)x0[t] + (double)d[t];
+    free(v32);
+    free(b32);
+}
+
 /* spts_syncfree_cuda_csr_wrt.h:208 (+ 4*Len, CUDA/mix/...:317) */
 double orc_datasize(int m, int n, int nnzL, int vb, int len)
 {

@@ -121,6 +121,15 @@ void orc_grad_val_d(int n, const int *rp, const int *ci, const double *y,
 void orc_residual_d(int n, const int *rp, const int *ci, const double *v,
                     int rhs, const double *b, const double *x, double *r);
 
+/* ---- two steps of mixed-precision refinement (no counterpart in the
+ *      reference; the method of the library's refined solves):
+ *      x0 = L32^{-1} b32 in FP32 (fused, stored order), r = b - L x0 in FP64
+ *      (one fma chain over the row in stored order, diagonal included) rounded
+ *      to FP32, d = L32^{-1} r32 in FP32, x = (double)x0 + (double)d.  L32 and
+ *      b32 are the FP32 roundings of L and b.  Multi-RHS interleaved. */
+void orc_refine2_d(int n, const int *rp, const int *ci, const double *v, int rhs,
+                   const double *b, float *x0, float *d, double *x);
+
 /* ---- metric formulas: CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:206-215
  *      (dataSize; mix adds 4*Len, CUDA/mix/spts_syncfree_cuda_csr_wrt.h:317). */
 double orc_datasize(int m, int n, int nnzL, int value_bytes, int len_warp_num);

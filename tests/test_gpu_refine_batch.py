@@ -26,13 +26,42 @@ def test_refined_reaches_1e12(cuda, cfg, scale, k, algo):
     plan = Plan(*d, algo=algo, sync="value")
     b = api.uniform(9, n * k)
     bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
-    for rep in range(2):  # second call replays the captured graph
-        x, iters, rel = plan.solve_refined(bt, tol=1e-12, max_iter=10)
+    out = torch.empty_like(bt)
+    for rep in range(2):  # same b and x: the second call replays the captured graph
+        x, iters, rel = plan.solve_refined(bt, tol=1e-12, max_iter=10, out=out)
         torch.cuda.synchronize()
         ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
         err = rel_maxnorm(x.cpu().numpy().reshape(-1), ref)
         assert err <= 1e-12, (rep, iters, rel, err)
         assert 2 <= iters <= 4
+
+
+@pytest.mark.parametrize("algo,readiness,lds,bpc,k", [
+    ("wrtfst", "staged", True, 1, 1), ("wrtfst", "staged", False, 2, 1), ("wrtfst", "in_place", True, 1, 1),
+    ("wrtfst", "in_place", False, 4, 1), ("wrtfst", "staged", True, 1, 8), ("wrtfst", "in_place", False, 2, 16),
+    ("local", "auto", False, 1, 1), ("local", "auto", False, 1, 16)])
+def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k):
+    """Thread-per-row plans run the first two refinement steps as one fused
+    sweep, one-workgroup plans as two solves with the residual between them;
+    both must give the oracle's two-step result bit for bit."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    for name, (rp, ci, v) in (("cfg1_small", api.make_config(1, seed=2, scale=0.05)),
+                              ("band", api.generate("band_rowcount", n=6000, offdiag=12, band=300, seed=5))):
+        n = len(rp) - 1
+        d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+        plan = Plan(*d, algo=algo, sync="value", lds_window=lds, blocks_per_cu=bpc, readiness=readiness)
+        b = api.uniform(21, n * k)
+        bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
+        _, _, want = oracle.refine2(rp, ci, v, b, rhs=k)
+        out = torch.empty_like(bt)
+        for rep in range(2):  # same b and x: the second call replays the captured graph
+            x, iters, rel = plan.solve_refined(bt, tol=1e-12, max_iter=10, out=out)
+            torch.cuda.synchronize()
+            assert plan.status() == 0
+            assert iters == 2, (name, rep, iters, rel)
+            assert np.array_equal(x.cpu().numpy().reshape(-1), want), (name, rep)
+        plan.close()
 
 
 def _batch(seed, n_mat, k, nmin=200, nmax=5000):
@@ -127,6 +156,8 @@ def test_batch_refined(cuda, algo):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
         ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
         assert rel_maxnorm(x[lo:hi], ref) <= 1e-12, (m, iters, rel)
+        if iters == 2:  # stopped after the first correction: the oracle's two steps exactly
+            assert np.array_equal(x[lo:hi], oracle.refine2(mrp, mci, mv, b[lo:hi], rhs=k)[2]), m
 
 
 def test_batch_rejects_bad_matrix(cuda):

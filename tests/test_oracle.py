@@ -18,6 +18,10 @@ same checks -- see test_planted_mistakes_are_caught.
   csr_forward          [C] bit-equal to the executor (unfused), [D]
   adjoint / grad_val   [R][G] (backward executor on L^T), [D]
   residual, datasize   [C]
+  refine2              [C] x0 bit-equal to the FP32 forward solve, exact
+                       inputs give d = 0, per-RHS columns; [D] x within 1e-12
+                       of the dense FP64 solve, d within 1e-5 of the dense
+                       solve of the FP64 residual
 """
 import json
 import os
@@ -250,6 +254,34 @@ def test_adjoint_against_dense():
     gv = oracle.grad_val(rp, ci, y, x)
     rows = np.repeat(np.arange(n), np.diff(rp))
     assert np.array_equal(gv, -y[rows] * x[ci])
+
+
+def test_refine2_pins():
+    rp, ci, v = random_lower(1500, max_off=8, band=200, seed=11)
+    n = 1500
+    b = np.random.default_rng(3).uniform(-1, 1, size=n)
+    x0, d, x = oracle.refine2(rp, ci, v, b)
+    # x0 is the FP32 solve on the rounded inputs
+    assert np.array_equal(x0, oracle.csr_forward(rp, ci, v.astype(np.float32), b.astype(np.float32), fused=True))
+    L = dense_lower(rp, ci, v)
+    xd = scipy.linalg.solve_triangular(L, b, lower=True)
+    assert rel_maxnorm(x0.astype(np.float64), xd) > 1e-10  # FP32 alone is not enough
+    assert rel_maxnorm(x, xd) <= 1e-12
+    assert np.array_equal(x, x0.astype(np.float64) + d.astype(np.float64))
+    r = b - L @ x0.astype(np.float64)
+    dd = scipy.linalg.solve_triangular(L, r, lower=True)
+    assert rel_maxnorm(d.astype(np.float64), dd) <= 1e-5
+    # exactly representable system: x0 = 1, residual 0, no correction
+    w = np.where(np.arange(len(ci)) == np.repeat(rp[1:] - 1, np.diff(rp)), 1.0, 0.5)
+    bw = dense_lower(rp, ci, w) @ np.ones(n)
+    x0, d, x = oracle.refine2(rp, ci, w, bw)
+    assert np.all(x0 == 1.0) and np.all(d == 0.0) and np.all(x == 1.0)
+    # interleaved right-hand sides: column k is the single-RHS result
+    B = np.random.default_rng(4).uniform(-1, 1, size=(n, 3))
+    X0, D, X = (a.reshape(n, 3) for a in oracle.refine2(rp, ci, v, B.reshape(-1), rhs=3))
+    for k in range(3):
+        a0, ad, ax = oracle.refine2(rp, ci, v, B[:, k].copy())
+        assert np.array_equal(X0[:, k], a0) and np.array_equal(D[:, k], ad) and np.array_equal(X[:, k], ax)
 
 
 def test_residual_and_datasize():
