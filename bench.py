@@ -152,7 +152,8 @@ def main():
             t = json.load(f)
         # only for the kernel configuration those counters were taken on
         if all(t.get("plan", {}).get(key) == info.get(key) for key in ("algo", "sync", "lds_window",
-                                                                       "blocks_per_cu", "readiness")) \
+                                                                       "blocks_per_cu", "readiness",
+                                                                       "row_threads")) \
                 and t.get("n") == n:
             traffic = t.get("hbm_bytes_per_launch")
 

@@ -51,7 +51,7 @@ class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
                 ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
-                ("upper", c_int), ("unit_diag", c_int), ("reserved", c_int * 1)]
+                ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int)]
 
 
 class cap_gen_params(Structure):
@@ -76,7 +76,7 @@ SIGNATURES = {
     "cap_plan_reset": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
-    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI]),
+    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI, PI]),
     "cap_batch_info": (c_int, [c_void_p, PI, PI]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

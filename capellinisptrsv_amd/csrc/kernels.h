@@ -33,7 +33,8 @@
 namespace cap {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;  // 4 wavefronts per workgroup
+constexpr int kBlock = 256;       // 4 wavefronts per workgroup
+constexpr int kBlockSmall = 128;  // the thread-per-row kernels' other workgroup size
 constexpr int kWin = 4;      // dependencies polled concurrently per lane (flag protocol)
 
 
@@ -510,25 +511,26 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 }
 
 // ourWrtFst (CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:10-57): one lane
-// per row and right-hand side; a workgroup ticket covers kBlock >> lpr_log2
-// rows with 1 << lpr_log2 lanes per row, lane q of a row owning right-hand
-// side q (lanes with q >= k idle).
+// per row and right-hand side; a workgroup of BLOCK threads (128 or 256)
+// takes tickets of BLOCK >> lpr_log2 rows with 1 << lpr_log2 lanes per row,
+// lane q of a row owning right-hand side q (lanes with q >= k idle).  The
+// workgroup size sets how many rows poll at once on each compute unit.
 // sched (optional, n_sched blocks): ticket t solves rows [sched[t].x,
 // sched[t].x + sched[t].y) instead of the t-th run of consecutive rows.  The
 // host orders the blocks so that every row comes after the rows it depends
 // on (each matrix of a batch keeps its row order), which keeps the
 // forward-progress argument of the ticket loop.
-template <typename T, int SYNC, int LDS>
-__global__ void __launch_bounds__(kBlock)
+template <typename T, int SYNC, int LDS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
              unsigned *__restrict__ ticket, int n_rev, const int2 *__restrict__ sched, int n_sched)
 {
     typedef typename VBits<T>::U U;
-    __shared__ U s_x[LDS ? kBlock : 1];
+    __shared__ U s_x[LDS ? BLOCK : 1];
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
-    const int rows = kBlock >> lpr_log2;
+    const int rows = BLOCK >> lpr_log2;
     ticket_loop(
         ticket, sched ? 1 : rows, sched ? n_sched : n,
         [&](int) {
@@ -676,22 +678,22 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
     } while (__any(!done));
 }
 
-// Thread-per-row launch of the fused sweep (lane layout, tickets, batch
-// schedule and LDS window as in k_wrtfst); pr holds the pairs and starts
-// filled with the sentinel.  norms[0] = max|d|, norms[1] = max|x0| (bits of
-// non-negative doubles, atomicMax per workgroup).
-template <int LDS>
-__global__ void __launch_bounds__(kBlock)
+// Thread-per-row launch of the fused sweep (workgroup size, lane layout,
+// tickets, batch schedule and LDS window as in k_wrtfst); pr holds the pairs
+// and starts filled with the sentinel.  norms[0] = max|d|, norms[1] =
+// max|x0| (bits of non-negative doubles, atomicMax per workgroup).
+template <int LDS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
     k_refine2(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
               const double *__restrict__ b, double *__restrict__ x, unsigned long long *__restrict__ pr, SyncCtx sc,
               int n, int k, int lpr_log2, unsigned *__restrict__ ticket, const int2 *__restrict__ sched, int n_sched,
               unsigned long long *__restrict__ norms)
 {
     typedef unsigned long long U;
-    __shared__ U s_x[LDS ? kBlock : 1];
+    __shared__ U s_x[LDS ? BLOCK : 1];
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
-    const int rows = kBlock >> lpr_log2;
+    const int rows = BLOCK >> lpr_log2;
     U dmax = 0, xmax = 0;
     ticket_loop(
         ticket, sched ? 1 : rows, sched ? n_sched : n,

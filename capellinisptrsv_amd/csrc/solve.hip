@@ -67,6 +67,7 @@ struct cap_plan_s {
     int *err = nullptr;
     int spin_limit = kSpinLimitDefault;
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
+    int wblock = kBlock;     // threads per workgroup of k_wrtfst / k_refine2 (128 or 256)
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
@@ -391,22 +392,23 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     if (algo == CAP_ALGO_WRTFST) {
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
-        const int rows_per_block = kBlock >> l;
+        const int rows_per_block = p->wblock >> l;
         // a batch's block schedule, when built for this block size
         const bool use_sched = p->sched && p->sched_rows == rows_per_block;
         const int2 *sched = use_sched ? p->sched : nullptr;
         const int n_sched = use_sched ? p->n_sched : 0;
         const unsigned nb =
             grid_for(p, use_sched ? n_sched : ((long long)n + rows_per_block - 1) / rows_per_block);
+        auto go = [&](auto kernel) {
+            kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched, n_sched);
+        };
+        const bool small = p->wblock == kBlockSmall;
         if (sync == CAP_SYNC_VALUE && p->lds)
-            k_wrtfst<T, 1, 1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched,
-                                                     n_sched);
+            small ? go(k_wrtfst<T, 1, 1, kBlockSmall>) : go(k_wrtfst<T, 1, 1, kBlock>);
         else if (sync == CAP_SYNC_VALUE)
-            k_wrtfst<T, 1, 0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched,
-                                                     n_sched);
+            small ? go(k_wrtfst<T, 1, 0, kBlockSmall>) : go(k_wrtfst<T, 1, 0, kBlock>);
         else
-            k_wrtfst<T, 2, 0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched,
-                                                     n_sched);
+            small ? go(k_wrtfst<T, 2, 0, kBlockSmall>) : go(k_wrtfst<T, 2, 0, kBlock>);
     } else if (algo == CAP_ALGO_TWOPART) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         const unsigned nb = grid_for(p, ((long long)n + kBlock - 1) / kBlock);
@@ -544,16 +546,16 @@ int ensure_csc(cap_plan p, hipStream_t st)
 
 // ----------------------------------------------------------- autotuner
 struct TuneKey {
-    int n, nnz, vt, rhs;
+    int n, nnz, vt, staged;
     unsigned long long fp;
     bool operator<(const TuneKey &o) const
     {
-        return std::tie(n, nnz, vt, rhs, fp) < std::tie(o.n, o.nnz, o.vt, o.rhs, o.fp);
+        return std::tie(n, nnz, vt, staged, fp) < std::tie(o.n, o.nnz, o.vt, o.staged, o.fp);
     }
 };
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int algo, sync, border, bpc, lds, staged;
+    int algo, sync, border, bpc, lds, staged, block;
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 
@@ -607,6 +609,7 @@ void apply_choice(cap_plan p, const TuneChoice &c)
     p->bpc = c.bpc;
     p->lds = c.lds;
     p->staged = c.staged;
+    p->wblock = c.block;
 }
 
 // Candidates are timed on the plan's own matrix (b = 0; the dependency
@@ -618,7 +621,7 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // it in every candidate.
 int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
-    TuneKey key{p->n, p->nnz, p->vt, 1, fingerprint(rp_host)};
+    TuneKey key{p->n, p->nnz, p->vt, p->staged, fingerprint(rp_host)};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -634,12 +637,15 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     CAP_HIP_TRY(hipMalloc(&x, (size_t)p->n * p->vbytes + 8));
     CAP_HIP_TRY(hipMemsetAsync(b, 0, (size_t)p->n * p->vbytes, st));
     const int stg = p->staged;
-    std::vector<TuneChoice> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, stg},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 0, stg},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1, stg},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 1, stg},
-                                     {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0, 0}};
-    if (stg) cands.push_back({CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, 0});
+    constexpr int B = kBlock, S = kBlockSmall;
+    std::vector<TuneChoice> cands = {{CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, stg, B},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 0, stg, B},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1, stg, B},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 2, 1, stg, B},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, stg, S},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1, stg, S},
+                                     {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0, 0, B}};
+    if (stg) cands.push_back({CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
     const double avg = (double)p->nnz / std::max(1, p->n);
     // one workgroup per matrix pays off only with few rows per level
     {
@@ -649,12 +655,12 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         // timed first: on such matrices it sets a low bound that lets the
         // slower candidates stop after their warm-up
         if ((double)p->n / std::max(1, levels) <= kLocalMaxWidth)
-            cands.insert(cands.begin(), {CAP_ALGO_LOCAL, CAP_SYNC_VALUE, 0, 1, 0, 0});
+            cands.insert(cands.begin(), {CAP_ALGO_LOCAL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
     }
     for (int bd : {8, 16, 32})
         if (avg * 4 >= bd) {
-            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg});
-            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg, B});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg, B});
         }
     float best = 1e30f;
     TuneChoice pick = cands[0];
@@ -749,6 +755,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         return CAP_ERR_ARG;
     if (!o.unit_diag && nnz < n) return CAP_ERR_ARG;
     if (o.blocks_per_cu < 0 || o.blocks_per_cu > kMaxBlocksPerCu) return CAP_ERR_ARG;
+    if (o.row_threads != 0 && o.row_threads != kBlock && o.row_threads != kBlockSmall) return CAP_ERR_ARG;
     *out = nullptr;
     if (o.upper || o.unit_diag) {
         // convert on the host, then create a plan over the owned lower matrix
@@ -833,6 +840,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if ((rc = init_ctrl(p)) != CAP_OK) break;
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
+        if (o.row_threads > 0) p->wblock = o.row_threads;
         p->lds = o.lds_window > 0;
         p->staged = o.readiness != CAP_READY_IN_PLACE;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
@@ -1020,10 +1028,12 @@ int cap_plan_status(cap_plan p, void *stream)
     return e ? CAP_ERR_WATCHDOG : CAP_OK;
 }
 
-int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint, int *readiness)
+int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint, int *readiness,
+                    int *row_threads)
 {
     if (!p) return CAP_ERR_ARG;
     if (blocks_per_cu) *blocks_per_cu = p->bpc;
+    if (row_threads) *row_threads = p->wblock;
     if (lds_window) *lds_window = p->lds;
     if (adjoint) *adjoint = p->adjoint_mode;
     if (readiness) *readiness = p->staged ? CAP_READY_STAGED : CAP_READY_IN_PLACE;
@@ -1050,7 +1060,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
         CAP_TRY(launch_prepare<double>(p, CAP_ALGO_WRTFST, CAP_SYNC_VALUE, grad_b, n, st));
         SyncCtx sc;
         CAP_TRY(sync_ctx<double>(p, CAP_SYNC_VALUE, n, &sc));
-        k_wrtfst<double, 1, 0><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        k_wrtfst<double, 1, 0, kBlock><<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
             p->rpT, p->ciT, p->vT, (const double *)g, (double *)grad_b, sc, n, 1, 0, p->ticket, n, nullptr, 0);
         if (grad_val)
             k_grad_val<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)grad_b, (const double *)x,
@@ -1134,19 +1144,22 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     CAP_HIP_TRY(hipGetLastError());
     if (p->n == 0) return CAP_OK;
     const int l = lpr_log2_for(rhs);
-    const int rows_per_block = kBlock >> l;
+    const int rows_per_block = p->wblock >> l;
     const bool use_sched = p->sched && p->sched_rows == rows_per_block;
     const int2 *sched = use_sched ? p->sched : nullptr;
     const int n_sched = use_sched ? p->n_sched : 0;
     const unsigned nb = grid_for(p, use_sched ? n_sched : ((long long)p->n + rows_per_block - 1) / rows_per_block);
     const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     const double *val = (const double *)p->val;
+    auto go = [&](auto kernel) {
+        kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
+                                         p->norms);
+    };
+    const bool small = p->wblock == kBlockSmall;
     if (p->lds)
-        k_refine2<1><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
-                                             p->norms);
+        small ? go(k_refine2<1, kBlockSmall>) : go(k_refine2<1, kBlock>);
     else
-        k_refine2<0><<<nb, kBlock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
-                                             p->norms);
+        small ? go(k_refine2<0, kBlockSmall>) : go(k_refine2<0, kBlock>);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }
@@ -1417,7 +1430,7 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
         return rc;
     }
     p->owns_matrix = true;
-    if ((rc = build_batch_schedule(p, ro, kBlock >> lpr_log2_for(k), st)) != CAP_OK ||
+    if ((rc = build_batch_schedule(p, ro, p->wblock >> lpr_log2_for(k), st)) != CAP_OK ||
         (rc = ensure_segments(p, &ro)) != CAP_OK || (rc = choose_batch_algo(p, o.algo, o.blocks_per_cu > 0, k, st)) != CAP_OK) {
         cap_plan_destroy(p);
         return rc;

@@ -38,6 +38,8 @@ class Plan:
     row_ptr/col_idx: int32 CUDA tensors; values: float64/float32 CUDA tensor.
     The plan keeps references to the three tensors (their memory is used in
     place).  algo: auto|wrtfst|twopart|mix|csc|local; sync: auto|value|flag.
+    row_threads: 128 or 256 threads per workgroup of the thread-per-row
+    kernels (0: tuned with algo auto, else 256).
     upper=True: upper triangular, diagonal first in each row; unit_diagonal=
     True: no diagonal stored (taken as 1).  Both convert the matrix into a
     plan-owned copy; such plans solve, but have no adjoint or refinement.
@@ -46,7 +48,7 @@ class Plan:
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
-                 upper: bool = False, unit_diagonal: bool = False, stream=None):
+                 upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -59,7 +61,7 @@ class Plan:
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
-                              unit_diag=int(unit_diagonal))
+                              unit_diag=int(unit_diagonal), row_threads=row_threads)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -71,13 +73,13 @@ class Plan:
         wo, eo = c_double(), c_double()
         N.check(N.lib().cap_plan_info(self._h, byref(a), byref(s), byref(b), byref(ln), byref(wo), byref(eo)),
                 "cap_plan_info")
-        bpc, lds, adj, rdy = c_int(), c_int(), c_int(), c_int()
-        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj), byref(rdy)),
+        bpc, lds, adj, rdy, thr = c_int(), c_int(), c_int(), c_int(), c_int()
+        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj), byref(rdy), byref(thr)),
                 "cap_plan_tuning")
         return dict(algo=N.ALGO_NAME[a.value], sync=N.SYNC_NAME[s.value], border=b.value, mix_len=ln.value,
                     warp_occupy=wo.value, element_occupy=eo.value, blocks_per_cu=bpc.value,
                     lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"),
-                    readiness=N.READINESS_NAME.get(rdy.value, "?"))
+                    readiness=N.READINESS_NAME.get(rdy.value, "?"), row_threads=thr.value)
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

@@ -106,7 +106,10 @@ typedef struct cap_opts {
     int upper;         /* 1 = the matrix is upper triangular, diagonal first
                           in each row (backward substitution) */
     int unit_diag;     /* 1 = implicit unit diagonal: no diagonal is stored */
-    int reserved[1];
+    int row_threads;   /* threads per workgroup of the thread-per-row solve
+                          and the fused refinement sweep: 128 or 256 rows
+                          polling per compute unit at one RHS
+                          (0 = tuned / default 256) */
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -140,7 +143,7 @@ int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
 /* Grid, window and readiness choices the plan uses (after autotuning);
  * *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE. */
 int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint,
-                    int *readiness);
+                    int *readiness, int *row_threads);
 int cap_plan_destroy(cap_plan plan);
 
 /* Adjoint of x = L^{-1} b: grad_b = L^{-T} g and grad_val[k] =

@@ -36,11 +36,12 @@ def test_refined_reaches_1e12(cuda, cfg, scale, k, algo):
         assert 2 <= iters <= 4
 
 
-@pytest.mark.parametrize("algo,readiness,lds,bpc,k", [
-    ("wrtfst", "staged", True, 1, 1), ("wrtfst", "staged", False, 2, 1), ("wrtfst", "in_place", True, 1, 1),
-    ("wrtfst", "in_place", False, 4, 1), ("wrtfst", "staged", True, 1, 8), ("wrtfst", "in_place", False, 2, 16),
-    ("local", "auto", False, 1, 1), ("local", "auto", False, 1, 16)])
-def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k):
+@pytest.mark.parametrize("algo,readiness,lds,bpc,k,threads", [
+    ("wrtfst", "staged", True, 1, 1, 0), ("wrtfst", "staged", False, 2, 1, 0), ("wrtfst", "in_place", True, 1, 1, 0),
+    ("wrtfst", "in_place", False, 4, 1, 0), ("wrtfst", "staged", True, 1, 8, 0),
+    ("wrtfst", "in_place", False, 2, 16, 0), ("wrtfst", "staged", True, 1, 1, 128),
+    ("wrtfst", "in_place", False, 1, 16, 128), ("local", "auto", False, 1, 1, 0), ("local", "auto", False, 1, 16, 0)])
+def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k, threads):
     """Thread-per-row plans run the first two refinement steps as one fused
     sweep, one-workgroup plans as two solves with the residual between them;
     both must give the oracle's two-step result bit for bit."""
@@ -50,7 +51,8 @@ def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k):
                               ("band", api.generate("band_rowcount", n=6000, offdiag=12, band=300, seed=5))):
         n = len(rp) - 1
         d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
-        plan = Plan(*d, algo=algo, sync="value", lds_window=lds, blocks_per_cu=bpc, readiness=readiness)
+        plan = Plan(*d, algo=algo, sync="value", lds_window=lds, blocks_per_cu=bpc, readiness=readiness,
+                    row_threads=threads)
         b = api.uniform(21, n * k)
         bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
         _, _, want = oracle.refine2(rp, ci, v, b, rhs=k)

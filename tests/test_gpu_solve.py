@@ -19,11 +19,12 @@ FP64_TOL = 1e-12
 FP32_TOL = 1e-5
 
 
-def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0, lds=False, bpc=0, readiness="auto"):
+def _solve(cuda, rp, ci, v, b, algo, sync, k=1, border=0, lds=False, bpc=0, readiness="auto", threads=0):
     import torch
     from capellinisptrsv_amd import Plan
     plan = Plan(torch.from_numpy(rp).to(cuda), torch.from_numpy(ci).to(cuda), torch.from_numpy(v).to(cuda),
-                algo=algo, sync=sync, border=border, lds_window=lds, blocks_per_cu=bpc, readiness=readiness)
+                algo=algo, sync=sync, border=border, lds_window=lds, blocks_per_cu=bpc, readiness=readiness,
+                row_threads=threads)
     bt = torch.from_numpy(np.ascontiguousarray(b.reshape(-1, k) if k > 1 else b)).to(cuda)
     x = plan.solve(bt)
     torch.cuda.synchronize()
@@ -48,18 +49,21 @@ def _matrices():
 MATS = list(_matrices())
 
 
-@pytest.mark.parametrize("algo,sync,lds,bpc", [("wrtfst", "value", False, 1), ("wrtfst", "value", True, 1),
-                                               ("wrtfst", "value", True, 4), ("wrtfst", "flag", False, 2),
-                                               ("twopart", "value", False, 1), ("twopart", "flag", False, 1),
-                                               ("local", "value", False, 1)])
+@pytest.mark.parametrize("algo,sync,lds,bpc,threads", [
+    ("wrtfst", "value", False, 1, 0), ("wrtfst", "value", True, 1, 0), ("wrtfst", "value", True, 4, 0),
+    ("wrtfst", "flag", False, 2, 0), ("wrtfst", "value", False, 1, 128), ("wrtfst", "value", True, 2, 128),
+    ("wrtfst", "flag", False, 1, 128), ("twopart", "value", False, 1, 0), ("twopart", "flag", False, 1, 0),
+    ("local", "value", False, 1, 0)])
 @pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
-def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, lds, bpc, name, mat):
+def test_thread_per_row_bit_exact_fp64(cuda, algo, sync, lds, bpc, threads, name, mat):
     rp, ci, v = mat
     n = len(rp) - 1
     b = api.uniform(7, n)
     ref = oracle.csr_forward(rp, ci, v, b, fused=True)
-    x, info = _solve(cuda, rp, ci, v, b, algo, sync, lds=lds, bpc=bpc)
+    x, info = _solve(cuda, rp, ci, v, b, algo, sync, lds=lds, bpc=bpc, threads=threads)
     assert info["algo"] == algo and info["sync"] == sync
+    if algo == "wrtfst":
+        assert info["row_threads"] == (threads or 256)
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
     # and within tolerance of the reference's unfused serial arithmetic
     assert rel_maxnorm(x, oracle.csr_forward(rp, ci, v, b, fused=False)) <= FP64_TOL
@@ -105,14 +109,15 @@ def test_csc_syncfree(cuda, name, mat):
 
 
 @pytest.mark.parametrize("k", [2, 3, 8, 16, 32, 64])
-@pytest.mark.parametrize("algo,sync,lds", [("wrtfst", "value", False), ("wrtfst", "value", True),
-                                           ("wrtfst", "flag", False), ("local", "value", False)])
-def test_multi_rhs_bit_exact(cuda, k, algo, sync, lds):
+@pytest.mark.parametrize("algo,sync,lds,threads", [("wrtfst", "value", False, 0), ("wrtfst", "value", True, 0),
+                                                   ("wrtfst", "value", True, 128), ("wrtfst", "flag", False, 0),
+                                                   ("local", "value", False, 0)])
+def test_multi_rhs_bit_exact(cuda, k, algo, sync, lds, threads):
     rp, ci, v = api.make_config(0, seed=3, scale=0.2)
     n = len(rp) - 1
     b = api.uniform(100 + k, n * k)
     ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
-    x, _ = _solve(cuda, rp, ci, v, b, algo, sync, k=k, lds=lds)
+    x, _ = _solve(cuda, rp, ci, v, b, algo, sync, k=k, lds=lds, threads=threads)
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
 
 

@@ -213,13 +213,14 @@ def test_error_strings():
     assert L.cap_version().startswith(b"capellini")
 
 
-def test_plan_create_refuses_more_than_four_blocks_per_cu():
+def test_plan_create_refuses_bad_grid_options():
     # argument checks run before any device call
     from ctypes import byref, c_void_p
     rp, ci, v = random_lower(50, seed=3)
     h = c_void_p()
-    for bpc, want in ((5, _native.CAP_ERR_ARG), (8, _native.CAP_ERR_ARG), (-1, _native.CAP_ERR_ARG)):
-        o = _native.default_opts(host_inputs=1, blocks_per_cu=bpc)
+    for bpc, thr in ((5, 0), (8, 0), (-1, 0), (1, 64), (1, 512), (1, 192)):
+        want = _native.CAP_ERR_ARG
+        o = _native.default_opts(host_inputs=1, blocks_per_cu=bpc, row_threads=thr)
         rc = _native.lib().cap_plan_create(byref(h), 50, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
                                            _native.CAP_F64, byref(o), None)
         assert rc == want, (bpc, rc)
