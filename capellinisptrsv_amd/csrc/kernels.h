@@ -295,6 +295,7 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
     typedef typename VBits<T>::U U;
     const int lane = threadIdx.x & (kWave - 1);
     const int d = rp[row + 1] - 1;
+    const T brow = b[row], vd = val[d];  // loaded now, off the critical path
     T s = (T)0;
     int idle = 0;
     bool failed = false;
@@ -330,7 +331,7 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
         if (failed) break;
     }
     if (lane == 0) {
-        const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[row] - s) / val[d];
+        const T xi = failed ? as_val(VBits<T>::kQNaN) : (brow - s) / vd;
         if (win.len) {
             U o = as_bits(xi);
             if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;
@@ -353,6 +354,10 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
     typedef typename VBits<T>::U U;
     constexpr U kS = VBits<T>::kSentinel;
     U *xr = ready_array(sc, x);
+    // b and the diagonal are loaded before the polling loop: read after the
+    // last dependency arrives, they would add a memory round trip per level
+    const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
+    const T bo = b[o], vd = val[d];
     int c0 = 0, c1 = 0;
     T v0 = (T)0, v1 = (T)0;
     U x0 = kS, x1 = kS;
@@ -401,8 +406,7 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
                 j = d;
             }
             if (j >= d) {  // write first: publish inside the polling loop
-                const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
-                const T xi = (b[o] - s) / val[d];
+                const T xi = (bo - s) / vd;
                 U u = as_bits(xi);
                 if (u == kS) u = VBits<T>::kQNaN;
                 if (win.len) win.store(i, q, u);
@@ -433,6 +437,8 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
     // flag protocol: a fixed window of up to kWin entries, refilled when consumed
     typedef typename VBits<T>::U U;
     U *xu = reinterpret_cast<U *>(x);
+    const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
+    const T bo = b[o], vd = val[d];  // before the loop, off the critical path
     int cw[kWin];
     T vw[kWin];
     U xw[kWin];
@@ -501,8 +507,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                 }
             }
             if (used == cnt && j >= d) {  // write first: publish inside the polling loop
-                const long long o = (long long)(n_rev ? n_rev - 1 - i : i) * k + q;
-                publish<T, SYNC>(x, sc, o, (b[o] - s) / val[d]);
+                publish<T, SYNC>(x, sc, o, (bo - s) / vd);
                 asm volatile("" ::: "memory");
                 done = true;
             }
@@ -610,6 +615,8 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
     U p0 = kS, p1 = kS;
     float s0 = 0.0f, s1 = 0.0f;
     double s = 0.0;
+    const long long o = (long long)i * k + q;
+    const double vd = val[d], bo = b[o];  // before the loop, off the critical path
     if (j < d) {
         c0 = ci[j];
         v0 = val[j];
@@ -656,8 +663,6 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
                 j = d;
             }
             if (j >= d) {  // write first: publish inside the polling loop
-                const long long o = (long long)i * k + q;
-                const double vd = val[d], bo = b[o];
                 const float vd32 = (float)vd;
                 const float x0 = ((float)bo - s0) / vd32;
                 s = __builtin_fma(vd, (double)x0, s);
@@ -1153,10 +1158,13 @@ __global__ void __launch_bounds__(kBlock)
         bool done = col >= m;
         if (!done && backward) col = m - 1 - col;
         int dpos = 0, start = 0, stop = 0;
+        T bc = (T)0, dg = (T)1;  // loaded before the wait, off the critical path
         if (!done) {
             dpos = backward ? cp[col + 1] - 1 : cp[col];
             start = backward ? cp[col] : cp[col] + 1;
             stop = backward ? cp[col + 1] - 1 : cp[col + 1];
+            bc = b[col];
+            dg = cv[dpos];
         }
         int idle = 0;
         do {  // wave-uniform exit
@@ -1165,7 +1173,7 @@ __global__ void __launch_bounds__(kBlock)
                 const bool failed = !ready && give_up(sc, ++idle);
                 if (ready || failed) {
                     const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
-                    const T xi = failed ? as_val(VBits<T>::kQNaN) : (b[col] - ls) / cv[dpos];
+                    const T xi = failed ? as_val(VBits<T>::kQNaN) : (bc - ls) / dg;
                     T token = (T)0;
                     for (int j = start + g; j < stop; j += G)
                         token += __hip_atomic_fetch_add(left_sum + ri[j], xi * cv[j], __ATOMIC_RELAXED,
@@ -1203,6 +1211,7 @@ __global__ void __launch_bounds__(kBlock)
         if (t >= n) return;
         const int i = n - 1 - t;
         const int a = rp[i], d = rp[i + 1] - 1;
+        const double gi = g[i], vd = val[d];  // before the wait, off the critical path
         int idle = 0;
         bool done = false;
         do {  // wave-uniform exit, as in wrtfst_row
@@ -1214,7 +1223,7 @@ __global__ void __launch_bounds__(kBlock)
                     // load after the count was seen; the count decrements
                     // depend on the returned left-sum atomics
                     const double ls = as_val(ld_agent(reinterpret_cast<const unsigned long long *>(left_sum) + i));
-                    const double y = failed ? as_val(VBits<double>::kQNaN) : (g[i] - ls) / val[d];
+                    const double y = failed ? as_val(VBits<double>::kQNaN) : (gi - ls) / vd;
                     grad_b[i] = y;
                     double token = 0.0;
                     for (int j = a; j < d; ++j)
