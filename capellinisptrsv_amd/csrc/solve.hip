@@ -67,7 +67,8 @@ struct cap_plan_s {
     int *err = nullptr;
     int spin_limit = kSpinLimitDefault;
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
-    int wblock = kBlock;     // threads per workgroup of k_wrtfst / k_refine2 (128 or 256)
+    int wblock = kBlock;     // threads per workgroup of k_wrtfst (128 or 256)
+    int rblock = 0;          // of k_refine2; 0 = not chosen yet (autotuned plans time it)
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
@@ -863,6 +864,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         plan_free(p);
         return rc;
     }
+    if (o.algo != CAP_ALGO_AUTO) p->rblock = p->wblock;
     *out = p;
     return CAP_OK;
 }
@@ -1144,7 +1146,8 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     CAP_HIP_TRY(hipGetLastError());
     if (p->n == 0) return CAP_OK;
     const int l = lpr_log2_for(rhs);
-    const int rows_per_block = p->wblock >> l;
+    const int block = p->rblock ? p->rblock : p->wblock;
+    const int rows_per_block = block >> l;
     const bool use_sched = p->sched && p->sched_rows == rows_per_block;
     const int2 *sched = use_sched ? p->sched : nullptr;
     const int n_sched = use_sched ? p->n_sched : 0;
@@ -1152,10 +1155,10 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     const double *val = (const double *)p->val;
     auto go = [&](auto kernel) {
-        kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
-                                         p->norms);
+        kernel<<<nb, block, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,
+                                     p->norms);
     };
-    const bool small = p->wblock == kBlockSmall;
+    const bool small = block == kBlockSmall;
     if (p->lds)
         small ? go(k_refine2<1, kBlockSmall>) : go(k_refine2<1, kBlock>);
     else
@@ -1185,6 +1188,37 @@ int refine_step(cap_plan p, const double *b, double *x, int rhs, bool first, hip
     return CAP_OK;
 }
 
+// Workgroup size of the fused sweep on an autotuned plan: its critical path
+// (two FP32 divisions and an FP64 fma per row) is longer than the solve's, so
+// its best size can differ.  The first refined solve times 128 and 256
+// threads on the caller's b and x (x is overwritten by the solve that
+// follows) and keeps the faster; 256 must be 2% faster to win.
+int tune_refine_block(cap_plan p, const double *b, double *x, int rhs, hipStream_t st)
+{
+    hipEvent_t e0, e1;
+    CAP_HIP_TRY(hipEventCreate(&e0));
+    CAP_HIP_TRY(hipEventCreate(&e1));
+    float best[2] = {1e30f, 1e30f};
+    const int sizes[2] = {kBlockSmall, kBlock};
+    int rc = CAP_OK;
+    for (int c = 0; c < 2 && rc == CAP_OK; ++c) {
+        p->rblock = sizes[c];
+        for (int r = 0; r < 3 && rc == CAP_OK; ++r) {
+            (void)hipEventRecord(e0, st);
+            rc = refine_fused(p, b, x, rhs, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = CAP_ERR_HIP;
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            if (r > 0) best[c] = std::min(best[c], t);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->rblock = best[1] < 0.98f * best[0] ? kBlock : kBlockSmall;
+    return rc;
+}
+
 int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, int max_iter, int *iters,
                  double *last_rel, hipStream_t st)
 {
@@ -1196,6 +1230,11 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
     // fused sweep publishes 8-byte pairs, the correction solves 4-byte values
     const bool fused = refine_fuses(p);
     if (p->staged) CAP_TRY(ensure_ready(p, (fused ? 8 : 4) * (size_t)p->n * rhs));
+    if (fused && !p->rblock) {
+        CAP_TRY(tune_refine_block(p, b, x, rhs, st));
+        if (p->refine_graph) CAP_HIP_TRY(hipGraphExecDestroy(p->refine_graph));
+        p->refine_graph = nullptr;
+    }
     // The first two steps (solve from b, one correction) are one graph:
     // captured once per (b, x, rhs), replayed on later calls.
     if (!p->refine_graph || p->graph_b != b || p->graph_x != x || p->graph_rhs != rhs) {
