@@ -1337,4 +1337,168 @@ __global__ void __launch_bounds__(kBlock)
     for (int j = rp[i]; j < rp[i + 1]; ++j) ci[j] = ci_local[j] + off;
 }
 
+// ------------------------------------------------ partitioned banded solve
+// For deep matrices whose lower bandwidth w = max(i - col) is at most kPartW
+// (the dependency chain is ~n/levels rows long and no row order can shorten
+// it).  Rows are cut into P blocks of m >= kPartW rows; block p's rows only
+// reach back into the last kPartW rows of block p-1.  With D the
+// block-diagonal part of L, y = D^{-1} b (independent blocks) and V_p =
+// D_pp^{-1} C_p (C_p: block p's entries in the coupling columns [pm - kPartW,
+// pm)), the exact solution is x_p = y_p - V_p t_{p-1} with t_p the last
+// kPartW values of x_p:
+//   t_p = a_p + B_p t_{p-1},   a_p = tail(y_p),   B_p = -tail(V_p).
+// The recurrence is an inclusive scan of affine maps, done in ceil(log2 Q)
+// rounds (Q = P - 1 tails): the matrix products are formed once per plan
+// (k_part_gemm); a solve only updates the vectors (k_part_scan).  The sums
+// are reassociated, so results match the sequential solve to rounding, not
+// bit for bit.  Matrices are column-major kPartW x kPartW (element (c, k) at
+// k * kPartW + c); V is two halves of 64 interleaved columns (n x 64 each).
+constexpr int kPartW = 128;
+
+// Coupling right-hand sides (two zeroed halves of 64 interleaved columns):
+// an entry of row i in block p >= 1 whose column c lies before the block
+// goes to column q = c - (pm - kPartW); duplicated entries add up.
+__global__ void __launch_bounds__(kBlock)
+    k_part_coupling(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val, int n,
+                    int m, double *__restrict__ r0, double *__restrict__ r1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int start = (i / m) * m;
+    if (start == 0) return;
+    const int base = start - kPartW;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        const int q = c - base;
+        if (c >= start || q < 0) continue;  // inside the block; q < 0 cannot occur (bandwidth checked on the host)
+        double *dst = q < 64 ? r0 + (long long)i * 64 + q : r1 + (long long)i * 64 + (q - 64);
+        *dst += val[j];
+    }
+}
+
+// B_p = -(the last kPartW rows of V_p) for p in [1, q_count), stored at
+// index p - 1 (the first round's matrices).
+__global__ void __launch_bounds__(kBlock)
+    k_part_tail_matrix(const double *__restrict__ v0, const double *__restrict__ v1, int m, int q_count,
+                       double *__restrict__ b0)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)(q_count - 1) * kPartW * kPartW) return;
+    const int c = (int)(g % kPartW);
+    const int k = (int)((g / kPartW) % kPartW);
+    const int p = 1 + (int)(g / ((long long)kPartW * kPartW));
+    const long long row = (long long)(p + 1) * m - kPartW + c;
+    b0[g] = -(k < 64 ? v0[row * 64 + k] : v1[row * 64 + (k - 64)]);
+}
+
+// One round of the plan-time scan: dst_p = src_p * src_{p-s} for p in
+// [2s, q_count).  src holds p in [s, q_count) at index p - s, dst holds p in
+// [2s, q_count) at index p - 2s.  One workgroup per product; each of the 256
+// threads owns an 8 x 8 tile of the result; the inner dimension is staged
+// through LDS 16 at a time.
+__global__ void __launch_bounds__(256)
+    k_part_gemm(const double *__restrict__ src, double *__restrict__ dst, int s, int q_count)
+{
+    constexpr long long kMat = (long long)kPartW * kPartW;
+    const int p = 2 * s + (int)blockIdx.x;
+    if (p >= q_count) return;
+    const double *A = src + (long long)(p - s) * kMat;
+    const double *B = src + (long long)(p - 2 * s) * kMat;
+    double *C = dst + (long long)(p - 2 * s) * kMat;
+    __shared__ double sa[16][kPartW];  // A(c, j0 + jj)
+    __shared__ double sb[16][kPartW];  // B(j0 + jj, k)
+    const int t = threadIdx.x;
+    const int c0 = (t & 15) * 8, k0 = (t >> 4) * 8;
+    double acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int j0 = 0; j0 < kPartW; j0 += 16) {
+        __syncthreads();
+        for (int e = t; e < 16 * kPartW; e += 256) {
+            sa[e / kPartW][e % kPartW] = A[(long long)(j0 + e / kPartW) * kPartW + e % kPartW];
+            sb[e % 16][e / 16] = B[(long long)(e / 16) * kPartW + j0 + e % 16];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < 16; ++jj) {
+            double av[8], bv[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) av[a] = sa[jj][c0 + a];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) bv[b] = sb[jj][k0 + b];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_fma(av[a], bv[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) C[(long long)(k0 + b) * kPartW + c0 + a] = acc[a][b];
+}
+
+// a_p = the last kPartW values of y in block p (p < q_count): the scan's input
+__global__ void __launch_bounds__(kBlock)
+    k_part_tails(const double *__restrict__ y, int m, int q_count, double *__restrict__ a)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)q_count * kPartW) return;
+    const int p = (int)(g / kPartW), c = (int)(g % kPartW);
+    a[g] = y[(long long)(p + 1) * m - kPartW + c];
+}
+
+// One round of the per-solve scan (one workgroup of kPartW threads per p):
+// dst_p = src_p + B_p src_{p-s} for p >= s (B_p at index p - s of the
+// round's matrices), dst_p = src_p below s.
+__global__ void __launch_bounds__(kPartW)
+    k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s)
+{
+    const int p = (int)blockIdx.x;
+    const int c = (int)threadIdx.x;
+    __shared__ double sv[kPartW];
+    double acc = src[(long long)p * kPartW + c];
+    if (p >= s) {  // uniform in the workgroup
+        sv[c] = src[(long long)(p - s) * kPartW + c];
+        __syncthreads();
+        const double *B = bm + (long long)(p - s) * kPartW * kPartW;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+        for (int k = 0; k < kPartW; k += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[u] = __builtin_fma(B[(long long)(k + u) * kPartW + c], sv[k + u], d[u]);
+        acc += (d[0] + d[1]) + (d[2] + d[3]);
+    }
+    dst[(long long)p * kPartW + c] = acc;
+}
+
+// x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in block 0
+__global__ void __launch_bounds__(kBlock)
+    k_part_finish(const double *__restrict__ y, const double *__restrict__ v0, const double *__restrict__ v1,
+                  const double *__restrict__ t, int n, int m, double *__restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = i / m;
+    double xi = y[i];
+    if (p > 0) {
+        const double2 *r0 = reinterpret_cast<const double2 *>(v0 + (long long)i * 64);
+        const double2 *r1 = reinterpret_cast<const double2 *>(v1 + (long long)i * 64);
+        const double *tp = t + (long long)(p - 1) * kPartW;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const double2 a = r0[k], b = r1[k];
+            s0 = __builtin_fma(a.x, tp[2 * k], s0);
+            s0 = __builtin_fma(a.y, tp[2 * k + 1], s0);
+            s1 = __builtin_fma(b.x, tp[64 + 2 * k], s1);
+            s1 = __builtin_fma(b.y, tp[64 + 2 * k + 1], s1);
+        }
+        xi -= s0 + s1;
+    }
+    x[i] = xi;
+}
+
 }  // namespace cap

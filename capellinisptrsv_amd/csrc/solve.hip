@@ -49,6 +49,20 @@ inline int lpr_log2_for(int k)
 
 }  // namespace
 
+// State of a partitioned banded solve: blocks of m rows (P of them, Q = P - 1
+// coupled tails), the block-diagonal part as a one-RHS batch, V = D^{-1} C in
+// two halves of 64 interleaved columns, the scan's matrices of every round
+// (round r holds Q - 2^r matrices from index boff[r]) and the solve's work
+// vectors.
+struct PartState {
+    int m = 0, P = 0, Q = 0, rounds = 0;
+    cap_batch yb = nullptr;
+    double *v0 = nullptr, *v1 = nullptr;
+    double *bm = nullptr;
+    std::vector<long long> boff;
+    double *y = nullptr, *a0 = nullptr, *a1 = nullptr;
+};
+
 struct cap_plan_s {
     int n = 0, nnz = 0, vt = CAP_F64, vbytes = 8;
     int algo = CAP_ALGO_WRTFST, sync = CAP_SYNC_VALUE, border = kMixBorderDefault;
@@ -111,6 +125,8 @@ struct cap_plan_s {
     const void *graph_b = nullptr;
     void *graph_x = nullptr;
     int graph_rhs = 0;
+    // partitioned banded solve (CAP_ALGO_PARTITION)
+    struct PartState *part = nullptr;
 };
 
 struct cap_batch_s {
@@ -223,9 +239,22 @@ int ensure_ready(cap_plan p, size_t bytes)
     return ready_acquire(p->device, bytes, &p->ready, &p->ready_bytes);
 }
 
+void free_partition(cap_plan p)
+{
+    PartState *ps = p->part;
+    if (!ps) return;
+    (void)hipDeviceSynchronize();
+    if (ps->yb) cap_batch_destroy(ps->yb);
+    for (double *b : {ps->v0, ps->v1, ps->bm, ps->y, ps->a0, ps->a1})
+        if (b) (void)hipFree(b);
+    delete ps;
+    p->part = nullptr;
+}
+
 void plan_free(cap_plan p)
 {
     if (!p) return;
+    free_partition(p);
     if (p->ready) {
         (void)hipDeviceSynchronize();  // no kernel of this plan may still poll the block
         ready_release(p->ready);
@@ -291,6 +320,11 @@ template <typename T>
 int launch_prepare(cap_plan p, int algo, int sync, void *x, long long count, hipStream_t st)
 {
     typedef typename VBits<T>::U U;
+    if (algo == CAP_ALGO_PARTITION) {
+        // the block solve resets itself; several right-hand sides go to LOCAL
+        if (count == p->n && sizeof(T) == 8) return CAP_OK;
+        algo = CAP_ALGO_LOCAL;
+    }
     if (algo == CAP_ALGO_LOCAL) sync = CAP_SYNC_VALUE;
     if (sync == CAP_SYNC_VALUE) {
         void *target = x;
@@ -383,11 +417,41 @@ int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
     return CAP_OK;
 }
 
+int plan_solve_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st);
+
+// One solve of a partitioned plan (one FP64 right-hand side): y = D^{-1} b
+// with the block batch, the block tails, ceil(log2 Q) scan rounds, then
+// x = y - V t.  Every step is on `st`.
+int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
+{
+    PartState *ps = p->part;
+    if (!ps) return CAP_ERR_ARG;
+    const cap_plan yp = ps->yb->plan;
+    CAP_TRY(plan_solve_with(yp, yp->algo, yp->sync, b, ps->y, 1, st));
+    const int Q = ps->Q;
+    const long long mat = (long long)kPartW * kPartW;
+    double *cur = ps->a0, *nxt = ps->a1;
+    k_part_tails<<<blocks_for((long long)Q * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, cur);
+    for (int r = 0; r < ps->rounds; ++r) {
+        k_part_scan<<<Q, kPartW, 0, st>>>(cur, nxt, ps->bm + ps->boff[r] * mat, 1 << r);
+        std::swap(cur, nxt);
+    }
+    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->v0, ps->v1, cur, p->n, ps->m, x);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
+
 template <typename T>
 int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x, int rhs, hipStream_t st)
 {
     const int n = p->n;
     if (n == 0) return CAP_OK;
+    if (algo == CAP_ALGO_PARTITION) {
+        if constexpr (sizeof(T) == 8) {
+            if (rhs == 1) return launch_partition(p, b, x, st);
+        }
+        algo = CAP_ALGO_LOCAL;  // the partition's tables are built for one FP64 right-hand side
+    }
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
     if (algo == CAP_ALGO_WRTFST) {
@@ -613,6 +677,111 @@ void apply_choice(cap_plan p, const TuneChoice &c)
     p->wblock = c.block;
 }
 
+// Tables of the partitioned banded solve (CAP_ALGO_PARTITION), built once:
+// FP64 plans whose lower bandwidth is at most kPartW and that split into at
+// least two blocks; CAP_ERR_UNSUPPORTED otherwise.  Blocks hold
+// m = max(kPartW, ceil(n / kPartBlocks)) rows.  The block-diagonal part
+// becomes a one-RHS batch (its creation times its own kernels); V = D^{-1} C
+// comes from two 64-RHS solves of a second batch over the same blocks,
+// dropped afterwards; the scan's matrices are formed round by round.
+constexpr int kPartBlocks = 512;
+
+int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
+{
+    if (p->part) return CAP_OK;
+    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
+    const int n = p->n, nnz = p->nnz;
+    const int m = std::max(kPartW, (n + kPartBlocks - 1) / kPartBlocks);
+    const int P = n > 0 ? (n + m - 1) / m : 0;
+    if (P < 2) return CAP_ERR_UNSUPPORTED;
+    std::vector<int> ci((size_t)nnz);
+    CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
+    int w = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) w = std::max(w, i - ci[j]);
+    if (w > kPartW) return CAP_ERR_UNSUPPORTED;
+    std::vector<double> v((size_t)nnz);
+    CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
+    // the block-diagonal part as a batch: entries of each block, local columns
+    std::vector<int> ro((size_t)P + 1), rpd((size_t)n + 1), cid;
+    std::vector<double> vd;
+    cid.reserve((size_t)nnz);
+    vd.reserve((size_t)nnz);
+    for (int q = 0; q <= P; ++q) ro[(size_t)q] = (int)std::min<long long>(n, (long long)q * m);
+    for (int i = 0; i < n; ++i) {
+        const int start = (i / m) * m;
+        rpd[(size_t)i] = (int)cid.size();
+        for (int j = rp_host[i]; j < rp_host[i + 1]; ++j)
+            if (ci[j] >= start) {
+                cid.push_back(ci[j] - start);
+                vd.push_back(v[j]);
+            }
+    }
+    rpd[(size_t)n] = (int)cid.size();
+    PartState *ps = new PartState();
+    ps->m = m;
+    ps->P = P;
+    ps->Q = P - 1;
+    const int Q = ps->Q;
+    while ((1 << ps->rounds) < Q) ++ps->rounds;
+    ps->boff.assign((size_t)ps->rounds + 1, 0);
+    for (int r = 0; r < ps->rounds; ++r) ps->boff[(size_t)r + 1] = ps->boff[(size_t)r] + (Q - (1 << r));
+    cap_opts o;
+    cap_opts_default(&o);
+    o.host_inputs = 1;
+    o.validate = 0;
+    o.spin_limit = p->spin_limit;
+    cap_batch b64 = nullptr;
+    double *r0 = nullptr, *r1 = nullptr;
+    const size_t half = (size_t)n * 64;
+    const long long mat = (long long)kPartW * kPartW;
+    int rc = CAP_OK;
+    do {
+        if ((rc = cap_batch_create(&ps->yb, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 1, &o, st)))
+            break;
+        o.readiness = CAP_READY_IN_PLACE;  // plan-time only: keep the uncached pool at the solve's size
+        if ((rc = cap_batch_create(&b64, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 64, &o, st)))
+            break;
+        // the block solve reports to this plan's watchdog word
+        ps->yb->plan->err = p->err;
+        if ((rc = dev_alloc(&r0, half)) || (rc = dev_alloc(&r1, half)) || (rc = dev_alloc(&ps->v0, half)) ||
+            (rc = dev_alloc(&ps->v1, half)) || (rc = dev_alloc(&ps->y, (size_t)n)) ||
+            (rc = dev_alloc(&ps->a0, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->a1, (size_t)Q * kPartW)))
+            break;
+        if (ps->rounds > 0 && (rc = dev_alloc(&ps->bm, (size_t)(ps->boff[(size_t)ps->rounds] * mat)))) break;
+        if (hipMemsetAsync(r0, 0, half * sizeof(double), st) != hipSuccess ||
+            hipMemsetAsync(r1, 0, half * sizeof(double), st) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        k_part_coupling<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, n, m, r0, r1);
+        if (hipGetLastError() != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        if ((rc = cap_batch_solve(b64, r0, ps->v0, st)) || (rc = cap_batch_solve(b64, r1, ps->v1, st))) break;
+        if ((rc = cap_plan_status(b64->plan, st)) != CAP_OK) break;
+        if (ps->rounds > 0) {
+            k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->v0, ps->v1, m, Q,
+                                                                                       ps->bm);
+            for (int r = 0; r + 1 < ps->rounds; ++r) {
+                const int s = 1 << r;
+                if (Q - 2 * s > 0)
+                    k_part_gemm<<<Q - 2 * s, 256, 0, st>>>(ps->bm + ps->boff[(size_t)r] * mat,
+                                                          ps->bm + ps->boff[(size_t)r + 1] * mat, s, Q);
+            }
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = CAP_ERR_HIP;
+    } while (0);
+    (void)hipStreamSynchronize(st);
+    if (b64) cap_batch_destroy(b64);
+    if (r0) (void)hipFree(r0);
+    if (r1) (void)hipFree(r1);
+    p->part = ps;
+    if (rc != CAP_OK) free_partition(p);
+    return rc;
+}
+
 // Candidates are timed on the plan's own matrix (b = 0; the dependency
 // structure, not the values, sets the time) and the fastest is cached per
 // matrix shape: thread-per-row with and without the LDS window, at 1 or 2
@@ -630,6 +799,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             apply_choice(p, it->second);
             if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, it->second.border);
             if (p->algo == CAP_ALGO_LOCAL) return ensure_segments(p, nullptr);
+            if (p->algo == CAP_ALGO_PARTITION) return build_partition(p, rp_host, st);
             return CAP_OK;
         }
     }
@@ -655,8 +825,12 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         const int levels = p->n ? cap_matrix_layer(p->n, rp_host.data(), ci.data(), nullptr, nullptr, nullptr) : 1;
         // timed first: on such matrices it sets a low bound that lets the
         // slower candidates stop after their warm-up
-        if ((double)p->n / std::max(1, levels) <= kLocalMaxWidth)
+        if ((double)p->n / std::max(1, levels) <= kLocalMaxWidth) {
             cands.insert(cands.begin(), {CAP_ALGO_LOCAL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
+            // deep and narrow-banded: the partitioned solve, timed before all
+            if (build_partition(p, rp_host, st) == CAP_OK)
+                cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
+        }
     }
     for (int bd : {8, 16, 32})
         if (avg * 4 >= bd) {
@@ -687,6 +861,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     apply_choice(p, pick);
     if (pick.algo == CAP_ALGO_MIX) CAP_TRY(build_mix(p, rp_host, pick.border));
     if (pick.algo == CAP_ALGO_LOCAL) CAP_TRY(ensure_segments(p, nullptr));
+    if (pick.algo != CAP_ALGO_PARTITION) free_partition(p);
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune_cache[key] = pick;
     return CAP_OK;
@@ -856,6 +1031,8 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
             rc = autotune(p, rp_host, st);
         } else if (p->algo == CAP_ALGO_LOCAL) {
             rc = ensure_segments(p, nullptr);
+        } else if (p->algo == CAP_ALGO_PARTITION) {
+            rc = build_partition(p, rp_host, st);
         } else if (p->algo != CAP_ALGO_WRTFST && p->algo != CAP_ALGO_TWOPART) {
             rc = CAP_ERR_ARG;
         }
@@ -1102,8 +1279,9 @@ inline unsigned stream_blocks(const cap_plan p, long long count)
 }
 
 // Plans whose solve is thread-per-row run the first two refinement steps as
-// one fused sweep; one-workgroup (LOCAL) plans run them as separate solves.
-inline bool refine_fuses(const cap_plan p) { return p->algo != CAP_ALGO_LOCAL; }
+// one fused sweep; one-workgroup (LOCAL) and partitioned plans run them as
+// separate one-workgroup solves.
+inline bool refine_fuses(const cap_plan p) { return p->algo != CAP_ALGO_LOCAL && p->algo != CAP_ALGO_PARTITION; }
 
 int ensure_refine(cap_plan p, int rhs, hipStream_t st)
 {
@@ -1179,8 +1357,9 @@ int refine_step(cap_plan p, const double *b, double *x, int rhs, bool first, hip
         k_residual32<<<stream_blocks(p, len), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, b, x, p->r32,
                                                                 p->n, rhs, p->norms + 1);
     CAP_HIP_TRY(hipGetLastError());
-    // the correction solves use the plan's multi-RHS path: LOCAL or WRTFST
-    const int algo = p->algo == CAP_ALGO_LOCAL ? CAP_ALGO_LOCAL : CAP_ALGO_WRTFST;
+    // the correction solves use the plan's multi-RHS path: LOCAL (deep plans,
+    // partitioned ones included) or WRTFST
+    const int algo = p->algo == CAP_ALGO_LOCAL || p->algo == CAP_ALGO_PARTITION ? CAP_ALGO_LOCAL : CAP_ALGO_WRTFST;
     CAP_TRY(launch_prepare<float>(p, algo, CAP_SYNC_VALUE, p->d32, len, st));
     CAP_TRY(launch_kernel<float>(p, algo, CAP_SYNC_VALUE, p->val32, p->r32, p->d32, rhs, st));
     k_accumulate<<<stream_blocks(p, len), kBlock, 0, st>>>(x, p->d32, len, p->norms, first ? 1 : 0);

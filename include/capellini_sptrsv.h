@@ -59,6 +59,12 @@ extern "C" {
 #define CAP_ALGO_LOCAL 5   /* one workgroup per matrix, wavefront per row, dependencies
                               resolved inside one compute unit: deep matrices
                               (few rows per level); value protocol on x itself */
+#define CAP_ALGO_PARTITION 6 /* FP64, lower bandwidth <= 128: row blocks solved
+                              independently, joined by a scan over the 128-value
+                              block tails (tables built at plan creation); sums
+                              are reassociated (within rounding of the
+                              sequential solve, not bit-exact).  Several
+                              right-hand sides run as CAP_ALGO_LOCAL. */
 
 /* adjoint method (cap_opts.adjoint) */
 #define CAP_ADJOINT_AUTO 0

@@ -1,0 +1,109 @@
+"""GPU tests of the partitioned banded solve (CAP_ALGO_PARTITION).
+
+The partition reassociates the sums (block solves joined by a scan over the
+128-value block tails), so its bar is the FP64 tolerance: relative max-norm
+error <= 1e-12 against the oracle's sequential forward substitution.  Cases:
+bandwidths 1 (one row per level), 16, 100 and 128 (the limit), block counts
+from 2 up, a ragged last block, duplicated entries, repeated solves (same
+bits each time), several right-hand sides (run by the one-workgroup kernel:
+bit-exact), refinement, and the refusals (bandwidth above 128, FP32, too few
+rows for two blocks).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from capellinisptrsv_amd import api
+from conftest import random_lower, rel_maxnorm
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _chain(n):
+    rp = np.r_[0, 1, np.arange(3, 2 * n, 2)].astype(np.int32)
+    ci = np.r_[0, np.c_[np.arange(0, n - 1), np.arange(1, n)].reshape(-1)].astype(np.int32)
+    return rp, ci, np.r_[1.0, np.tile([0.5, 2.0], n - 1)]
+
+
+def _cases():
+    yield "deep_band_20k", api.make_config(2, seed=5, scale=0.04)  # bandwidth 128, ~38 nnz/row
+    yield "chain_5000", _chain(5000)
+    yield "band16", api.generate("band_rowcount", n=9000, offdiag=6, band=16, seed=3)
+    yield "band100_ragged", api.generate("band_rowcount", n=70001, offdiag=12, band=100, seed=4)
+    yield "band128_two_blocks", api.generate("band_rowcount", n=200, offdiag=10, band=128, seed=6)
+    yield "dup", random_lower(4000, max_off=6, band=90, seed=7, dup=True)
+
+
+CASES = list(_cases())
+
+
+def _plan(cuda, rp, ci, v, **kw):
+    import torch
+    from capellinisptrsv_amd import Plan
+    return Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, v)), **kw)
+
+
+@pytest.mark.parametrize("name,mat", CASES, ids=[c[0] for c in CASES])
+def test_partition_within_fp64_tolerance(cuda, name, mat):
+    import torch
+    rp, ci, v = mat
+    n = len(rp) - 1
+    b = api.uniform(31, n)
+    plan = _plan(cuda, rp, ci, v, algo="partition")
+    assert plan.info()["algo"] == "partition"
+    bt = torch.from_numpy(b).to(cuda)
+    x1 = plan.solve(bt).cpu().numpy()
+    x2 = plan.solve(bt).cpu().numpy()
+    assert plan.status() == 0
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    assert rel_maxnorm(x1, ref) <= FP64_TOL, rel_maxnorm(x1, ref)
+    assert np.array_equal(x1, x2)  # deterministic: same bits on every solve
+    plan.close()
+
+
+def test_partition_several_rhs_and_refinement(cuda):
+    import torch
+    rp, ci, v = api.make_config(2, seed=8, scale=0.02)
+    n = len(rp) - 1
+    plan = _plan(cuda, rp, ci, v, algo="partition")
+    k = 4
+    b = api.uniform(32, n * k)
+    x = plan.solve(torch.from_numpy(b.reshape(n, k)).to(cuda)).cpu().numpy().reshape(-1)
+    assert plan.status() == 0
+    assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True))  # one-workgroup path
+    b1 = api.uniform(33, n)
+    xr, iters, rel = plan.solve_refined(torch.from_numpy(b1).to(cuda), tol=1e-12)
+    assert rel_maxnorm(xr.cpu().numpy(), oracle.csr_forward(rp, ci, v, b1, fused=True)) <= FP64_TOL
+    plan.close()
+
+
+def test_partition_refusals(cuda):
+    from capellinisptrsv_amd import CapError
+    rp, ci, v = api.generate("band_rowcount", n=5000, offdiag=8, band=300, seed=9)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v, algo="partition")  # bandwidth 300 > 128
+    rp, ci, v = api.generate("band_rowcount", n=5000, offdiag=8, band=64, seed=9)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v.astype(np.float32), algo="partition")  # FP64 only
+    rp, ci, v = api.generate("band_rowcount", n=128, offdiag=4, band=64, seed=9)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v, algo="partition")  # one block
+
+
+def test_autotuner_on_a_deep_band(cuda):
+    import torch
+    rp, ci, v = api.make_config(2, seed=10, scale=0.04)
+    n = len(rp) - 1
+    plan = _plan(cuda, rp, ci, v)
+    info = plan.info()
+    b = api.uniform(34, n)
+    x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+    assert plan.status() == 0
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    if info["algo"] == "partition":
+        assert rel_maxnorm(x, ref) <= FP64_TOL
+    else:
+        assert np.array_equal(x, ref), info
+    plan.close()
