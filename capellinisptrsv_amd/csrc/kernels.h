@@ -1376,11 +1376,22 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// V from the two 64-column halves the batch solve writes (row-major, 64
+// per row) to column-major vt (column k at k * n): the finish then reads
+// consecutive rows per column.
+__global__ void __launch_bounds__(kBlock)
+    k_part_transpose(const double *__restrict__ v0, const double *__restrict__ v1, int n, double *__restrict__ vt)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)n * kPartW) return;
+    const int k = (int)(g / n), i = (int)(g % n);
+    vt[g] = k < 64 ? v0[(long long)i * 64 + k] : v1[(long long)i * 64 + (k - 64)];
+}
+
 // B_p = -(the last kPartW rows of V_p) for p in [1, q_count), stored at
 // index p - 1 (the first round's matrices).
 __global__ void __launch_bounds__(kBlock)
-    k_part_tail_matrix(const double *__restrict__ v0, const double *__restrict__ v1, int m, int q_count,
-                       double *__restrict__ b0)
+    k_part_tail_matrix(const double *__restrict__ vt, int n, int m, int q_count, double *__restrict__ b0)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)(q_count - 1) * kPartW * kPartW) return;
@@ -1388,7 +1399,116 @@ __global__ void __launch_bounds__(kBlock)
     const int k = (int)((g / kPartW) % kPartW);
     const int p = 1 + (int)(g / ((long long)kPartW * kPartW));
     const long long row = (long long)(p + 1) * m - kPartW + c;
-    b0[g] = -(k < 64 ? v0[row * 64 + k] : v1[row * 64 + (k - 64)]);
+    b0[g] = -vt[(long long)k * n + row];
+}
+
+// y = D^{-1} b: one wavefront per block, rows in order.  The block's
+// in-block off-diagonal entries are stored ELL-style, kPartEll per row
+// (column, value; column -1 pads), the rest of a longer row in an overflow
+// CSR (ov_rp per row).  Entry l of a row sits in lane l; the products with
+// the block's earlier values are summed by a DPP tree over the lanes and
+// multiplied by the reciprocal diagonal.  Row data are loaded a group of
+// kPartPre rows ahead.  The sum is reassociated and the division is a
+// multiplication (tolerance, not bit-exact), like the whole partitioned
+// solve.
+constexpr int kPartEll = 64;
+constexpr int kPartPre = 16;
+
+// A double moved between lanes by DPP (row shifts and row broadcasts of
+// the GFX9 DPP set); lanes with no source, or outside ROW_MASK, read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes, in every lane: prefix sums within rows of 16 by
+// row_shr 1, 2, 4, 8 (lane 15 of a row then holds the row's sum), row 0's
+// sum broadcast into row 1 and row 2's into row 3 (row_bcast:15), then lane
+// 31's into rows 2 and 3 (row_bcast:31): lane 63 holds the total.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_d<0x111, 0xf>(v);
+    v += dpp_d<0x112, 0xf>(v);
+    v += dpp_d<0x114, 0xf>(v);
+    v += dpp_d<0x118, 0xf>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    return bcast(v, kWave - 1);
+}
+
+__global__ void __launch_bounds__(kWave)
+    k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
+                 const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
+                 const double *__restrict__ b, double *__restrict__ y, int n, int m)
+{
+    constexpr int kRing = 2 * kPartW;
+    __shared__ double ring[kRing];
+    const int lane = threadIdx.x;
+    const int start = blockIdx.x * m;
+    const int end = min(n, start + m);
+    // a group's row data: entry `lane` of each of its kPartPre rows, and, in
+    // lane r, row r's b, reciprocal diagonal and overflow start (lane
+    // kPartPre: the end)
+    int nc[kPartPre];
+    double nv[kPartPre], nb = 0.0, nd = 1.0;
+    int no = 0;
+    auto fetch = [&](int g0) {
+#pragma unroll
+        for (int r = 0; r < kPartPre; ++r) {
+            const long long e = (long long)(g0 + r) * kPartEll + lane;
+            const bool in = g0 + r < end;
+            nc[r] = in ? ell_c[e] : -1;
+            nv[r] = in ? ell_v[e] : 0.0;
+        }
+        const int row = min(g0 + lane, end);
+        nb = lane < kPartPre && row < end ? b[row] : 0.0;
+        nd = lane < kPartPre && row < end ? inv_diag[row] : 1.0;
+        no = lane <= kPartPre ? ov_rp[row] : 0;
+    };
+    // Every lane computes each row's value.  The ring (rows' values by row
+    // mod kRing) is read one row ahead: row i+1's entry is fetched while row
+    // i is summed (its column is below i, so already written -- one
+    // wavefront's LDS accesses complete in order); an entry in column i
+    // itself takes the value just computed.
+    double yprev = 0.0, rv = 0.0;
+    fetch(start);
+    for (int g0 = start; g0 < end; g0 += kPartPre) {
+        int cc[kPartPre + 1];
+        double cv[kPartPre];
+#pragma unroll
+        for (int r = 0; r < kPartPre; ++r) {
+            cc[r] = nc[r];
+            cv[r] = nv[r];
+        }
+        const double bg = nb, dg = nd;
+        const int og = no;
+        if (g0 + kPartPre < end) fetch(g0 + kPartPre);
+        cc[kPartPre] = g0 + kPartPre < end ? nc[0] : -1;  // the next group's first row
+#pragma unroll
+        for (int r = 0; r < kPartPre; ++r) {
+            const int i = g0 + r;
+            if (i >= end) continue;  // uniform: the block's last group
+            double prod = 0.0;
+            if (cc[r] >= 0) prod = cv[r] * (cc[r] == i - 1 ? yprev : rv);
+            const int o0 = __builtin_amdgcn_readlane(og, r), o1 = __builtin_amdgcn_readlane(og, r + 1);
+            for (int j = o0 + lane; j < o1; j += kWave) {  // rows longer than kPartEll
+                const int c = ov_c[j];
+                prod = __builtin_fma(ov_v[j], c == i - 1 ? yprev : ring[(c - start) & (kRing - 1)], prod);
+            }
+            const int cn = cc[r + 1];
+            rv = cn >= 0 && cn < i ? ring[(cn - start) & (kRing - 1)] : 0.0;
+            const double yi = (bcast(bg, r) - wave_sum(prod)) * bcast(dg, r);
+            if (lane == 0) {
+                ring[(i - start) & (kRing - 1)] = yi;
+                y[i] = yi;
+            }
+            yprev = yi;
+        }
+    }
 }
 
 // One round of the plan-time scan: dst_p = src_p * src_{p-s} for p in
@@ -1474,29 +1594,24 @@ __global__ void __launch_bounds__(kPartW)
     dst[(long long)p * kPartW + c] = acc;
 }
 
-// x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in block 0
+// x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in
+// block 0.  V column-major: for each column the lanes read consecutive rows.
 __global__ void __launch_bounds__(kBlock)
-    k_part_finish(const double *__restrict__ y, const double *__restrict__ v0, const double *__restrict__ v1,
-                  const double *__restrict__ t, int n, int m, double *__restrict__ x)
+    k_part_finish(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t, int n,
+                  int m, double *__restrict__ x)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = i / m;
     double xi = y[i];
     if (p > 0) {
-        const double2 *r0 = reinterpret_cast<const double2 *>(v0 + (long long)i * 64);
-        const double2 *r1 = reinterpret_cast<const double2 *>(v1 + (long long)i * 64);
         const double *tp = t + (long long)(p - 1) * kPartW;
-        double s0 = 0.0, s1 = 0.0;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
-        for (int k = 0; k < 32; ++k) {
-            const double2 a = r0[k], b = r1[k];
-            s0 = __builtin_fma(a.x, tp[2 * k], s0);
-            s0 = __builtin_fma(a.y, tp[2 * k + 1], s0);
-            s1 = __builtin_fma(b.x, tp[64 + 2 * k], s1);
-            s1 = __builtin_fma(b.y, tp[64 + 2 * k + 1], s1);
-        }
-        xi -= s0 + s1;
+        for (int k = 0; k < kPartW; k += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = __builtin_fma(vt[(long long)(k + u) * n + i], tp[k + u], s[u]);
+        xi -= (s[0] + s[1]) + (s[2] + s[3]);
     }
     x[i] = xi;
 }

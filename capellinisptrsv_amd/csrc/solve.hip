@@ -50,14 +50,15 @@ inline int lpr_log2_for(int k)
 }  // namespace
 
 // State of a partitioned banded solve: blocks of m rows (P of them, Q = P - 1
-// coupled tails), the block-diagonal part as a one-RHS batch, V = D^{-1} C in
-// two halves of 64 interleaved columns, the scan's matrices of every round
-// (round r holds Q - 2^r matrices from index boff[r]) and the solve's work
-// vectors.
+// coupled tails); the block-diagonal part's off-diagonals in ELL slabs of
+// kPartEll per row plus an overflow CSR, and its reciprocal diagonal; V = D^{-1} C
+// column-major (n x kPartW); the scan's matrices of every round (round r
+// holds Q - 2^r matrices from index boff[r]); the solve's work vectors.
 struct PartState {
     int m = 0, P = 0, Q = 0, rounds = 0;
-    cap_batch yb = nullptr;
-    double *v0 = nullptr, *v1 = nullptr;
+    int *ell_c = nullptr, *ov_rp = nullptr, *ov_c = nullptr;
+    double *ell_v = nullptr, *ov_v = nullptr, *diag = nullptr;
+    double *vt = nullptr;
     double *bm = nullptr;
     std::vector<long long> boff;
     double *y = nullptr, *a0 = nullptr, *a1 = nullptr;
@@ -244,8 +245,9 @@ void free_partition(cap_plan p)
     PartState *ps = p->part;
     if (!ps) return;
     (void)hipDeviceSynchronize();
-    if (ps->yb) cap_batch_destroy(ps->yb);
-    for (double *b : {ps->v0, ps->v1, ps->bm, ps->y, ps->a0, ps->a1})
+    for (void *b : {(void *)ps->ell_c, (void *)ps->ov_rp, (void *)ps->ov_c, (void *)ps->ell_v, (void *)ps->ov_v,
+                    (void *)ps->diag, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a0,
+                    (void *)ps->a1})
         if (b) (void)hipFree(b);
     delete ps;
     p->part = nullptr;
@@ -417,17 +419,16 @@ int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
     return CAP_OK;
 }
 
-int plan_solve_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st);
-
 // One solve of a partitioned plan (one FP64 right-hand side): y = D^{-1} b
-// with the block batch, the block tails, ceil(log2 Q) scan rounds, then
-// x = y - V t.  Every step is on `st`.
+// (one wavefront per block), the block tails, ceil(log2 Q) scan rounds, then
+// x = y - V t.  Every step is on `st`; none of them waits on another
+// workgroup.
 int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
 {
     PartState *ps = p->part;
     if (!ps) return CAP_ERR_ARG;
-    const cap_plan yp = ps->yb->plan;
-    CAP_TRY(plan_solve_with(yp, yp->algo, yp->sync, b, ps->y, 1, st));
+    k_part_block<<<ps->P, kWave, 0, st>>>(ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b, ps->y,
+                                          p->n, ps->m);
     const int Q = ps->Q;
     const long long mat = (long long)kPartW * kPartW;
     double *cur = ps->a0, *nxt = ps->a1;
@@ -436,7 +437,7 @@ int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
         k_part_scan<<<Q, kPartW, 0, st>>>(cur, nxt, ps->bm + ps->boff[r] * mat, 1 << r);
         std::swap(cur, nxt);
     }
-    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->v0, ps->v1, cur, p->n, ps->m, x);
+    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->vt, cur, p->n, ps->m, x);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }
@@ -680,10 +681,11 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // Tables of the partitioned banded solve (CAP_ALGO_PARTITION), built once:
 // FP64 plans whose lower bandwidth is at most kPartW and that split into at
 // least two blocks; CAP_ERR_UNSUPPORTED otherwise.  Blocks hold
-// m = max(kPartW, ceil(n / kPartBlocks)) rows.  The block-diagonal part
-// becomes a one-RHS batch (its creation times its own kernels); V = D^{-1} C
-// comes from two 64-RHS solves of a second batch over the same blocks,
-// dropped afterwards; the scan's matrices are formed round by round.
+// m = max(kPartW, ceil(n / kPartBlocks)) rows.  The host lays out the
+// block-diagonal part (ELL slabs + overflow CSR + reciprocal diagonal); V = D^{-1} C
+// comes from two 64-RHS solves of a batch over the blocks (dropped
+// afterwards) and is stored column-major; the scan's matrices are formed
+// round by round.
 constexpr int kPartBlocks = 512;
 
 int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
@@ -702,21 +704,39 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     if (w > kPartW) return CAP_ERR_UNSUPPORTED;
     std::vector<double> v((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
-    // the block-diagonal part as a batch: entries of each block, local columns
-    std::vector<int> ro((size_t)P + 1), rpd((size_t)n + 1), cid;
-    std::vector<double> vd;
+    // the block-diagonal part: in-block off-diagonals in stored order, the
+    // first kPartEll of a row in its ELL slab (column -1 pads), the rest in
+    // the overflow CSR; and, for the coupling solves, the same entries as a
+    // batch with local column indices
+    std::vector<int> ell_c((size_t)n * kPartEll, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1),
+        rpd((size_t)n + 1), cid;
+    std::vector<double> ell_v((size_t)n * kPartEll, 0.0), ov_v, dg((size_t)n), vd;
     cid.reserve((size_t)nnz);
     vd.reserve((size_t)nnz);
     for (int q = 0; q <= P; ++q) ro[(size_t)q] = (int)std::min<long long>(n, (long long)q * m);
     for (int i = 0; i < n; ++i) {
-        const int start = (i / m) * m;
+        const int start = (i / m) * m, d = rp_host[i + 1] - 1;
+        ov_rp[(size_t)i] = (int)ov_c.size();
         rpd[(size_t)i] = (int)cid.size();
-        for (int j = rp_host[i]; j < rp_host[i + 1]; ++j)
-            if (ci[j] >= start) {
-                cid.push_back(ci[j] - start);
-                vd.push_back(v[j]);
+        int used = 0;
+        for (int j = rp_host[i]; j < d; ++j) {
+            if (ci[j] < start) continue;
+            if (used < kPartEll) {
+                ell_c[(size_t)i * kPartEll + used] = ci[j];
+                ell_v[(size_t)i * kPartEll + used] = v[j];
+            } else {
+                ov_c.push_back(ci[j]);
+                ov_v.push_back(v[j]);
             }
+            ++used;
+            cid.push_back(ci[j] - start);
+            vd.push_back(v[j]);
+        }
+        dg[(size_t)i] = 1.0 / v[(size_t)d];
+        cid.push_back(i - start);
+        vd.push_back(v[(size_t)d]);
     }
+    ov_rp[(size_t)n] = (int)ov_c.size();
     rpd[(size_t)n] = (int)cid.size();
     PartState *ps = new PartState();
     ps->m = m;
@@ -731,24 +751,38 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     o.host_inputs = 1;
     o.validate = 0;
     o.spin_limit = p->spin_limit;
+    o.readiness = CAP_READY_IN_PLACE;  // plan-time only: keep the uncached pool at the solves' size
     cap_batch b64 = nullptr;
-    double *r0 = nullptr, *r1 = nullptr;
+    double *r0 = nullptr, *r1 = nullptr, *v0 = nullptr, *v1 = nullptr;
     const size_t half = (size_t)n * 64;
     const long long mat = (long long)kPartW * kPartW;
+    const size_t n_ov = std::max<size_t>(1, ov_c.size());
     int rc = CAP_OK;
+    p->part = ps;
     do {
-        if ((rc = cap_batch_create(&ps->yb, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 1, &o, st)))
-            break;
-        o.readiness = CAP_READY_IN_PLACE;  // plan-time only: keep the uncached pool at the solve's size
-        if ((rc = cap_batch_create(&b64, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 64, &o, st)))
-            break;
-        // the block solve reports to this plan's watchdog word
-        ps->yb->plan->err = p->err;
-        if ((rc = dev_alloc(&r0, half)) || (rc = dev_alloc(&r1, half)) || (rc = dev_alloc(&ps->v0, half)) ||
-            (rc = dev_alloc(&ps->v1, half)) || (rc = dev_alloc(&ps->y, (size_t)n)) ||
-            (rc = dev_alloc(&ps->a0, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->a1, (size_t)Q * kPartW)))
+        if ((rc = dev_alloc(&ps->ell_c, ell_c.size())) || (rc = dev_alloc(&ps->ell_v, ell_v.size())) ||
+            (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
+            (rc = dev_alloc(&ps->ov_v, n_ov)) || (rc = dev_alloc(&ps->diag, (size_t)n)) ||
+            (rc = dev_alloc(&ps->y, (size_t)n)) || (rc = dev_alloc(&ps->a0, (size_t)Q * kPartW)) ||
+            (rc = dev_alloc(&ps->a1, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)))
             break;
         if (ps->rounds > 0 && (rc = dev_alloc(&ps->bm, (size_t)(ps->boff[(size_t)ps->rounds] * mat)))) break;
+        if (hipMemcpy(ps->ell_c, ell_c.data(), sizeof(int) * ell_c.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ps->ell_v, ell_v.data(), sizeof(double) * ell_v.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ps->ov_rp, ov_rp.data(), sizeof(int) * ov_rp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            (!ov_c.empty() &&
+             (hipMemcpy(ps->ov_c, ov_c.data(), sizeof(int) * ov_c.size(), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(ps->ov_v, ov_v.data(), sizeof(double) * ov_v.size(), hipMemcpyHostToDevice) != hipSuccess)) ||
+            hipMemcpy(ps->diag, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            break;
+        }
+        // V = D^{-1} C: the coupling columns as two 64-RHS solves of the blocks
+        if ((rc = cap_batch_create(&b64, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 64, &o, st)))
+            break;
+        if ((rc = dev_alloc(&r0, half)) || (rc = dev_alloc(&r1, half)) || (rc = dev_alloc(&v0, half)) ||
+            (rc = dev_alloc(&v1, half)))
+            break;
         if (hipMemsetAsync(r0, 0, half * sizeof(double), st) != hipSuccess ||
             hipMemsetAsync(r1, 0, half * sizeof(double), st) != hipSuccess) {
             rc = CAP_ERR_HIP;
@@ -759,11 +793,11 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             rc = CAP_ERR_HIP;
             break;
         }
-        if ((rc = cap_batch_solve(b64, r0, ps->v0, st)) || (rc = cap_batch_solve(b64, r1, ps->v1, st))) break;
+        if ((rc = cap_batch_solve(b64, r0, v0, st)) || (rc = cap_batch_solve(b64, r1, v1, st))) break;
         if ((rc = cap_plan_status(b64->plan, st)) != CAP_OK) break;
+        k_part_transpose<<<blocks_for((long long)n * kPartW), kBlock, 0, st>>>(v0, v1, n, ps->vt);
         if (ps->rounds > 0) {
-            k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->v0, ps->v1, m, Q,
-                                                                                       ps->bm);
+            k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->vt, n, m, Q, ps->bm);
             for (int r = 0; r + 1 < ps->rounds; ++r) {
                 const int s = 1 << r;
                 if (Q - 2 * s > 0)
@@ -775,9 +809,8 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     } while (0);
     (void)hipStreamSynchronize(st);
     if (b64) cap_batch_destroy(b64);
-    if (r0) (void)hipFree(r0);
-    if (r1) (void)hipFree(r1);
-    p->part = ps;
+    for (double *t : {r0, r1, v0, v1})
+        if (t) (void)hipFree(t);
     if (rc != CAP_OK) free_partition(p);
     return rc;
 }

@@ -15,7 +15,8 @@
         the wide-level matrix: N FP64 solves, then N FP32 + FP64 refined
         solves (fused first two steps) to 1e-12
     python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto]
-        the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels)
+        the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels);
+        --algo partition builds the partitioned solve without the autotuner
 """
 import argparse
 import sys

@@ -5,9 +5,9 @@ The partition reassociates the sums (block solves joined by a scan over the
 error <= 1e-12 against the oracle's sequential forward substitution.  Cases:
 bandwidths 1 (one row per level), 16, 100 and 128 (the limit), block counts
 from 2 up, a ragged last block, duplicated entries, repeated solves (same
-bits each time), several right-hand sides (run by the one-workgroup kernel:
-bit-exact), refinement, and the refusals (bandwidth above 128, FP32, too few
-rows for two blocks).
+bits each time), rows longer than the 64 entries of an ELL slab, several
+right-hand sides (run by the one-workgroup kernel: bit-exact), refinement,
+and the refusals (bandwidth above 128, FP32, too few rows for two blocks).
 """
 import numpy as np
 import pytest
@@ -27,6 +27,20 @@ def _chain(n):
     return rp, ci, np.r_[1.0, np.tile([0.5, 2.0], n - 1)]
 
 
+def _dense_band(n, w, seed):
+    """Every entry within the bandwidth present: rows of up to w off-diagonals
+    (past the 64 a row keeps in its ELL slab)."""
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        cols = np.arange(max(0, i - w), i)
+        off = rng.uniform(-1, 1, size=len(cols))
+        ci += list(cols) + [i]
+        v += list(off) + [1.0 + np.abs(off).sum()]
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
 def _cases():
     yield "deep_band_20k", api.make_config(2, seed=5, scale=0.04)  # bandwidth 128, ~38 nnz/row
     yield "chain_5000", _chain(5000)
@@ -34,6 +48,7 @@ def _cases():
     yield "band100_ragged", api.generate("band_rowcount", n=70001, offdiag=12, band=100, seed=4)
     yield "band128_two_blocks", api.generate("band_rowcount", n=200, offdiag=10, band=128, seed=6)
     yield "dup", random_lower(4000, max_off=6, band=90, seed=7, dup=True)
+    yield "dense_band128", _dense_band(3000, 128, seed=8)
 
 
 CASES = list(_cases())
