@@ -224,12 +224,17 @@ def test_watchdog_forced_from_host(cuda):
 
 
 def test_autotuner_picks_a_correct_path(cuda):
-    # deep band (few rows per level): the one-workgroup kernel is a candidate
+    # deep band (few rows per level): the one-workgroup kernel and, the band
+    # being narrow, the partitioned solve are candidates
     rp, ci, v = api.make_config(2, seed=1, scale=0.01)
     b = api.uniform(9, len(rp) - 1)
     x, info = _solve(cuda, rp, ci, v, b, "auto", "auto")
-    assert info["algo"] in ("wrtfst", "mix", "twopart", "local")
-    assert np.array_equal(x.view(np.uint64), oracle.csr_forward(rp, ci, v, b, fused=True).view(np.uint64))
+    assert info["algo"] in ("wrtfst", "mix", "twopart", "local", "partition")
+    ref = oracle.csr_forward(rp, ci, v, b, fused=True)
+    if info["algo"] == "partition":  # reassociated sums: the FP64 tolerance
+        assert rel_maxnorm(x, ref) <= FP64_TOL
+    else:
+        assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
     # wide levels: it is not (n / levels far above the threshold)
     rp, ci, v = api.make_config(1, seed=1, scale=0.02)
     b = api.uniform(9, len(rp) - 1)
