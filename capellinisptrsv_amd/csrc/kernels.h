@@ -1286,6 +1286,40 @@ __global__ void __launch_bounds__(kBlock)
     block_max_to(m, xmax_bits);
 }
 
+// r = b - L x in FP64 (one right-hand side, fma chain in CSR order) and
+// max|x|: the residual of a refinement whose corrections are FP64 solves.
+__global__ void __launch_bounds__(kBlock)
+    k_residual64(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
+                 const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r, int n,
+                 unsigned long long *__restrict__ xmax_bits)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long m = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double s = 0.0;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[ci[j]], s);
+        r[i] = b[i] - s;
+        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(x[i]));
+        m = a > m ? a : m;
+    }
+    block_max_to(m, xmax_bits);
+}
+
+// x += d (FP64 correction) and max|d| (grid-stride)
+__global__ void __launch_bounds__(kBlock)
+    k_accumulate64(double *__restrict__ x, const double *__restrict__ d, long long count,
+                   unsigned long long *__restrict__ dmax_bits)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long m = 0;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < count; g += stride) {
+        x[g] += d[g];
+        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(d[g]));
+        m = a > m ? a : m;
+    }
+    block_max_to(m, dmax_bits);
+}
+
 // x += d (FP32 correction) and max|d| for the convergence test (grid-stride)
 __global__ void __launch_bounds__(kBlock)
     k_accumulate(double *__restrict__ x, const float *__restrict__ dcor, long long count,

@@ -62,6 +62,7 @@ struct PartState {
     double *bm = nullptr;
     std::vector<long long> boff;
     double *y = nullptr, *a0 = nullptr, *a1 = nullptr;
+    double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction
 };
 
 struct cap_plan_s {
@@ -247,7 +248,7 @@ void free_partition(cap_plan p)
     (void)hipDeviceSynchronize();
     for (void *b : {(void *)ps->ell_c, (void *)ps->ov_rp, (void *)ps->ov_c, (void *)ps->ell_v, (void *)ps->ov_v,
                     (void *)ps->diag, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a0,
-                    (void *)ps->a1})
+                    (void *)ps->a1, (void *)ps->res, (void *)ps->cor})
         if (b) (void)hipFree(b);
     delete ps;
     p->part = nullptr;
@@ -1431,11 +1432,57 @@ int tune_refine_block(cap_plan p, const double *b, double *x, int rhs, hipStream
     return rc;
 }
 
+// Refined solve of a partitioned plan (one right-hand side).  The partition
+// keeps FP64 tables only, so the solves are FP64 partitioned solves: x = P b,
+// then corrections d = P (b - L x), x += d, with the same convergence test as
+// the FP32 refinement (the first correction is normally the last).
+int refined_partition(cap_plan p, const double *b, double *x, double tol, int max_iter, int *iters,
+                      double *last_rel, hipStream_t st)
+{
+    PartState *ps = p->part;
+    if (!ps->res) {
+        CAP_TRY(dev_alloc(&ps->res, (size_t)p->n));
+        CAP_TRY(dev_alloc(&ps->cor, (size_t)p->n));
+    }
+    const unsigned nb = stream_blocks(p, p->n);
+    CAP_TRY(launch_partition(p, b, x, st));
+    double prev_rel = 1.0, rel = 1.0;
+    int it = 1, rc = CAP_OK;
+    while (true) {
+        k_prepare<unsigned long long><<<1, kBlock, 0, st>>>(p->norms, 2, 0ULL, p->ticket, 0);
+        k_residual64<<<nb, kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, b, x, ps->res, p->n, p->norms + 1);
+        CAP_HIP_TRY(hipGetLastError());
+        CAP_TRY(launch_partition(p, ps->res, ps->cor, st));
+        k_accumulate64<<<nb, kBlock, 0, st>>>(x, ps->cor, p->n, p->norms);
+        CAP_HIP_TRY(hipGetLastError());
+        ++it;
+        unsigned long long nbits[2];
+        CAP_HIP_TRY(hipMemcpyAsync(nbits, p->norms, sizeof(nbits), hipMemcpyDeviceToHost, st));
+        CAP_HIP_TRY(hipStreamSynchronize(st));
+        double dmax, xmax;
+        std::memcpy(&dmax, &nbits[0], 8);
+        std::memcpy(&xmax, &nbits[1], 8);
+        prev_rel = it == 2 ? 1.0 : rel;
+        rel = xmax > 0 ? dmax / xmax : dmax;
+        const double est = rel * std::min(1.0, rel / std::max(prev_rel, 1e-300));
+        if (rel <= tol || est <= tol) break;
+        if (it >= max_iter) {
+            rc = CAP_ERR_NOT_CONVERGED;
+            break;
+        }
+    }
+    if (iters) *iters = it;
+    if (last_rel) *last_rel = rel;
+    return rc;
+}
+
 int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, int max_iter, int *iters,
                  double *last_rel, hipStream_t st)
 {
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
     if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
+    if (p->algo == CAP_ALGO_PARTITION && p->part && rhs == 1 && max_iter >= 2)
+        return refined_partition(p, b, x, tol, max_iter, iters, last_rel, st);
     CAP_TRY(ensure_refine(p, rhs, st));
     // the staged readiness array must exist before capture (no allocation
     // inside the captured steps, whose kernels bake in its address); the

@@ -64,7 +64,9 @@ extern "C" {
                               block tails (tables built at plan creation); sums
                               are reassociated (within rounding of the
                               sequential solve, not bit-exact).  Several
-                              right-hand sides run as CAP_ALGO_LOCAL. */
+                              right-hand sides run as CAP_ALGO_LOCAL; the
+                              refined solve uses FP64 partitioned solves
+                              (no FP32 tables) with one FP64 correction. */
 
 /* adjoint method (cap_opts.adjoint) */
 #define CAP_ADJOINT_AUTO 0

@@ -6,7 +6,8 @@ error <= 1e-12 against the oracle's sequential forward substitution.  Cases:
 bandwidths 1 (one row per level), 16, 100 and 128 (the limit), block counts
 from 2 up, a ragged last block, duplicated entries, repeated solves (same
 bits each time), rows longer than the 64 entries of an ELL slab, several
-right-hand sides (run by the one-workgroup kernel: bit-exact), refinement,
+right-hand sides (run by the one-workgroup kernel: bit-exact), refinement
+(FP64 partitioned solves and one correction),
 and the refusals (bandwidth above 128, FP32, too few rows for two blocks).
 """
 import numpy as np
@@ -90,6 +91,8 @@ def test_partition_several_rhs_and_refinement(cuda):
     assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True))  # one-workgroup path
     b1 = api.uniform(33, n)
     xr, iters, rel = plan.solve_refined(torch.from_numpy(b1).to(cuda), tol=1e-12)
+    # FP64 partitioned solves: the first correction is already below 1e-12
+    assert iters == 2 and rel <= 1e-12, (iters, rel)
     assert rel_maxnorm(xr.cpu().numpy(), oracle.csr_forward(rp, ci, v, b1, fused=True)) <= FP64_TOL
     plan.close()
 
