@@ -1437,7 +1437,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // y = D^{-1} b: one wavefront per block, rows in order.  The block's
-// in-block off-diagonal entries are stored ELL-style, kPartEll per row
+// in-block off-diagonal entries are stored ELL-style, ew <= kPartEll per row
 // (column, value; column -1 pads), the rest of a longer row in an overflow
 // CSR (ov_rp per row).  Entry l of a row sits in lane l; the products with
 // the block's earlier values are summed by a DPP tree over the lanes and
@@ -1477,7 +1477,7 @@ __device__ __forceinline__ double wave_sum(double v)
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
-                 const double *__restrict__ b, double *__restrict__ y, int n, int m)
+                 const double *__restrict__ b, double *__restrict__ y, int n, int m, int ew)
 {
     constexpr int kRing = 2 * kPartW;
     __shared__ double ring[kRing];
@@ -1493,8 +1493,8 @@ __global__ void __launch_bounds__(kWave)
     auto fetch = [&](int g0) {
 #pragma unroll
         for (int r = 0; r < kPartPre; ++r) {
-            const long long e = (long long)(g0 + r) * kPartEll + lane;
-            const bool in = g0 + r < end;
+            const long long e = (long long)(g0 + r) * ew + lane;
+            const bool in = g0 + r < end && lane < ew;
             nc[r] = in ? ell_c[e] : -1;
             nv[r] = in ? ell_v[e] : 0.0;
         }

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
 #include <tuple>
 #include <vector>
 
@@ -51,11 +52,11 @@ inline int lpr_log2_for(int k)
 
 // State of a partitioned banded solve: blocks of m rows (P of them, Q = P - 1
 // coupled tails); the block-diagonal part's off-diagonals in ELL slabs of
-// kPartEll per row plus an overflow CSR, and its reciprocal diagonal; V = D^{-1} C
+// ew per row plus an overflow CSR, and its reciprocal diagonal; V = D^{-1} C
 // column-major (n x kPartW); the scan's matrices of every round (round r
 // holds Q - 2^r matrices from index boff[r]); the solve's work vectors.
 struct PartState {
-    int m = 0, P = 0, Q = 0, rounds = 0;
+    int m = 0, P = 0, Q = 0, rounds = 0, ew = 0;  // ew: ELL slab width (<= kPartEll)
     int *ell_c = nullptr, *ov_rp = nullptr, *ov_c = nullptr;
     double *ell_v = nullptr, *ov_v = nullptr, *diag = nullptr;
     double *vt = nullptr;
@@ -429,7 +430,7 @@ int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
     PartState *ps = p->part;
     if (!ps) return CAP_ERR_ARG;
     k_part_block<<<ps->P, kWave, 0, st>>>(ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b, ps->y,
-                                          p->n, ps->m);
+                                          p->n, ps->m, ps->ew);
     const int Q = ps->Q;
     const long long mat = (long long)kPartW * kPartW;
     double *cur = ps->a0, *nxt = ps->a1;
@@ -688,8 +689,24 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // afterwards) and is stored column-major; the scan's matrices are formed
 // round by round.
 constexpr int kPartBlocks = 512;
+// Largest table set a partitioned plan may hold (device; the host builds the
+// same tables once): larger matrices are refused rather than risk exhausting
+// memory behind the caller's back.
+constexpr double kPartMaxBytes = 16.0 * (1ull << 30);
+
+int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStream_t st);
 
 int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
+{
+    try {
+        return build_partition_tables(p, rp_host, st);
+    } catch (const std::bad_alloc &) {
+        free_partition(p);
+        return CAP_ERR_ALLOC;
+    }
+}
+
+int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
     if (p->part) return CAP_OK;
     if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
@@ -703,15 +720,32 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     for (int i = 0; i < n; ++i)
         for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) w = std::max(w, i - ci[j]);
     if (w > kPartW) return CAP_ERR_UNSUPPORTED;
+    // ELL slab width: the longest row's in-block off-diagonals, rounded up to
+    // 8, at most kPartEll (the rest of a longer row goes to the overflow)
+    int longest = 1;
+    for (int i = 0; i < n; ++i) {
+        const int start = (i / m) * m;
+        int cnt = 0;
+        for (int j = rp_host[i]; j < rp_host[i + 1] - 1; ++j) cnt += ci[j] >= start;
+        longest = std::max(longest, cnt);
+    }
+    const int ew = std::min(kPartEll, (longest + 7) / 8 * 8);
+    {
+        int q = P - 1, rounds = 0;
+        while ((1 << rounds) < q) ++rounds;
+        const double mats = (double)rounds * q * kPartW * kPartW * 8.0;
+        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 24.0) + mats + 12.0 * nnz;
+        if (bytes > kPartMaxBytes) return CAP_ERR_UNSUPPORTED;
+    }
     std::vector<double> v((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
     // the block-diagonal part: in-block off-diagonals in stored order, the
     // first kPartEll of a row in its ELL slab (column -1 pads), the rest in
     // the overflow CSR; and, for the coupling solves, the same entries as a
     // batch with local column indices
-    std::vector<int> ell_c((size_t)n * kPartEll, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1),
-        rpd((size_t)n + 1), cid;
-    std::vector<double> ell_v((size_t)n * kPartEll, 0.0), ov_v, dg((size_t)n), vd;
+    std::vector<int> ell_c((size_t)n * ew, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1), rpd((size_t)n + 1),
+        cid;
+    std::vector<double> ell_v((size_t)n * ew, 0.0), ov_v, dg((size_t)n), vd;
     cid.reserve((size_t)nnz);
     vd.reserve((size_t)nnz);
     for (int q = 0; q <= P; ++q) ro[(size_t)q] = (int)std::min<long long>(n, (long long)q * m);
@@ -722,9 +756,9 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         int used = 0;
         for (int j = rp_host[i]; j < d; ++j) {
             if (ci[j] < start) continue;
-            if (used < kPartEll) {
-                ell_c[(size_t)i * kPartEll + used] = ci[j];
-                ell_v[(size_t)i * kPartEll + used] = v[j];
+            if (used < ew) {
+                ell_c[(size_t)i * ew + used] = ci[j];
+                ell_v[(size_t)i * ew + used] = v[j];
             } else {
                 ov_c.push_back(ci[j]);
                 ov_v.push_back(v[j]);
@@ -741,6 +775,7 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     rpd[(size_t)n] = (int)cid.size();
     PartState *ps = new PartState();
     ps->m = m;
+    ps->ew = ew;
     ps->P = P;
     ps->Q = P - 1;
     const int Q = ps->Q;
