@@ -279,7 +279,9 @@ def extra_workloads(args):
     res = {}
 
     # band (n = 100,000, 3 off-diagonals in the previous 1,000 rows, b = L*1) and
-    # deep band (n = 500,000, bandwidth 128, p = 0.3): autotuned plans
+    # deep band (n = 500,000, bandwidth 128, p = 0.3): autotuned plans (the
+    # deep band resolves to the partitioned solve: within rounding, not
+    # bit-exact, so the relative error is reported too)
     for name, cfg in (("band_100k", 0), ("deep_band_500k", 2)):
         log(f"extra workload {name}")
         rp, ci, v = api.make_config(cfg, seed=0)
@@ -291,12 +293,13 @@ def extra_workloads(args):
             b = torch.from_numpy(api.uniform(1, n)).to(dev)
         x = torch.empty_like(b)
         p = Plan(*d)
-        ms = _time_gpu(lambda: p.solve(b, out=x), 20 if cfg == 0 else 5, warmup=2)
+        ms = _time_gpu(lambda: p.solve(b, out=x), 20, warmup=2)
         assert p.status() == 0
-        err = float(np.max(np.abs(x.cpu().numpy() - oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True))))
+        ref = oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True)
+        err = float(np.max(np.abs(x.cpu().numpy() - ref)))
         res[name] = {"n": n, "nnzL": nnz, "levels": api.matrix_layer(n, n, nnz, rp, ci, v)[0], "plan": p.info(),
                      "ms_per_solve": round(ms, 4), "gflops": round(2 * nnz / (ms * 1e6), 3),
-                     "max_abs_err_vs_oracle": err}
+                     "max_abs_err_vs_oracle": err, "rel_err_vs_oracle": err / float(np.max(np.abs(ref)))}
         if not args.no_cpu:
             kind, cpu_ms, reps = _cpu_serial_ms(rp, ci, v, b.cpu().numpy(), seconds=3.0)
             res[name]["cpu_serial"] = {"ms_per_solve": round(cpu_ms, 3), "kind": kind, "cores": 1, "solves": reps,
