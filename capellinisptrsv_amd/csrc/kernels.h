@@ -1381,11 +1381,13 @@ __global__ void __launch_bounds__(kBlock)
 // pm)), the exact solution is x_p = y_p - V_p t_{p-1} with t_p the last
 // kPartW values of x_p:
 //   t_p = a_p + B_p t_{p-1},   a_p = tail(y_p),   B_p = -tail(V_p).
-// The recurrence is an inclusive scan of affine maps, done in ceil(log2 Q)
-// rounds (Q = P - 1 tails): the matrix products are formed once per plan
-// (k_part_gemm); a solve only updates the vectors (k_part_scan).  The sums
-// are reassociated, so results match the sequential solve to rounding, not
-// bit for bit.  Matrices are column-major kPartW x kPartW (element (c, k) at
+// The recurrence over the Q = P - 1 tails is scanned in two levels: in
+// groups of gs consecutive tails sequentially (u_p from the group's first
+// tail on), then over the group carries T_g in ceil(log2 G) rounds, then
+// every tail is fixed up with its in-group product W_p = B_p ... B_first:
+// t_p = u_p + W_p T_{g-1}.  The matrix products are formed once per plan
+// (k_part_gemm); a solve only touches vectors.  The sums are reassociated,
+// so results match the sequential solve to rounding, not bit for bit.  Matrices are column-major kPartW x kPartW (element (c, k) at
 // k * kPartW + c); V is two halves of 64 interleaved columns (n x 64 each).
 constexpr int kPartW = 128;
 
@@ -1545,23 +1547,24 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
-// One round of the plan-time scan: dst_p = src_p * src_{p-s} for p in
-// [2s, q_count).  src holds p in [s, q_count) at index p - s, dst holds p in
-// [2s, q_count) at index p - 2s.  One workgroup per product; each of the 256
-// threads owns an 8 x 8 tile of the result; the inner dimension is staged
-// through LDS 16 at a time.
-__global__ void __launch_bounds__(256)
-    k_part_gemm(const double *__restrict__ src, double *__restrict__ dst, int s, int q_count)
+// Plan-time matrix products C = A * B of the partition's tables, one per
+// workgroup: ops[i] = {C, A, B} as matrix indices into bm (B < 0: C = A, a
+// copy).  Each of the 256 threads owns an 8 x 8 tile of C; the inner
+// dimension is staged through LDS 16 at a time.
+__global__ void __launch_bounds__(256) k_part_gemm(const int3 *__restrict__ ops, double *__restrict__ bm)
 {
     constexpr long long kMat = (long long)kPartW * kPartW;
-    const int p = 2 * s + (int)blockIdx.x;
-    if (p >= q_count) return;
-    const double *A = src + (long long)(p - s) * kMat;
-    const double *B = src + (long long)(p - 2 * s) * kMat;
-    double *C = dst + (long long)(p - 2 * s) * kMat;
+    const int3 op = ops[blockIdx.x];
+    const double *A = bm + (long long)op.y * kMat;
+    double *C = bm + (long long)op.x * kMat;
+    const int t = threadIdx.x;
+    if (op.z < 0) {  // copy (uniform)
+        for (int e = t; e < kPartW * kPartW; e += 256) C[e] = A[e];
+        return;
+    }
+    const double *B = bm + (long long)op.z * kMat;
     __shared__ double sa[16][kPartW];  // A(c, j0 + jj)
     __shared__ double sb[16][kPartW];  // B(j0 + jj, k)
-    const int t = threadIdx.x;
     const int c0 = (t & 15) * 8, k0 = (t >> 4) * 8;
     double acc[8][8];
 #pragma unroll
@@ -1594,6 +1597,58 @@ __global__ void __launch_bounds__(256)
         for (int b = 0; b < 8; ++b) C[(long long)(k0 + b) * kPartW + c0 + a] = acc[a][b];
 }
 
+// y = M v for one kPartW x kPartW column-major M and v in LDS: thread c
+// returns row c (four partial sums).
+__device__ __forceinline__ double part_matvec(const double *__restrict__ M, const double *sv, int c)
+{
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int k = 0; k < kPartW; k += 4)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d[u] = __builtin_fma(M[(long long)(k + u) * kPartW + c], sv[k + u], d[u]);
+    return (d[0] + d[1]) + (d[2] + d[3]);
+}
+
+// Level 1 of the tail scan, one workgroup of kPartW threads per group of
+// `gs` consecutive tails: u_first = a_first, u_p = a_p + B_p u_{p-1} (B_p at
+// index p - 1 of bm).  The group's last u also goes to carry[g].
+__global__ void __launch_bounds__(kPartW)
+    k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
+                      const double *__restrict__ bm, int gs, int q_count)
+{
+    const int g = (int)blockIdx.x, c = (int)threadIdx.x;
+    const int p0 = g * gs, p1 = min(q_count, p0 + gs);
+    __shared__ double sv[kPartW];
+    double v = a[(long long)p0 * kPartW + c];
+    u[(long long)p0 * kPartW + c] = v;
+    for (int p = p0 + 1; p < p1; ++p) {
+        __syncthreads();  // the previous matrix-vector product is done with sv
+        sv[c] = v;
+        __syncthreads();
+        v = a[(long long)p * kPartW + c] + part_matvec(bm + (long long)(p - 1) * kPartW * kPartW, sv, c);
+        u[(long long)p * kPartW + c] = v;
+    }
+    carry[(long long)g * kPartW + c] = v;
+}
+
+// Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
+// p - gs of wm, T the scanned group carries), t_p = u_p in group 0.
+__global__ void __launch_bounds__(kPartW)
+    k_part_fixup(const double *__restrict__ u, const double *__restrict__ tg, const double *__restrict__ wm, int gs,
+                 double *__restrict__ t)
+{
+    const int p = (int)blockIdx.x, c = (int)threadIdx.x;
+    const int g = p / gs;
+    __shared__ double sv[kPartW];
+    double v = u[(long long)p * kPartW + c];
+    if (g > 0) {  // uniform in the workgroup
+        sv[c] = tg[(long long)(g - 1) * kPartW + c];
+        __syncthreads();
+        v += part_matvec(wm + (long long)(p - gs) * kPartW * kPartW, sv, c);
+    }
+    t[(long long)p * kPartW + c] = v;
+}
+
 // a_p = the last kPartW values of y in block p (p < q_count): the scan's input
 __global__ void __launch_bounds__(kBlock)
     k_part_tails(const double *__restrict__ y, int m, int q_count, double *__restrict__ a)
@@ -1604,9 +1659,9 @@ __global__ void __launch_bounds__(kBlock)
     a[g] = y[(long long)(p + 1) * m - kPartW + c];
 }
 
-// One round of the per-solve scan (one workgroup of kPartW threads per p):
-// dst_p = src_p + B_p src_{p-s} for p >= s (B_p at index p - s of the
-// round's matrices), dst_p = src_p below s.
+// One round of the log-depth scan over the group carries (one workgroup of
+// kPartW threads per entry): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
+// index g - s of the round's matrices), dst_g = src_g below s.
 __global__ void __launch_bounds__(kPartW)
     k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s)
 {
@@ -1617,13 +1672,7 @@ __global__ void __launch_bounds__(kPartW)
     if (p >= s) {  // uniform in the workgroup
         sv[c] = src[(long long)(p - s) * kPartW + c];
         __syncthreads();
-        const double *B = bm + (long long)(p - s) * kPartW * kPartW;
-        double d[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-        for (int k = 0; k < kPartW; k += 4)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) d[u] = __builtin_fma(B[(long long)(k + u) * kPartW + c], sv[k + u], d[u]);
-        acc += (d[0] + d[1]) + (d[2] + d[3]);
+        acc += part_matvec(bm + (long long)(p - s) * kPartW * kPartW, sv, c);
     }
     dst[(long long)p * kPartW + c] = acc;
 }

@@ -53,16 +53,19 @@ inline int lpr_log2_for(int k)
 // State of a partitioned banded solve: blocks of m rows (P of them, Q = P - 1
 // coupled tails); the block-diagonal part's off-diagonals in ELL slabs of
 // ew per row plus an overflow CSR, and its reciprocal diagonal; V = D^{-1} C
-// column-major (n x kPartW); the scan's matrices of every round (round r
-// holds Q - 2^r matrices from index boff[r]); the solve's work vectors.
+// column-major (n x kPartW); the tail scan's matrices in bm: B_p at p - 1,
+// the in-group products W_p at w_off + p - gs, and for each round r of the
+// group-carry scan G - 2^r matrices from m_off[r]; the solve's work vectors.
 struct PartState {
-    int m = 0, P = 0, Q = 0, rounds = 0, ew = 0;  // ew: ELL slab width (<= kPartEll)
+    int m = 0, P = 0, Q = 0, ew = 0;  // ew: ELL slab width (<= kPartEll)
+    int gs = 0, G = 0, rounds = 0;    // tails per group, groups, carry-scan rounds
     int *ell_c = nullptr, *ov_rp = nullptr, *ov_c = nullptr;
     double *ell_v = nullptr, *ov_v = nullptr, *diag = nullptr;
     double *vt = nullptr;
     double *bm = nullptr;
-    std::vector<long long> boff;
-    double *y = nullptr, *a0 = nullptr, *a1 = nullptr;
+    long long w_off = 0;
+    std::vector<long long> m_off;
+    double *y = nullptr, *a = nullptr, *u = nullptr, *t = nullptr, *c0 = nullptr, *c1 = nullptr;
     double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction
 };
 
@@ -248,8 +251,8 @@ void free_partition(cap_plan p)
     if (!ps) return;
     (void)hipDeviceSynchronize();
     for (void *b : {(void *)ps->ell_c, (void *)ps->ov_rp, (void *)ps->ov_c, (void *)ps->ell_v, (void *)ps->ov_v,
-                    (void *)ps->diag, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a0,
-                    (void *)ps->a1, (void *)ps->res, (void *)ps->cor})
+                    (void *)ps->diag, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a, (void *)ps->u,
+                    (void *)ps->t, (void *)ps->c0, (void *)ps->c1, (void *)ps->res, (void *)ps->cor})
         if (b) (void)hipFree(b);
     delete ps;
     p->part = nullptr;
@@ -422,9 +425,9 @@ int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
 }
 
 // One solve of a partitioned plan (one FP64 right-hand side): y = D^{-1} b
-// (one wavefront per block), the block tails, ceil(log2 Q) scan rounds, then
-// x = y - V t.  Every step is on `st`; none of them waits on another
-// workgroup.
+// (one wavefront per block), the block tails, the two-level tail scan (in
+// groups, over the group carries, then the fix-up), then x = y - V t.  Every
+// step is on `st`; none of them waits on another workgroup.
 int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
 {
     PartState *ps = p->part;
@@ -433,13 +436,15 @@ int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
                                           p->n, ps->m, ps->ew);
     const int Q = ps->Q;
     const long long mat = (long long)kPartW * kPartW;
-    double *cur = ps->a0, *nxt = ps->a1;
-    k_part_tails<<<blocks_for((long long)Q * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, cur);
+    k_part_tails<<<blocks_for((long long)Q * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, ps->a);
+    k_part_group_scan<<<ps->G, kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q);
+    double *cur = ps->c0, *nxt = ps->c1;
     for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<<<Q, kPartW, 0, st>>>(cur, nxt, ps->bm + ps->boff[r] * mat, 1 << r);
+        k_part_scan<<<ps->G, kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r);
         std::swap(cur, nxt);
     }
-    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->vt, cur, p->n, ps->m, x);
+    k_part_fixup<<<Q, kPartW, 0, st>>>(ps->u, cur, ps->bm + ps->w_off * mat, ps->gs, ps->t);
+    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, x);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }
@@ -688,7 +693,8 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // comes from two 64-RHS solves of a batch over the blocks (dropped
 // afterwards) and is stored column-major; the scan's matrices are formed
 // round by round.
-constexpr int kPartBlocks = 512;
+constexpr int kPartBlocks = 1024;
+constexpr int kPartGroup = 8;  // tails per group of the scan's first level
 // Largest table set a partitioned plan may hold (device; the host builds the
 // same tables once): larger matrices are refused rather than risk exhausting
 // memory behind the caller's back.
@@ -730,11 +736,18 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         longest = std::max(longest, cnt);
     }
     const int ew = std::min(kPartEll, (longest + 7) / 8 * 8);
+    // the tail scan's matrix count: B (Q - 1), W (Q - gs), carry rounds
+    const int Q = P - 1, gs = kPartGroup, G = (Q + gs - 1) / gs;
+    int rounds = 0;
+    while ((1 << rounds) < G) ++rounds;
+    std::vector<long long> m_off((size_t)rounds + 1);
+    const long long w_off = Q - 1;
+    m_off[0] = w_off + std::max(0, Q - gs);
+    for (int r = 0; r < rounds; ++r) m_off[(size_t)r + 1] = m_off[(size_t)r] + (G - (1 << r));
+    const long long n_mat = m_off[(size_t)rounds];
     {
-        int q = P - 1, rounds = 0;
-        while ((1 << rounds) < q) ++rounds;
-        const double mats = (double)rounds * q * kPartW * kPartW * 8.0;
-        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 24.0) + mats + 12.0 * nnz;
+        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 24.0) + (double)n_mat * kPartW * kPartW * 8.0 +
+                             12.0 * nnz;
         if (bytes > kPartMaxBytes) return CAP_ERR_UNSUPPORTED;
     }
     std::vector<double> v((size_t)nnz);
@@ -777,11 +790,49 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     ps->m = m;
     ps->ew = ew;
     ps->P = P;
-    ps->Q = P - 1;
-    const int Q = ps->Q;
-    while ((1 << ps->rounds) < Q) ++ps->rounds;
-    ps->boff.assign((size_t)ps->rounds + 1, 0);
-    for (int r = 0; r < ps->rounds; ++r) ps->boff[(size_t)r + 1] = ps->boff[(size_t)r] + (Q - (1 << r));
+    ps->Q = Q;
+    ps->gs = gs;
+    ps->G = G;
+    ps->rounds = rounds;
+    ps->w_off = w_off;
+    ps->m_off = m_off;
+    // plan-time products, launch by launch (each list depends on the one
+    // before): the in-group products W_p = B_p W_{p-1} (W_first = B_first)
+    // for groups g >= 1, the carries' first-round matrices (the W of each
+    // group's last tail), then each carry round's products
+    std::vector<std::vector<int3>> launches;
+    for (int j = 0; j < gs; ++j) {
+        std::vector<int3> ops;
+        for (int g = 1; g * gs + j < Q; ++g) {
+            const int q = g * gs + j;
+            ops.push_back(j == 0 ? make_int3((int)(w_off + q - gs), q - 1, -1)
+                                 : make_int3((int)(w_off + q - gs), q - 1, (int)(w_off + q - 1 - gs)));
+        }
+        if (!ops.empty()) launches.push_back(ops);
+    }
+    {
+        std::vector<int3> ops;
+        for (int g = 1; g < G; ++g) {
+            const int last = std::min(Q, (g + 1) * gs) - 1;
+            ops.push_back(make_int3((int)(m_off[0] + g - 1), (int)(w_off + last - gs), -1));
+        }
+        if (!ops.empty()) launches.push_back(ops);
+    }
+    for (int r = 0; r + 1 < rounds; ++r) {
+        std::vector<int3> ops;
+        const int s = 1 << r;
+        for (int g = 2 * s; g < G; ++g)
+            ops.push_back(make_int3((int)(m_off[(size_t)r + 1] + g - 2 * s), (int)(m_off[(size_t)r] + g - s),
+                                    (int)(m_off[(size_t)r] + g - 2 * s)));
+        if (!ops.empty()) launches.push_back(ops);
+    }
+    std::vector<int3> all_ops;
+    std::vector<size_t> op_start;
+    for (const auto &l : launches) {
+        op_start.push_back(all_ops.size());
+        all_ops.insert(all_ops.end(), l.begin(), l.end());
+    }
+    int3 *d_ops = nullptr;
     cap_opts o;
     cap_opts_default(&o);
     o.host_inputs = 1;
@@ -799,10 +850,13 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         if ((rc = dev_alloc(&ps->ell_c, ell_c.size())) || (rc = dev_alloc(&ps->ell_v, ell_v.size())) ||
             (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
             (rc = dev_alloc(&ps->ov_v, n_ov)) || (rc = dev_alloc(&ps->diag, (size_t)n)) ||
-            (rc = dev_alloc(&ps->y, (size_t)n)) || (rc = dev_alloc(&ps->a0, (size_t)Q * kPartW)) ||
-            (rc = dev_alloc(&ps->a1, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)))
+            (rc = dev_alloc(&ps->y, (size_t)n)) || (rc = dev_alloc(&ps->a, (size_t)Q * kPartW)) ||
+            (rc = dev_alloc(&ps->u, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->t, (size_t)Q * kPartW)) ||
+            (rc = dev_alloc(&ps->c0, (size_t)G * kPartW)) || (rc = dev_alloc(&ps->c1, (size_t)G * kPartW)) ||
+            (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)))
             break;
-        if (ps->rounds > 0 && (rc = dev_alloc(&ps->bm, (size_t)(ps->boff[(size_t)ps->rounds] * mat)))) break;
+        if (n_mat > 0 && (rc = dev_alloc(&ps->bm, (size_t)(n_mat * mat)))) break;
+        if (!all_ops.empty() && (rc = dev_alloc(&d_ops, all_ops.size()))) break;
         if (hipMemcpy(ps->ell_c, ell_c.data(), sizeof(int) * ell_c.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ps->ell_v, ell_v.data(), sizeof(double) * ell_v.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ps->ov_rp, ov_rp.data(), sizeof(int) * ov_rp.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -832,14 +886,16 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         if ((rc = cap_batch_solve(b64, r0, v0, st)) || (rc = cap_batch_solve(b64, r1, v1, st))) break;
         if ((rc = cap_plan_status(b64->plan, st)) != CAP_OK) break;
         k_part_transpose<<<blocks_for((long long)n * kPartW), kBlock, 0, st>>>(v0, v1, n, ps->vt);
-        if (ps->rounds > 0) {
+        if (Q > 1)
             k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->vt, n, m, Q, ps->bm);
-            for (int r = 0; r + 1 < ps->rounds; ++r) {
-                const int s = 1 << r;
-                if (Q - 2 * s > 0)
-                    k_part_gemm<<<Q - 2 * s, 256, 0, st>>>(ps->bm + ps->boff[(size_t)r] * mat,
-                                                          ps->bm + ps->boff[(size_t)r + 1] * mat, s, Q);
+        if (!all_ops.empty()) {
+            if (hipMemcpy(d_ops, all_ops.data(), sizeof(int3) * all_ops.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                rc = CAP_ERR_HIP;
+                break;
             }
+            for (size_t l = 0; l < launches.size(); ++l)
+                k_part_gemm<<<(unsigned)launches[l].size(), 256, 0, st>>>(d_ops + op_start[l], ps->bm);
         }
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = CAP_ERR_HIP;
     } while (0);
@@ -847,6 +903,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     if (b64) cap_batch_destroy(b64);
     for (double *t : {r0, r1, v0, v1})
         if (t) (void)hipFree(t);
+    if (d_ops) (void)hipFree(d_ops);
     if (rc != CAP_OK) free_partition(p);
     return rc;
 }
