@@ -753,8 +753,8 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     std::vector<double> v((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
     // the block-diagonal part: in-block off-diagonals in stored order, the
-    // first kPartEll of a row in its ELL slab (column -1 pads), the rest in
-    // the overflow CSR; and, for the coupling solves, the same entries as a
+    // first ew of a row in its ELL slab (column -1 pads), the rest in the
+    // overflow CSR; and, for the coupling solves, the same entries as a
     // batch with local column indices
     std::vector<int> ell_c((size_t)n * ew, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1), rpd((size_t)n + 1),
         cid;
@@ -787,6 +787,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     ov_rp[(size_t)n] = (int)ov_c.size();
     rpd[(size_t)n] = (int)cid.size();
     PartState *ps = new PartState();
+    p->part = ps;  // freed by free_partition on every failure from here on
     ps->m = m;
     ps->ew = ew;
     ps->P = P;
@@ -845,7 +846,6 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     const long long mat = (long long)kPartW * kPartW;
     const size_t n_ov = std::max<size_t>(1, ov_c.size());
     int rc = CAP_OK;
-    p->part = ps;
     do {
         if ((rc = dev_alloc(&ps->ell_c, ell_c.size())) || (rc = dev_alloc(&ps->ell_v, ell_v.size())) ||
             (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
