@@ -1290,16 +1290,18 @@ __global__ void __launch_bounds__(kBlock)
 // max|x|: the residual of a refinement whose corrections are FP64 solves.
 __global__ void __launch_bounds__(kBlock)
     k_residual64(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
-                 const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r, int n,
+                 const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r, int n, int k,
                  unsigned long long *__restrict__ xmax_bits)
 {
     const long long stride = (long long)gridDim.x * blockDim.x;
     unsigned long long m = 0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    // g = row * k + RHS: the k threads of a row read its entries together
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < (long long)n * k; g += stride) {
+        const int i = (int)(g / k), q = (int)(g % k);
         double s = 0.0;
-        for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[ci[j]], s);
-        r[i] = b[i] - s;
-        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(x[i]));
+        for (int j = rp[i]; j < rp[i + 1]; ++j) s = __builtin_fma(val[j], x[(long long)ci[j] * k + q], s);
+        r[g] = b[g] - s;
+        const unsigned long long a = (unsigned long long)__double_as_longlong(fabs(x[g]));
         m = a > m ? a : m;
     }
     block_max_to(m, xmax_bits);
@@ -1479,11 +1481,12 @@ __device__ __forceinline__ double wave_sum(double v)
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
-                 const double *__restrict__ b, double *__restrict__ y, int n, int m, int ew)
+                 const double *__restrict__ b, double *__restrict__ y, int n, int m, int ew, int k)
 {
     constexpr int kRing = 2 * kPartW;
     __shared__ double ring[kRing];
     const int lane = threadIdx.x;
+    const int q = (int)blockIdx.y;  // right-hand side (b and y interleaved, k per row)
     const int start = blockIdx.x * m;
     const int end = min(n, start + m);
     // a group's row data: entry `lane` of each of its kPartPre rows, and, in
@@ -1501,7 +1504,7 @@ __global__ void __launch_bounds__(kWave)
             nv[r] = in ? ell_v[e] : 0.0;
         }
         const int row = min(g0 + lane, end);
-        nb = lane < kPartPre && row < end ? b[row] : 0.0;
+        nb = lane < kPartPre && row < end ? b[(long long)row * k + q] : 0.0;
         nd = lane < kPartPre && row < end ? inv_diag[row] : 1.0;
         no = lane <= kPartPre ? ov_rp[row] : 0;
     };
@@ -1540,7 +1543,7 @@ __global__ void __launch_bounds__(kWave)
             const double yi = (bcast(bg, r) - wave_sum(prod)) * bcast(dg, r);
             if (lane == 0) {
                 ring[(i - start) & (kRing - 1)] = yi;
-                y[i] = yi;
+                y[(long long)i * k + q] = yi;
             }
             yprev = yi;
         }
@@ -1609,94 +1612,104 @@ __device__ __forceinline__ double part_matvec(const double *__restrict__ M, cons
     return (d[0] + d[1]) + (d[2] + d[3]);
 }
 
+// The scan's vectors hold k right-hand sides: tail p of RHS q at (p * k + q)
+// * kPartW; blockIdx.y is the RHS.
+__device__ __forceinline__ long long part_vec(int p, int k) { return ((long long)p * k + blockIdx.y) * kPartW; }
+
 // Level 1 of the tail scan, one workgroup of kPartW threads per group of
-// `gs` consecutive tails: u_first = a_first, u_p = a_p + B_p u_{p-1} (B_p at
-// index p - 1 of bm).  The group's last u also goes to carry[g].
+// `gs` consecutive tails and RHS: u_first = a_first, u_p = a_p + B_p u_{p-1}
+// (B_p at index p - 1 of bm).  The group's last u also goes to carry[g].
 __global__ void __launch_bounds__(kPartW)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
-                      const double *__restrict__ bm, int gs, int q_count)
+                      const double *__restrict__ bm, int gs, int q_count, int k)
 {
     const int g = (int)blockIdx.x, c = (int)threadIdx.x;
     const int p0 = g * gs, p1 = min(q_count, p0 + gs);
     __shared__ double sv[kPartW];
-    double v = a[(long long)p0 * kPartW + c];
-    u[(long long)p0 * kPartW + c] = v;
+    double v = a[part_vec(p0, k) + c];
+    u[part_vec(p0, k) + c] = v;
     for (int p = p0 + 1; p < p1; ++p) {
         __syncthreads();  // the previous matrix-vector product is done with sv
         sv[c] = v;
         __syncthreads();
-        v = a[(long long)p * kPartW + c] + part_matvec(bm + (long long)(p - 1) * kPartW * kPartW, sv, c);
-        u[(long long)p * kPartW + c] = v;
+        v = a[part_vec(p, k) + c] + part_matvec(bm + (long long)(p - 1) * kPartW * kPartW, sv, c);
+        u[part_vec(p, k) + c] = v;
     }
-    carry[(long long)g * kPartW + c] = v;
+    carry[part_vec(g, k) + c] = v;
 }
 
 // Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
 // p - gs of wm, T the scanned group carries), t_p = u_p in group 0.
 __global__ void __launch_bounds__(kPartW)
     k_part_fixup(const double *__restrict__ u, const double *__restrict__ tg, const double *__restrict__ wm, int gs,
-                 double *__restrict__ t)
+                 double *__restrict__ t, int k)
 {
     const int p = (int)blockIdx.x, c = (int)threadIdx.x;
     const int g = p / gs;
     __shared__ double sv[kPartW];
-    double v = u[(long long)p * kPartW + c];
+    double v = u[part_vec(p, k) + c];
     if (g > 0) {  // uniform in the workgroup
-        sv[c] = tg[(long long)(g - 1) * kPartW + c];
+        sv[c] = tg[part_vec(g - 1, k) + c];
         __syncthreads();
         v += part_matvec(wm + (long long)(p - gs) * kPartW * kPartW, sv, c);
     }
-    t[(long long)p * kPartW + c] = v;
+    t[part_vec(p, k) + c] = v;
 }
 
-// a_p = the last kPartW values of y in block p (p < q_count): the scan's input
+// a_p = the last kPartW values of y in block p (p < q_count), per RHS: the
+// scan's input, (p * k + q) * kPartW + c
 __global__ void __launch_bounds__(kBlock)
-    k_part_tails(const double *__restrict__ y, int m, int q_count, double *__restrict__ a)
+    k_part_tails(const double *__restrict__ y, int m, int q_count, int k, double *__restrict__ a)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)q_count * kPartW) return;
-    const int p = (int)(g / kPartW), c = (int)(g % kPartW);
-    a[g] = y[(long long)(p + 1) * m - kPartW + c];
+    if (g >= (long long)q_count * k * kPartW) return;
+    const int c = (int)(g % kPartW);
+    const long long pq = g / kPartW;
+    const int p = (int)(pq / k), q = (int)(pq % k);
+    a[g] = y[((long long)(p + 1) * m - kPartW + c) * k + q];
 }
 
 // One round of the log-depth scan over the group carries (one workgroup of
 // kPartW threads per entry): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
 // index g - s of the round's matrices), dst_g = src_g below s.
 __global__ void __launch_bounds__(kPartW)
-    k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s)
+    k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s, int k)
 {
     const int p = (int)blockIdx.x;
     const int c = (int)threadIdx.x;
     __shared__ double sv[kPartW];
-    double acc = src[(long long)p * kPartW + c];
+    double acc = src[part_vec(p, k) + c];
     if (p >= s) {  // uniform in the workgroup
-        sv[c] = src[(long long)(p - s) * kPartW + c];
+        sv[c] = src[part_vec(p - s, k) + c];
         __syncthreads();
         acc += part_matvec(bm + (long long)(p - s) * kPartW * kPartW, sv, c);
     }
-    dst[(long long)p * kPartW + c] = acc;
+    dst[part_vec(p, k) + c] = acc;
 }
 
 // x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in
-// block 0.  V column-major: for each column the lanes read consecutive rows.
+// block 0.  One thread per (row, RHS), g = row * k + q: V column-major, so
+// for each column the lanes read consecutive rows (the k lanes of one row
+// share an element).
 __global__ void __launch_bounds__(kBlock)
     k_part_finish(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t, int n,
-                  int m, double *__restrict__ x)
+                  int m, int k, double *__restrict__ x)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= (long long)n * k) return;
+    const int i = k == 1 ? (int)gi : (int)(gi / k), q = k == 1 ? 0 : (int)(gi % k);
     const int p = i / m;
-    double xi = y[i];
+    double xi = y[gi];
     if (p > 0) {
-        const double *tp = t + (long long)(p - 1) * kPartW;
+        const double *tp = t + ((long long)(p - 1) * k + q) * kPartW;
         double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
-        for (int k = 0; k < kPartW; k += 4)
+        for (int c = 0; c < kPartW; c += 4)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = __builtin_fma(vt[(long long)(k + u) * n + i], tp[k + u], s[u]);
+            for (int u = 0; u < 4; ++u) s[u] = __builtin_fma(vt[(long long)(c + u) * n + i], tp[c + u], s[u]);
         xi -= (s[0] + s[1]) + (s[2] + s[3]);
     }
-    x[i] = xi;
+    x[gi] = xi;
 }
 
 }  // namespace cap

@@ -65,8 +65,10 @@ struct PartState {
     double *bm = nullptr;
     long long w_off = 0;
     std::vector<long long> m_off;
+    // work vectors, sized for kcap right-hand sides (grown by a solve with more)
+    int kcap = 0;
     double *y = nullptr, *a = nullptr, *u = nullptr, *t = nullptr, *c0 = nullptr, *c1 = nullptr;
-    double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction
+    double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction (n * kcap)
 };
 
 struct cap_plan_s {
@@ -328,8 +330,8 @@ int launch_prepare(cap_plan p, int algo, int sync, void *x, long long count, hip
 {
     typedef typename VBits<T>::U U;
     if (algo == CAP_ALGO_PARTITION) {
-        // the block solve resets itself; several right-hand sides go to LOCAL
-        if (count == p->n && sizeof(T) == 8) return CAP_OK;
+        // the partitioned solve keeps no readiness state (its tables are FP64)
+        if (sizeof(T) == 8) return CAP_OK;
         algo = CAP_ALGO_LOCAL;
     }
     if (algo == CAP_ALGO_LOCAL) sync = CAP_SYNC_VALUE;
@@ -424,27 +426,55 @@ int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
     return CAP_OK;
 }
 
-// One solve of a partitioned plan (one FP64 right-hand side): y = D^{-1} b
-// (one wavefront per block), the block tails, the two-level tail scan (in
-// groups, over the group carries, then the fix-up), then x = y - V t.  Every
-// step is on `st`; none of them waits on another workgroup.
-int launch_partition(cap_plan p, const double *b, double *x, hipStream_t st)
+// The partitioned solve's work vectors for k right-hand sides.  Growing them
+// waits for the device (earlier solves may still use the old ones), so a
+// caller that captures solves into a graph runs one uncaptured solve with its
+// largest k first.
+int ensure_part_rhs(cap_plan p, int k)
 {
     PartState *ps = p->part;
-    if (!ps) return CAP_ERR_ARG;
-    k_part_block<<<ps->P, kWave, 0, st>>>(ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b, ps->y,
-                                          p->n, ps->m, ps->ew);
+    if (k <= ps->kcap) return CAP_OK;
+    CAP_HIP_TRY(hipDeviceSynchronize());
+    for (double **v : {&ps->y, &ps->a, &ps->u, &ps->t, &ps->c0, &ps->c1, &ps->res, &ps->cor}) {
+        if (*v) (void)hipFree(*v);
+        *v = nullptr;  // res / cor come back at the new size when a refined solve needs them
+    }
+    ps->kcap = 0;
+    const size_t tails = (size_t)ps->Q * k * kPartW, carries = (size_t)ps->G * k * kPartW;
+    CAP_TRY(dev_alloc(&ps->y, (size_t)p->n * k));
+    CAP_TRY(dev_alloc(&ps->a, tails));
+    CAP_TRY(dev_alloc(&ps->u, tails));
+    CAP_TRY(dev_alloc(&ps->t, tails));
+    CAP_TRY(dev_alloc(&ps->c0, carries));
+    CAP_TRY(dev_alloc(&ps->c1, carries));
+    ps->kcap = k;
+    return CAP_OK;
+}
+
+// One solve of a partitioned plan (k FP64 right-hand sides, interleaved:
+// b[row * k + q]): y = D^{-1} b (one wavefront per block and RHS), the block
+// tails, the two-level tail scan (in groups, over the group carries, then the
+// fix-up; grids over tails x RHS), then x = y - V t.  Every step is on `st`;
+// none of them waits on another workgroup.
+int launch_partition(cap_plan p, const double *b, double *x, int k, hipStream_t st)
+{
+    PartState *ps = p->part;
+    if (!ps || k < 1 || k > kWave) return CAP_ERR_ARG;
+    CAP_TRY(ensure_part_rhs(p, k));
+    const unsigned ku = (unsigned)k;
+    k_part_block<<<dim3(ps->P, ku), kWave, 0, st>>>(ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b,
+                                                   ps->y, p->n, ps->m, ps->ew, k);
     const int Q = ps->Q;
     const long long mat = (long long)kPartW * kPartW;
-    k_part_tails<<<blocks_for((long long)Q * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, ps->a);
-    k_part_group_scan<<<ps->G, kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q);
+    k_part_tails<<<blocks_for((long long)Q * k * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, k, ps->a);
+    k_part_group_scan<<<dim3(ps->G, ku), kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q, k);
     double *cur = ps->c0, *nxt = ps->c1;
     for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<<<ps->G, kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r);
+        k_part_scan<<<dim3(ps->G, ku), kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r, k);
         std::swap(cur, nxt);
     }
-    k_part_fixup<<<Q, kPartW, 0, st>>>(ps->u, cur, ps->bm + ps->w_off * mat, ps->gs, ps->t);
-    k_part_finish<<<blocks_for(p->n), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, x);
+    k_part_fixup<<<dim3(Q, ku), kPartW, 0, st>>>(ps->u, cur, ps->bm + ps->w_off * mat, ps->gs, ps->t, k);
+    k_part_finish<<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }
@@ -455,10 +485,8 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     const int n = p->n;
     if (n == 0) return CAP_OK;
     if (algo == CAP_ALGO_PARTITION) {
-        if constexpr (sizeof(T) == 8) {
-            if (rhs == 1) return launch_partition(p, b, x, st);
-        }
-        algo = CAP_ALGO_LOCAL;  // the partition's tables are built for one FP64 right-hand side
+        if constexpr (sizeof(T) == 8) return launch_partition(p, b, x, rhs, st);
+        algo = CAP_ALGO_LOCAL;  // the partition's tables are FP64
     }
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
@@ -850,10 +878,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         if ((rc = dev_alloc(&ps->ell_c, ell_c.size())) || (rc = dev_alloc(&ps->ell_v, ell_v.size())) ||
             (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
             (rc = dev_alloc(&ps->ov_v, n_ov)) || (rc = dev_alloc(&ps->diag, (size_t)n)) ||
-            (rc = dev_alloc(&ps->y, (size_t)n)) || (rc = dev_alloc(&ps->a, (size_t)Q * kPartW)) ||
-            (rc = dev_alloc(&ps->u, (size_t)Q * kPartW)) || (rc = dev_alloc(&ps->t, (size_t)Q * kPartW)) ||
-            (rc = dev_alloc(&ps->c0, (size_t)G * kPartW)) || (rc = dev_alloc(&ps->c1, (size_t)G * kPartW)) ||
-            (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)))
+            (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)) || (rc = ensure_part_rhs(p, 1)))
             break;
         if (n_mat > 0 && (rc = dev_alloc(&ps->bm, (size_t)(n_mat * mat)))) break;
         if (!all_ops.empty() && (rc = dev_alloc(&d_ops, all_ops.size()))) break;
@@ -1524,28 +1549,31 @@ int tune_refine_block(cap_plan p, const double *b, double *x, int rhs, hipStream
     return rc;
 }
 
-// Refined solve of a partitioned plan (one right-hand side).  The partition
+// Refined solve of a partitioned plan (k right-hand sides).  The partition
 // keeps FP64 tables only, so the solves are FP64 partitioned solves: x = P b,
 // then corrections d = P (b - L x), x += d, with the same convergence test as
-// the FP32 refinement (the first correction is normally the last).
-int refined_partition(cap_plan p, const double *b, double *x, double tol, int max_iter, int *iters,
+// the FP32 refinement, over all right-hand sides (the first correction is
+// normally the last).
+int refined_partition(cap_plan p, const double *b, double *x, int k, double tol, int max_iter, int *iters,
                       double *last_rel, hipStream_t st)
 {
     PartState *ps = p->part;
+    const long long count = (long long)p->n * k;
+    CAP_TRY(launch_partition(p, b, x, k, st));  // sizes the work vectors for k
     if (!ps->res) {
-        CAP_TRY(dev_alloc(&ps->res, (size_t)p->n));
-        CAP_TRY(dev_alloc(&ps->cor, (size_t)p->n));
+        CAP_TRY(dev_alloc(&ps->res, (size_t)p->n * ps->kcap));
+        CAP_TRY(dev_alloc(&ps->cor, (size_t)p->n * ps->kcap));
     }
-    const unsigned nb = stream_blocks(p, p->n);
-    CAP_TRY(launch_partition(p, b, x, st));
+    const unsigned nb = stream_blocks(p, count);
     double prev_rel = 1.0, rel = 1.0;
     int it = 1, rc = CAP_OK;
     while (true) {
         k_prepare<unsigned long long><<<1, kBlock, 0, st>>>(p->norms, 2, 0ULL, p->ticket, 0);
-        k_residual64<<<nb, kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, b, x, ps->res, p->n, p->norms + 1);
+        k_residual64<<<nb, kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, b, x, ps->res, p->n, k,
+                                            p->norms + 1);
         CAP_HIP_TRY(hipGetLastError());
-        CAP_TRY(launch_partition(p, ps->res, ps->cor, st));
-        k_accumulate64<<<nb, kBlock, 0, st>>>(x, ps->cor, p->n, p->norms);
+        CAP_TRY(launch_partition(p, ps->res, ps->cor, k, st));
+        k_accumulate64<<<nb, kBlock, 0, st>>>(x, ps->cor, count, p->norms);
         CAP_HIP_TRY(hipGetLastError());
         ++it;
         unsigned long long nbits[2];
@@ -1573,8 +1601,8 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
 {
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
     if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
-    if (p->algo == CAP_ALGO_PARTITION && p->part && rhs == 1 && max_iter >= 2)
-        return refined_partition(p, b, x, tol, max_iter, iters, last_rel, st);
+    if (p->algo == CAP_ALGO_PARTITION && p->part && max_iter >= 2)
+        return refined_partition(p, b, x, rhs, tol, max_iter, iters, last_rel, st);
     CAP_TRY(ensure_refine(p, rhs, st));
     // the staged readiness array must exist before capture (no allocation
     // inside the captured steps, whose kernels bake in its address); the

@@ -63,10 +63,12 @@ extern "C" {
                               independently, joined by a scan over the 128-value
                               block tails (tables built at plan creation); sums
                               are reassociated (within rounding of the
-                              sequential solve, not bit-exact).  Several
-                              right-hand sides run as CAP_ALGO_LOCAL; the
-                              refined solve uses FP64 partitioned solves
-                              (no FP32 tables) with one FP64 correction. */
+                              sequential solve, not bit-exact).  Up to 64
+                              right-hand sides per solve (its work vectors
+                              grow on the first solve with a larger count,
+                              which waits for the device); the refined solve
+                              uses FP64 partitioned solves (no FP32 tables)
+                              with one FP64 correction. */
 
 /* adjoint method (cap_opts.adjoint) */
 #define CAP_ADJOINT_AUTO 0

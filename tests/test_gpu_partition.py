@@ -6,9 +6,11 @@ error <= 1e-12 against the oracle's sequential forward substitution.  Cases:
 bandwidths 1 (one row per level), 16, 100 and 128 (the limit), block counts
 from 2 up, a ragged last block, duplicated entries, repeated solves (same
 bits each time), rows longer than the 64 entries of an ELL slab, several
-right-hand sides (run by the one-workgroup kernel: bit-exact), refinement
-(FP64 partitioned solves and one correction),
-and the refusals (bandwidth above 128, FP32, too few rows for two blocks).
+right-hand sides (1 to 64, interleaved; each column against its own
+sequential solve, and the work vectors growing and shrinking between
+solves), refinement (FP64 partitioned solves and one correction, one and
+several right-hand sides), and the refusals (bandwidth above 128, FP32, too
+few rows for two blocks).
 """
 import numpy as np
 import pytest
@@ -79,21 +81,48 @@ def test_partition_within_fp64_tolerance(cuda, name, mat):
     plan.close()
 
 
-def test_partition_several_rhs_and_refinement(cuda):
+def _columns_within_tol(x, ref, k):
+    x, ref = x.reshape(-1, k), ref.reshape(-1, k)
+    errs = [rel_maxnorm(x[:, q], ref[:, q]) for q in range(k)]
+    assert max(errs) <= FP64_TOL, errs
+
+
+def test_partition_several_rhs(cuda):
+    """k = 16, 64, then 2 and 1 again (the work vectors grow, then serve the
+    smaller k as they are), then 4; one RHS equal to another must give the
+    same bits in both columns."""
     import torch
     rp, ci, v = api.make_config(2, seed=8, scale=0.02)
     n = len(rp) - 1
     plan = _plan(cuda, rp, ci, v, algo="partition")
-    k = 4
-    b = api.uniform(32, n * k)
-    x = plan.solve(torch.from_numpy(b.reshape(n, k)).to(cuda)).cpu().numpy().reshape(-1)
-    assert plan.status() == 0
-    assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True))  # one-workgroup path
-    b1 = api.uniform(33, n)
-    xr, iters, rel = plan.solve_refined(torch.from_numpy(b1).to(cuda), tol=1e-12)
+    for seed, k in ((40, 16), (41, 64), (42, 2), (43, 1), (44, 4)):
+        b = api.uniform(seed, n * k).reshape(n, k)
+        if k > 1:
+            b[:, k - 1] = b[:, 0]
+        x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy().reshape(n, k)
+        assert plan.status() == 0
+        _columns_within_tol(x, oracle.csr_forward(rp, ci, v, b.reshape(-1), rhs=k, fused=True), k)
+        assert np.array_equal(x[:, k - 1], x[:, 0]), k
+        if k == 1:
+            # the k = 1 solve through work vectors sized for 64 gives the bits of a fresh plan
+            fresh = _plan(cuda, rp, ci, v, algo="partition")
+            assert np.array_equal(x[:, 0], fresh.solve(torch.from_numpy(b[:, 0].copy()).to(cuda)).cpu().numpy())
+            fresh.close()
+    plan.close()
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_partition_refinement(cuda, k):
+    import torch
+    rp, ci, v = api.make_config(2, seed=8, scale=0.02)
+    n = len(rp) - 1
+    plan = _plan(cuda, rp, ci, v, algo="partition")
+    b = api.uniform(33, n * k)
+    bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
+    xr, iters, rel = plan.solve_refined(bt, tol=1e-12)
     # FP64 partitioned solves: the first correction is already below 1e-12
     assert iters == 2 and rel <= 1e-12, (iters, rel)
-    assert rel_maxnorm(xr.cpu().numpy(), oracle.csr_forward(rp, ci, v, b1, fused=True)) <= FP64_TOL
+    _columns_within_tol(xr.cpu().numpy().reshape(-1), oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True), k)
     plan.close()
 
 
