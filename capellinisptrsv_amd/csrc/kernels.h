@@ -1478,22 +1478,24 @@ __device__ __forceinline__ double wave_sum(double v)
     return bcast(v, kWave - 1);
 }
 
+template <int KR>
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
                  const double *__restrict__ b, double *__restrict__ y, int n, int m, int ew, int k)
 {
     constexpr int kRing = 2 * kPartW;
-    __shared__ double ring[kRing];
+    __shared__ double ring[KR][kRing];
     const int lane = threadIdx.x;
-    const int q = (int)blockIdx.y;  // right-hand side (b and y interleaved, k per row)
+    // right-hand sides q0 .. q0 + KR - 1 (b and y interleaved, k per row)
+    const long long q0 = (long long)blockIdx.y * KR;
     const int start = blockIdx.x * m;
     const int end = min(n, start + m);
     // a group's row data: entry `lane` of each of its kPartPre rows, and, in
-    // lane r, row r's b, reciprocal diagonal and overflow start (lane
+    // lane r, row r's b values, reciprocal diagonal and overflow start (lane
     // kPartPre: the end)
     int nc[kPartPre];
-    double nv[kPartPre], nb = 0.0, nd = 1.0;
+    double nv[kPartPre], nb[KR], nd = 1.0;
     int no = 0;
     auto fetch = [&](int g0) {
 #pragma unroll
@@ -1504,50 +1506,81 @@ __global__ void __launch_bounds__(kWave)
             nv[r] = in ? ell_v[e] : 0.0;
         }
         const int row = min(g0 + lane, end);
-        nb = lane < kPartPre && row < end ? b[(long long)row * k + q] : 0.0;
+#pragma unroll
+        for (int j = 0; j < KR; ++j) nb[j] = lane < kPartPre && row < end ? b[(long long)row * k + q0 + j] : 0.0;
         nd = lane < kPartPre && row < end ? inv_diag[row] : 1.0;
         no = lane <= kPartPre ? ov_rp[row] : 0;
     };
-    // Every lane computes each row's value.  The ring (rows' values by row
-    // mod kRing) is read one row ahead: row i+1's entry is fetched while row
-    // i is summed (its column is below i, so already written -- one
-    // wavefront's LDS accesses complete in order); an entry in column i
-    // itself takes the value just computed.
-    double yprev = 0.0, rv = 0.0;
+    // Every lane computes each row's values.  Inside a group the ring (rows'
+    // values by row mod kRing) is read one row ahead: row i+1's entry is
+    // fetched while row i is summed (its column is below i, so already
+    // written -- one wavefront's LDS accesses complete in order); an entry in
+    // column i itself takes the value just computed.  A group's first row
+    // reads its entry at the group's top, before the next prefetch is
+    // issued: reading it ahead from the prefetch would make the previous
+    // row wait for every load of that prefetch.  The KR right-hand sides
+    // share the row data; their sums are independent chains.  The values go
+    // to y from the ring, a group at a time (one store of kPartPre x KR
+    // values at the next group's top): a store per row would make the waits
+    // for the prefetch (the memory counter covers stores too) wait for it.
+    static_assert(kPartPre * KR <= kWave, "a group's values are stored by one instruction");
+    const int wr = lane / KR, wj = lane % KR;  // the (row, RHS) this lane stores
+    auto store_group = [&](int s0) {
+        const int i = s0 + wr;
+        if (lane < kPartPre * KR && i < end) y[(long long)i * k + q0 + wj] = ring[wj][(i - start) & (kRing - 1)];
+    };
+    double yprev[KR], rv[KR];
+#pragma unroll
+    for (int j = 0; j < KR; ++j) yprev[j] = 0.0;
     fetch(start);
     for (int g0 = start; g0 < end; g0 += kPartPre) {
-        int cc[kPartPre + 1];
-        double cv[kPartPre];
+        int cc[kPartPre];
+        double cv[kPartPre], bg[KR];
 #pragma unroll
         for (int r = 0; r < kPartPre; ++r) {
             cc[r] = nc[r];
             cv[r] = nv[r];
         }
-        const double bg = nb, dg = nd;
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            bg[j] = nb[j];
+            rv[j] = cc[0] >= 0 && cc[0] < g0 ? ring[j][(cc[0] - start) & (kRing - 1)] : 0.0;
+        }
+        const double dg = nd;
         const int og = no;
         if (g0 + kPartPre < end) fetch(g0 + kPartPre);
-        cc[kPartPre] = g0 + kPartPre < end ? nc[0] : -1;  // the next group's first row
+        if (g0 > start) store_group(g0 - kPartPre);
 #pragma unroll
         for (int r = 0; r < kPartPre; ++r) {
             const int i = g0 + r;
             if (i >= end) continue;  // uniform: the block's last group
-            double prod = 0.0;
-            if (cc[r] >= 0) prod = cv[r] * (cc[r] == i - 1 ? yprev : rv);
+            double prod[KR];
+#pragma unroll
+            for (int j = 0; j < KR; ++j) prod[j] = cc[r] >= 0 ? cv[r] * (cc[r] == i - 1 ? yprev[j] : rv[j]) : 0.0;
             const int o0 = __builtin_amdgcn_readlane(og, r), o1 = __builtin_amdgcn_readlane(og, r + 1);
-            for (int j = o0 + lane; j < o1; j += kWave) {  // rows longer than kPartEll
-                const int c = ov_c[j];
-                prod = __builtin_fma(ov_v[j], c == i - 1 ? yprev : ring[(c - start) & (kRing - 1)], prod);
+            for (int e = o0 + lane; e < o1; e += kWave) {  // rows longer than kPartEll
+                const int c = ov_c[e];
+                const double w = ov_v[e];
+#pragma unroll
+                for (int j = 0; j < KR; ++j)
+                    prod[j] = __builtin_fma(w, c == i - 1 ? yprev[j] : ring[j][(c - start) & (kRing - 1)], prod[j]);
             }
-            const int cn = cc[r + 1];
-            rv = cn >= 0 && cn < i ? ring[(cn - start) & (kRing - 1)] : 0.0;
-            const double yi = (bcast(bg, r) - wave_sum(prod)) * bcast(dg, r);
-            if (lane == 0) {
-                ring[(i - start) & (kRing - 1)] = yi;
-                y[(long long)i * k + q] = yi;
+            const int cn = r + 1 < kPartPre ? cc[r + 1] : -1;  // the next row's column (in this group)
+            const bool ahead = cn >= 0 && cn < i;
+#pragma unroll
+            for (int j = 0; j < KR; ++j) rv[j] = ahead ? ring[j][(cn - start) & (kRing - 1)] : 0.0;
+            const double di = bcast(dg, r);
+            double mine = 0.0;  // lane j keeps right-hand side j's value
+#pragma unroll
+            for (int j = 0; j < KR; ++j) {
+                const double yi = (bcast(bg[j], r) - wave_sum(prod[j])) * di;
+                yprev[j] = yi;
+                if (lane == j) mine = yi;
             }
-            yprev = yi;
+            if (lane < KR) ring[lane][(i - start) & (kRing - 1)] = mine;
         }
     }
+    store_group(start + (end - 1 - start) / kPartPre * kPartPre);  // the last group
 }
 
 // Plan-time matrix products C = A * B of the partition's tables, one per
@@ -1600,60 +1633,102 @@ __global__ void __launch_bounds__(256) k_part_gemm(const int3 *__restrict__ ops,
         for (int b = 0; b < 8; ++b) C[(long long)(k0 + b) * kPartW + c0 + a] = acc[a][b];
 }
 
-// y = M v for one kPartW x kPartW column-major M and v in LDS: thread c
-// returns row c (four partial sums).
-__device__ __forceinline__ double part_matvec(const double *__restrict__ M, const double *sv, int c)
+// y_j = M v_j for one kPartW x kPartW column-major M and KQ vectors v_j in
+// LDS: thread c returns row c of each (four partial sums, by index mod 4, in
+// the same order for every KQ, so a right-hand side gets the same bits
+// whichever chunk it is solved in).  M is read once for all KQ vectors.
+template <int KQ>
+__device__ __forceinline__ void part_matvec(const double *__restrict__ M, const double (*sv)[kPartW], int c,
+                                            double (&out)[KQ])
 {
-    double d[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-    for (int k = 0; k < kPartW; k += 4)
+    double d[KQ][4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) d[u] = __builtin_fma(M[(long long)(k + u) * kPartW + c], sv[k + u], d[u]);
-    return (d[0] + d[1]) + (d[2] + d[3]);
+    for (int j = 0; j < KQ; ++j)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d[j][u] = 0.0;
+    constexpr int kUnroll = KQ >= 8 ? 2 : 8;
+#pragma unroll kUnroll
+    for (int kk = 0; kk < kPartW; kk += 4) {
+        double mv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mv[u] = M[(long long)(kk + u) * kPartW + c];
+#pragma unroll
+        for (int j = 0; j < KQ; ++j)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[j][u] = __builtin_fma(mv[u], sv[j][kk + u], d[j][u]);
+    }
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) out[j] = (d[j][0] + d[j][1]) + (d[j][2] + d[j][3]);
 }
 
 // The scan's vectors hold k right-hand sides: tail p of RHS q at (p * k + q)
-// * kPartW; blockIdx.y is the RHS.
-__device__ __forceinline__ long long part_vec(int p, int k) { return ((long long)p * k + blockIdx.y) * kPartW; }
+// * kPartW.  A workgroup takes the KQ right-hand sides from blockIdx.y * KQ;
+// part_vec gives the j-th of them.
+template <int KQ>
+__device__ __forceinline__ long long part_vec(int p, int k, int j)
+{
+    return ((long long)p * k + (long long)blockIdx.y * KQ + j) * kPartW;
+}
 
 // Level 1 of the tail scan, one workgroup of kPartW threads per group of
-// `gs` consecutive tails and RHS: u_first = a_first, u_p = a_p + B_p u_{p-1}
-// (B_p at index p - 1 of bm).  The group's last u also goes to carry[g].
+// `gs` consecutive tails and chunk of KQ right-hand sides: u_first =
+// a_first, u_p = a_p + B_p u_{p-1} (B_p at index p - 1 of bm).  The group's
+// last u also goes to carry[g].
+template <int KQ>
 __global__ void __launch_bounds__(kPartW)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
                       const double *__restrict__ bm, int gs, int q_count, int k)
 {
     const int g = (int)blockIdx.x, c = (int)threadIdx.x;
     const int p0 = g * gs, p1 = min(q_count, p0 + gs);
-    __shared__ double sv[kPartW];
-    double v = a[part_vec(p0, k) + c];
-    u[part_vec(p0, k) + c] = v;
+    __shared__ double sv[KQ][kPartW];
+    double v[KQ];
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) {
+        v[j] = a[part_vec<KQ>(p0, k, j) + c];
+        u[part_vec<KQ>(p0, k, j) + c] = v[j];
+    }
     for (int p = p0 + 1; p < p1; ++p) {
         __syncthreads();  // the previous matrix-vector product is done with sv
-        sv[c] = v;
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) sv[j][c] = v[j];
         __syncthreads();
-        v = a[part_vec(p, k) + c] + part_matvec(bm + (long long)(p - 1) * kPartW * kPartW, sv, c);
-        u[part_vec(p, k) + c] = v;
+        double mv[KQ];
+        part_matvec<KQ>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, mv);
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) {
+            v[j] = a[part_vec<KQ>(p, k, j) + c] + mv[j];
+            u[part_vec<KQ>(p, k, j) + c] = v[j];
+        }
     }
-    carry[part_vec(g, k) + c] = v;
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) carry[part_vec<KQ>(g, k, j) + c] = v[j];
 }
 
 // Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
 // p - gs of wm, T the scanned group carries), t_p = u_p in group 0.
+template <int KQ>
 __global__ void __launch_bounds__(kPartW)
     k_part_fixup(const double *__restrict__ u, const double *__restrict__ tg, const double *__restrict__ wm, int gs,
                  double *__restrict__ t, int k)
 {
     const int p = (int)blockIdx.x, c = (int)threadIdx.x;
     const int g = p / gs;
-    __shared__ double sv[kPartW];
-    double v = u[part_vec(p, k) + c];
+    __shared__ double sv[KQ][kPartW];
+    double v[KQ];
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) v[j] = u[part_vec<KQ>(p, k, j) + c];
     if (g > 0) {  // uniform in the workgroup
-        sv[c] = tg[part_vec(g - 1, k) + c];
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) sv[j][c] = tg[part_vec<KQ>(g - 1, k, j) + c];
         __syncthreads();
-        v += part_matvec(wm + (long long)(p - gs) * kPartW * kPartW, sv, c);
+        double mv[KQ];
+        part_matvec<KQ>(wm + (long long)(p - gs) * kPartW * kPartW, sv, c, mv);
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) v[j] += mv[j];
     }
-    t[part_vec(p, k) + c] = v;
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) t[part_vec<KQ>(p, k, j) + c] = v[j];
 }
 
 // a_p = the last kPartW values of y in block p (p < q_count), per RHS: the
@@ -1670,21 +1745,29 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // One round of the log-depth scan over the group carries (one workgroup of
-// kPartW threads per entry): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
+// kPartW threads per entry and chunk of KQ right-hand sides): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
 // index g - s of the round's matrices), dst_g = src_g below s.
+template <int KQ>
 __global__ void __launch_bounds__(kPartW)
     k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s, int k)
 {
     const int p = (int)blockIdx.x;
     const int c = (int)threadIdx.x;
-    __shared__ double sv[kPartW];
-    double acc = src[part_vec(p, k) + c];
+    __shared__ double sv[KQ][kPartW];
+    double acc[KQ];
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) acc[j] = src[part_vec<KQ>(p, k, j) + c];
     if (p >= s) {  // uniform in the workgroup
-        sv[c] = src[part_vec(p - s, k) + c];
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) sv[j][c] = src[part_vec<KQ>(p - s, k, j) + c];
         __syncthreads();
-        acc += part_matvec(bm + (long long)(p - s) * kPartW * kPartW, sv, c);
+        double mv[KQ];
+        part_matvec<KQ>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, mv);
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) acc[j] += mv[j];
     }
-    dst[part_vec(p, k) + c] = acc;
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) dst[part_vec<KQ>(p, k, j) + c] = acc[j];
 }
 
 // x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in
@@ -1710,6 +1793,57 @@ __global__ void __launch_bounds__(kBlock)
         xi -= (s[0] + s[1]) + (s[2] + s[3]);
     }
     x[gi] = xi;
+}
+
+// The finish for chunks of KQ >= 2 right-hand sides: a workgroup takes
+// kPartW consecutive rows (at most two blocks, as m >= kPartW) and RHS
+// blockIdx.y * KQ + j.  It stages both blocks' t_{p-1} in LDS (RHS fastest,
+// read as broadcasts), and each thread reads its row of V once for all KQ.
+// The sums run in the single-RHS kernel's order (four partial sums by
+// column mod 4), so each RHS gets that kernel's bits.
+template <int KQ>
+__global__ void __launch_bounds__(kPartW)
+    k_part_finish_multi(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t,
+                        int n, int m, int k, double *__restrict__ x)
+{
+    __shared__ double st[2][kPartW][KQ];
+    const int lo = (int)blockIdx.x * kPartW;
+    const long long q0 = (long long)blockIdx.y * KQ;
+    const int p_lo = lo / m, p_hi = (min(n, lo + kPartW) - 1) / m;
+    for (int e = (int)threadIdx.x; e < 2 * KQ * kPartW; e += kPartW) {
+        const int sblk = e / (KQ * kPartW), j = (e / kPartW) % KQ, c = e % kPartW;
+        const int p = sblk == 0 ? p_lo : p_hi;
+        st[sblk][c][j] = p > 0 ? t[((long long)(p - 1) * k + q0 + j) * kPartW + c] : 0.0;
+    }
+    __syncthreads();
+    const int i = lo + (int)threadIdx.x;
+    if (i >= n) return;
+    const int p = i / m;
+    const int sblk = p == p_lo ? 0 : 1;
+    double acc[KQ][4];
+#pragma unroll
+    for (int j = 0; j < KQ; ++j)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[j][u] = 0.0;
+    if (p > 0) {
+#pragma unroll 2
+        for (int c = 0; c < kPartW; c += 4) {
+            double mv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) mv[u] = vt[(long long)(c + u) * n + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int j = 0; j < KQ; ++j) acc[j][u] = __builtin_fma(mv[u], st[sblk][c + u][j], acc[j][u]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) {
+        const long long g = (long long)i * k + q0 + j;
+        double xi = y[g];
+        if (p > 0) xi -= (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
+        x[g] = xi;
+    }
 }
 
 }  // namespace cap

@@ -451,30 +451,98 @@ int ensure_part_rhs(cap_plan p, int k)
     return CAP_OK;
 }
 
+// Right-hand sides per wavefront of the block solve, at most (they share the
+// row data; each adds an independent reduction chain and 2 KB of LDS ring).
+constexpr int kPartBlockRhs = 4;
+
+// The largest power of two up to `cap` that divides k: the chunk of
+// right-hand sides one workgroup takes.
+inline int part_chunk(int k, int cap)
+{
+    int c = 1;
+    while (c < cap && k % (2 * c) == 0) c *= 2;
+    return c;
+}
+
+template <int KR>
+void launch_part_block(cap_plan p, const double *b, int k, hipStream_t st)
+{
+    const PartState *ps = p->part;
+    k_part_block<KR><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
+        ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b, ps->y, p->n, ps->m, ps->ew, k);
+}
+
+// Right-hand sides per workgroup of the group scan and the carry rounds, at
+// most: those grids have only G (= P / gs) workgroups per chunk, and each step
+// of a workgroup is a chain, so they keep more, smaller chunks than the
+// fix-up and the finish (Q and n / kPartW workgroups), which take 16.
+constexpr int kPartScanRhs = 4;
+
+// The tail scan over the group carries, KQ right-hand sides per workgroup
+// (KQ | k); returns the buffer that holds the scanned carries.
+template <int KQ>
+double *launch_part_carries(cap_plan p, int k, hipStream_t st)
+{
+    const PartState *ps = p->part;
+    const int Q = ps->Q;
+    const unsigned chunks = (unsigned)(k / KQ);
+    const long long mat = (long long)kPartW * kPartW;
+    k_part_tails<<<blocks_for((long long)Q * k * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, k, ps->a);
+    k_part_group_scan<KQ><<<dim3(ps->G, chunks), kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q, k);
+    double *cur = ps->c0, *nxt = ps->c1;
+    for (int r = 0; r < ps->rounds; ++r) {
+        k_part_scan<KQ><<<dim3(ps->G, chunks), kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat,
+                                                                1 << r, k);
+        std::swap(cur, nxt);
+    }
+    return cur;
+}
+
+// The fix-up of every tail and the finish, KQ right-hand sides per
+// workgroup (KQ | k); `tg`: the scanned group carries.
+template <int KQ>
+void launch_part_fix(cap_plan p, const double *tg, double *x, int k, hipStream_t st)
+{
+    const PartState *ps = p->part;
+    const unsigned chunks = (unsigned)(k / KQ);
+    const long long mat = (long long)kPartW * kPartW;
+    k_part_fixup<KQ><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs, ps->t, k);
+    if constexpr (KQ == 1)
+        k_part_finish<<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+    else  // m >= kPartW: a workgroup's kPartW rows span at most two blocks
+        k_part_finish_multi<KQ><<<dim3((unsigned)((p->n + kPartW - 1) / kPartW), chunks), kPartW, 0, st>>>(
+            ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+}
+
 // One solve of a partitioned plan (k FP64 right-hand sides, interleaved:
-// b[row * k + q]): y = D^{-1} b (one wavefront per block and RHS), the block
-// tails, the two-level tail scan (in groups, over the group carries, then the
-// fix-up; grids over tails x RHS), then x = y - V t.  Every step is on `st`;
-// none of them waits on another workgroup.
+// b[row * k + q]): y = D^{-1} b (one wavefront per block and chunk of
+// right-hand sides), the block tails, the two-level tail scan (in groups,
+// over the group carries, then the fix-up; grids over tails x chunks), then
+// x = y - V t.  Every step is on `st`; none of them waits on another
+// workgroup.  Each right-hand side gets the bits of its own one-RHS solve.
 int launch_partition(cap_plan p, const double *b, double *x, int k, hipStream_t st)
 {
     PartState *ps = p->part;
-    if (!ps || k < 1 || k > kWave) return CAP_ERR_ARG;
+    if (!ps || k < 1 || k > kWave || ps->m < kPartW) return CAP_ERR_ARG;
     CAP_TRY(ensure_part_rhs(p, k));
-    const unsigned ku = (unsigned)k;
-    k_part_block<<<dim3(ps->P, ku), kWave, 0, st>>>(ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b,
-                                                   ps->y, p->n, ps->m, ps->ew, k);
-    const int Q = ps->Q;
-    const long long mat = (long long)kPartW * kPartW;
-    k_part_tails<<<blocks_for((long long)Q * k * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, k, ps->a);
-    k_part_group_scan<<<dim3(ps->G, ku), kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q, k);
-    double *cur = ps->c0, *nxt = ps->c1;
-    for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<<<dim3(ps->G, ku), kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r, k);
-        std::swap(cur, nxt);
+    switch (part_chunk(k, kPartBlockRhs)) {
+    case 4: launch_part_block<4>(p, b, k, st); break;
+    case 2: launch_part_block<2>(p, b, k, st); break;
+    default: launch_part_block<1>(p, b, k, st); break;
     }
-    k_part_fixup<<<dim3(Q, ku), kPartW, 0, st>>>(ps->u, cur, ps->bm + ps->w_off * mat, ps->gs, ps->t, k);
-    k_part_finish<<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+    const double *tg = nullptr;
+    switch (part_chunk(k, kPartScanRhs)) {
+    case 4: tg = launch_part_carries<4>(p, k, st); break;
+    case 2: tg = launch_part_carries<2>(p, k, st); break;
+    default: tg = launch_part_carries<1>(p, k, st); break;
+    }
+    switch (part_chunk(k, 16)) {
+    case 16: launch_part_fix<16>(p, tg, x, k, st); break;
+    case 8: launch_part_fix<8>(p, tg, x, k, st); break;
+    case 4: launch_part_fix<4>(p, tg, x, k, st); break;
+    case 2: launch_part_fix<2>(p, tg, x, k, st); break;
+    default: launch_part_fix<1>(p, tg, x, k, st); break;
+    }
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
 }

@@ -14,9 +14,10 @@
     python -m capellinisptrsv_amd.profile_targets refine [--steps N]
         the wide-level matrix: N FP64 solves, then N FP32 + FP64 refined
         solves (fused first two steps) to 1e-12
-    python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto]
+    python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto] [--rhs K]
         the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels);
-        --algo partition builds the partitioned solve without the autotuner
+        --algo partition builds the partitioned solve without the autotuner;
+        --rhs K solves K right-hand sides at once
 """
 import argparse
 import sys
@@ -28,6 +29,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--algo", default="auto")
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--rhs", type=int, default=1)
     a = ap.parse_args(argv)
 
     import torch
@@ -64,6 +66,8 @@ def main(argv=None):
     n = len(rp) - 1
     d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
     b = torch.from_numpy(api.uniform(1, n)).to(dev)
+    if a.target == "deep" and a.rhs > 1:
+        b = torch.from_numpy(api.uniform(1, n * a.rhs).reshape(n, a.rhs)).to(dev)
     x = torch.empty_like(b)
     plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep", "refine") else Plan(*d, algo="wrtfst", sync="value")
     for _ in range(a.steps):

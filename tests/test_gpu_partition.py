@@ -87,22 +87,28 @@ def _columns_within_tol(x, ref, k):
     assert max(errs) <= FP64_TOL, errs
 
 
-def test_partition_several_rhs(cuda):
-    """k = 16, 64, then 2 and 1 again (the work vectors grow, then serve the
-    smaller k as they are), then 4; one RHS equal to another must give the
-    same bits in both columns."""
+@pytest.mark.parametrize("name,mat", [CASES[0], CASES[3], CASES[6]], ids=[CASES[i][0] for i in (0, 3, 6)])
+def test_partition_several_rhs(cuda, name, mat):
+    """k = 16, 64, then smaller counts (the work vectors grow, then serve a
+    smaller k as they are), with chunks of 16, 8, 4, 2 and 1 right-hand sides
+    per workgroup (k = 12: 4; k = 6: 2; k = 3: 1).  Each column is within the
+    tolerance of its own sequential solve and has the bits of the same column
+    solved alone (the chunked kernels keep the one-RHS summation order)."""
     import torch
-    rp, ci, v = api.make_config(2, seed=8, scale=0.02)
+    rp, ci, v = mat
     n = len(rp) - 1
     plan = _plan(cuda, rp, ci, v, algo="partition")
-    for seed, k in ((40, 16), (41, 64), (42, 2), (43, 1), (44, 4)):
+
+    def alone(col):
+        return plan.solve(torch.from_numpy(np.ascontiguousarray(col)).to(cuda)).cpu().numpy()
+
+    for seed, k in ((40, 16), (41, 64), (42, 2), (43, 1), (44, 4), (45, 8), (46, 12), (47, 6), (48, 3)):
         b = api.uniform(seed, n * k).reshape(n, k)
-        if k > 1:
-            b[:, k - 1] = b[:, 0]
         x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy().reshape(n, k)
         assert plan.status() == 0
         _columns_within_tol(x, oracle.csr_forward(rp, ci, v, b.reshape(-1), rhs=k, fused=True), k)
-        assert np.array_equal(x[:, k - 1], x[:, 0]), k
+        for q in sorted({0, k // 2, k - 1}):
+            assert np.array_equal(x[:, q], alone(b[:, q])), (k, q)
         if k == 1:
             # the k = 1 solve through work vectors sized for 64 gives the bits of a fresh plan
             fresh = _plan(cuda, rp, ci, v, algo="partition")
