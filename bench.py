@@ -304,6 +304,18 @@ def extra_workloads(args):
             kind, cpu_ms, reps = _cpu_serial_ms(rp, ci, v, b.cpu().numpy(), seconds=3.0)
             res[name]["cpu_serial"] = {"ms_per_solve": round(cpu_ms, 3), "kind": kind, "cores": 1, "solves": reps,
                                        "gpu_speedup": round(cpu_ms / ms, 2)}
+        if cfg == 2 and p.info()["algo"] == "partition":
+            # 16 right-hand sides at once through the same plan (interleaved)
+            k = 16
+            bh = api.uniform(5, n * k)
+            bk = torch.from_numpy(bh.reshape(n, k)).to(dev)
+            xk = torch.empty_like(bk)
+            ms_k = _time_gpu(lambda: p.solve(bk, out=xk), 10, warmup=2)
+            assert p.status() == 0
+            ref = oracle.csr_forward(rp, ci, v, bh, rhs=k, fused=True).reshape(n, k)
+            err = float(np.max(np.abs(xk.cpu().numpy() - ref)))
+            res[name]["rhs16"] = {"ms_per_solve": round(ms_k, 4), "ms_per_rhs": round(ms_k / k, 4),
+                                  "rel_err_vs_oracle": err / float(np.max(np.abs(ref)))}
         p.close()
 
     # differentiable solve: n = 1,000,000 level matrix, forward + adjoint
