@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace cap {
 
 constexpr int kWave = 64;
@@ -1440,15 +1442,19 @@ __global__ void __launch_bounds__(kBlock)
     b0[g] = -vt[(long long)k * n + row];
 }
 
-// y = D^{-1} b: one wavefront per block, rows in order.  The block's
-// in-block off-diagonal entries are stored ELL-style, ew <= kPartEll per row
-// (column, value; column -1 pads), the rest of a longer row in an overflow
-// CSR (ov_rp per row).  Entry l of a row sits in lane l; the products with
-// the block's earlier values are summed by a DPP tree over the lanes and
-// multiplied by the reciprocal diagonal.  Row data are loaded a group of
-// kPartPre rows ahead.  The sum is reassociated and the division is a
-// multiplication (tolerance, not bit-exact), like the whole partitioned
-// solve.
+// y = D^{-1} b: one wavefront per block, rows in order.  A row's in-block
+// entries in columns i-1 and i-2 are kept apart (near: their values, summed
+// per column); the others (columns <= i-3) are stored ELL-style, ew <=
+// kPartEll per row (column, value; column -1 pads), the rest of a longer
+// row in an overflow CSR (ov_rp per row).  Entry l of a row sits in lane l;
+// its products with the block's earlier values are summed by a DPP tree over
+// the lanes, and
+//   y_i = ((b_i - S_i - w2_i y_{i-2}) - w1_i y_{i-1}) / d_i
+// (as a multiplication by the reciprocal diagonal).  S_i needs only values
+// up to y_{i-3}, so it is summed two rows ahead, off the chain from one row
+// to the next.  Row data are loaded a group of kPartPre rows ahead.  The sum
+// is reassociated and the division is a multiplication (tolerance, not
+// bit-exact), like the whole partitioned solve.
 constexpr int kPartEll = 64;
 constexpr int kPartPre = 16;
 
@@ -1482,7 +1488,8 @@ template <int KR>
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
-                 const double *__restrict__ b, double *__restrict__ y, int n, int m, int ew, int k)
+                 const double2 *__restrict__ near, const double *__restrict__ b, double *__restrict__ y, int n, int m,
+                 int ew, int k)
 {
     constexpr int kRing = 2 * kPartW;
     __shared__ double ring[KR][kRing];
@@ -1492,10 +1499,10 @@ __global__ void __launch_bounds__(kWave)
     const int start = blockIdx.x * m;
     const int end = min(n, start + m);
     // a group's row data: entry `lane` of each of its kPartPre rows, and, in
-    // lane r, row r's b values, reciprocal diagonal and overflow start (lane
-    // kPartPre: the end)
+    // lane r, row r's b values, reciprocal diagonal, near values and
+    // overflow start (lane kPartPre: the end)
     int nc[kPartPre];
-    double nv[kPartPre], nb[KR], nd = 1.0;
+    double nv[kPartPre], nb[KR], nd = 1.0, nw1 = 0.0, nw2 = 0.0;
     int no = 0;
     auto fetch = [&](int g0) {
 #pragma unroll
@@ -1506,32 +1513,30 @@ __global__ void __launch_bounds__(kWave)
             nv[r] = in ? ell_v[e] : 0.0;
         }
         const int row = min(g0 + lane, end);
+        const bool mine = lane < kPartPre && row < end;
 #pragma unroll
-        for (int j = 0; j < KR; ++j) nb[j] = lane < kPartPre && row < end ? b[(long long)row * k + q0 + j] : 0.0;
-        nd = lane < kPartPre && row < end ? inv_diag[row] : 1.0;
+        for (int j = 0; j < KR; ++j) nb[j] = mine ? b[(long long)row * k + q0 + j] : 0.0;
+        nd = mine ? inv_diag[row] : 1.0;
+        const double2 w = mine ? near[row] : make_double2(0.0, 0.0);
+        nw1 = w.x;
+        nw2 = w.y;
         no = lane <= kPartPre ? ov_rp[row] : 0;
     };
-    // Every lane computes each row's values.  Inside a group the ring (rows'
-    // values by row mod kRing) is read one row ahead: row i+1's entry is
-    // fetched while row i is summed (its column is below i, so already
-    // written -- one wavefront's LDS accesses complete in order); an entry in
-    // column i itself takes the value just computed.  A group's first row
-    // reads its entry at the group's top, before the next prefetch is
-    // issued: reading it ahead from the prefetch would make the previous
-    // row wait for every load of that prefetch.  The KR right-hand sides
-    // share the row data; their sums are independent chains.  The values go
-    // to y from the ring, a group at a time (one store of kPartPre x KR
-    // values at the next group's top): a store per row would make the waits
-    // for the prefetch (the memory counter covers stores too) wait for it.
+    // The values go to y from the ring (rows' values by row mod kRing), a
+    // group at a time (one store of kPartPre x KR values at the next group's
+    // top): a store per row would make the waits for the prefetch (the
+    // memory counter covers stores too) wait for it.
     static_assert(kPartPre * KR <= kWave, "a group's values are stored by one instruction");
     const int wr = lane / KR, wj = lane % KR;  // the (row, RHS) this lane stores
     auto store_group = [&](int s0) {
         const int i = s0 + wr;
         if (lane < kPartPre * KR && i < end) y[(long long)i * k + q0 + wj] = ring[wj][(i - start) & (kRing - 1)];
     };
-    double yprev[KR], rv[KR];
+    // y_{i-1}, y_{i-2} per right-hand side (0 before the block's first row,
+    // where the near values are 0 too)
+    double y1[KR], y2[KR];
 #pragma unroll
-    for (int j = 0; j < KR; ++j) yprev[j] = 0.0;
+    for (int j = 0; j < KR; ++j) y1[j] = y2[j] = 0.0;
     fetch(start);
     for (int g0 = start; g0 < end; g0 += kPartPre) {
         int cc[kPartPre];
@@ -1542,43 +1547,86 @@ __global__ void __launch_bounds__(kWave)
             cv[r] = nv[r];
         }
 #pragma unroll
-        for (int j = 0; j < KR; ++j) {
-            bg[j] = nb[j];
-            rv[j] = cc[0] >= 0 && cc[0] < g0 ? ring[j][(cc[0] - start) & (kRing - 1)] : 0.0;
-        }
-        const double dg = nd;
+        for (int j = 0; j < KR; ++j) bg[j] = nb[j];
+        const double dg = nd, w1g = nw1, w2g = nw2;
         const int og = no;
-        if (g0 + kPartPre < end) fetch(g0 + kPartPre);
-        if (g0 > start) store_group(g0 - kPartPre);
+        // FAST: a whole group with no overflow entries (no branch between
+        // its rows, so their sums and chains can interleave)
+        const bool fast = g0 + kPartPre <= end &&
+                          __builtin_amdgcn_readlane(og, 0) == __builtin_amdgcn_readlane(og, kPartPre);
+        auto group = [&](auto fast_tag) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            // ring values of row r's ELL entries (all in columns the ring
+            // holds); pad lanes read slot 0 and drop it, so that no lane
+            // mask splits the rows' code
+            auto ring_vals = [&](int r, double (&rv)[KR]) {
+                const bool has = cc[r] >= 0;
+                const int slot = (has ? cc[r] - start : 0) & (kRing - 1);
 #pragma unroll
-        for (int r = 0; r < kPartPre; ++r) {
-            const int i = g0 + r;
-            if (i >= end) continue;  // uniform: the block's last group
-            double prod[KR];
+                for (int j = 0; j < KR; ++j) {
+                    const double v = ring[j][slot];
+                    rv[j] = has ? v : 0.0;
+                }
+            };
+            // S of row r from its ring values (plus its overflow entries)
+            auto row_sum = [&](int r, const double (&rv)[KR], double (&s)[KR]) {
+                double prod[KR];
 #pragma unroll
-            for (int j = 0; j < KR; ++j) prod[j] = cc[r] >= 0 ? cv[r] * (cc[r] == i - 1 ? yprev[j] : rv[j]) : 0.0;
-            const int o0 = __builtin_amdgcn_readlane(og, r), o1 = __builtin_amdgcn_readlane(og, r + 1);
-            for (int e = o0 + lane; e < o1; e += kWave) {  // rows longer than kPartEll
-                const int c = ov_c[e];
-                const double w = ov_v[e];
+                for (int j = 0; j < KR; ++j) prod[j] = cv[r] * rv[j];
+                if constexpr (!FAST) {
+                    const int o0 = __builtin_amdgcn_readlane(og, r), o1 = __builtin_amdgcn_readlane(og, r + 1);
+                    for (int e = o0 + lane; e < o1; e += kWave) {  // rows longer than kPartEll
+                        const int c = ov_c[e];
+                        const double w = ov_v[e];
 #pragma unroll
-                for (int j = 0; j < KR; ++j)
-                    prod[j] = __builtin_fma(w, c == i - 1 ? yprev[j] : ring[j][(c - start) & (kRing - 1)], prod[j]);
+                        for (int j = 0; j < KR; ++j)
+                            prod[j] = __builtin_fma(w, ring[j][(c - start) & (kRing - 1)], prod[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < KR; ++j) s[j] = wave_sum(prod[j]);
+            };
+            // the sums of the group's first two rows (columns below g0 - 1)
+            double s0[KR], s1[KR], rv0[KR], rv1[KR];
+            ring_vals(0, rv0);
+            ring_vals(1, rv1);
+            if (g0 + kPartPre < end) fetch(g0 + kPartPre);
+            if (g0 > start) store_group(g0 - kPartPre);
+            row_sum(0, rv0, s0);
+            row_sum(1, rv1, s1);
+#pragma unroll
+            for (int r = 0; r < kPartPre; ++r) {
+                const int i = g0 + r;
+                if constexpr (!FAST) {
+                    if (i >= end) continue;  // uniform: the block's last group
+                }
+                // row r + 2's ring values (columns <= i - 1) before y_i is written
+                double rv2[KR];
+                if (r + 2 < kPartPre) ring_vals(r + 2, rv2);
+                const double di = bcast(dg, r), w1 = bcast(w1g, r), w2 = bcast(w2g, r);
+                double mine = 0.0;  // lanes wj = j (lane mod KR) keep right-hand side j's value
+#pragma unroll
+                for (int j = 0; j < KR; ++j) {
+                    double t = bcast(bg[j], r) - s0[j];
+                    t = __builtin_fma(-w2, y2[j], t);
+                    t = __builtin_fma(-w1, y1[j], t);
+                    const double yi = t * di;
+                    y2[j] = y1[j];
+                    y1[j] = yi;
+                    if (wj == j) mine = yi;
+                }
+                // every lane writes (the lanes of one RHS the same value to
+                // the same slot), so no lane mask splits the rows' code
+                ring[wj][(i - start) & (kRing - 1)] = mine;
+#pragma unroll
+                for (int j = 0; j < KR; ++j) s0[j] = s1[j];
+                if (r + 2 < kPartPre) row_sum(r + 2, rv2, s1);
             }
-            const int cn = r + 1 < kPartPre ? cc[r + 1] : -1;  // the next row's column (in this group)
-            const bool ahead = cn >= 0 && cn < i;
-#pragma unroll
-            for (int j = 0; j < KR; ++j) rv[j] = ahead ? ring[j][(cn - start) & (kRing - 1)] : 0.0;
-            const double di = bcast(dg, r);
-            double mine = 0.0;  // lane j keeps right-hand side j's value
-#pragma unroll
-            for (int j = 0; j < KR; ++j) {
-                const double yi = (bcast(bg[j], r) - wave_sum(prod[j])) * di;
-                yprev[j] = yi;
-                if (lane == j) mine = yi;
-            }
-            if (lane < KR) ring[lane][(i - start) & (kRing - 1)] = mine;
-        }
+        };
+        if (fast)
+            group(std::true_type{});
+        else
+            group(std::false_type{});
     }
     store_group(start + (end - 1 - start) / kPartPre * kPartPre);  // the last group
 }

@@ -61,6 +61,7 @@ struct PartState {
     int gs = 0, G = 0, rounds = 0;    // tails per group, groups, carry-scan rounds
     int *ell_c = nullptr, *ov_rp = nullptr, *ov_c = nullptr;
     double *ell_v = nullptr, *ov_v = nullptr, *diag = nullptr;
+    double *near = nullptr;  // per row: the entries in columns i-1 and i-2 (in-block)
     double *vt = nullptr;
     double *bm = nullptr;
     long long w_off = 0;
@@ -253,7 +254,7 @@ void free_partition(cap_plan p)
     if (!ps) return;
     (void)hipDeviceSynchronize();
     for (void *b : {(void *)ps->ell_c, (void *)ps->ov_rp, (void *)ps->ov_c, (void *)ps->ell_v, (void *)ps->ov_v,
-                    (void *)ps->diag, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a, (void *)ps->u,
+                    (void *)ps->diag, (void *)ps->near, (void *)ps->vt, (void *)ps->bm, (void *)ps->y, (void *)ps->a, (void *)ps->u,
                     (void *)ps->t, (void *)ps->c0, (void *)ps->c1, (void *)ps->res, (void *)ps->cor})
         if (b) (void)hipFree(b);
     delete ps;
@@ -469,7 +470,8 @@ void launch_part_block(cap_plan p, const double *b, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     k_part_block<KR><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
-        ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, b, ps->y, p->n, ps->m, ps->ew, k);
+        ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, (const double2 *)ps->near, b, ps->y, p->n,
+        ps->m, ps->ew, k);
 }
 
 // Right-hand sides per workgroup of the group scan and the carry rounds, at
@@ -822,13 +824,14 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     for (int i = 0; i < n; ++i)
         for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) w = std::max(w, i - ci[j]);
     if (w > kPartW) return CAP_ERR_UNSUPPORTED;
-    // ELL slab width: the longest row's in-block off-diagonals, rounded up to
-    // 8, at most kPartEll (the rest of a longer row goes to the overflow)
+    // ELL slab width: the longest row's in-block off-diagonals in columns
+    // <= i - 3 (columns i-1 and i-2 go to the near table), rounded up to 8,
+    // at most kPartEll (the rest of a longer row goes to the overflow)
     int longest = 1;
     for (int i = 0; i < n; ++i) {
         const int start = (i / m) * m;
         int cnt = 0;
-        for (int j = rp_host[i]; j < rp_host[i + 1] - 1; ++j) cnt += ci[j] >= start;
+        for (int j = rp_host[i]; j < rp_host[i + 1] - 1; ++j) cnt += ci[j] >= start && ci[j] < i - 2;
         longest = std::max(longest, cnt);
     }
     const int ew = std::min(kPartEll, (longest + 7) / 8 * 8);
@@ -842,19 +845,20 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     for (int r = 0; r < rounds; ++r) m_off[(size_t)r + 1] = m_off[(size_t)r] + (G - (1 << r));
     const long long n_mat = m_off[(size_t)rounds];
     {
-        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 24.0) + (double)n_mat * kPartW * kPartW * 8.0 +
+        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 40.0) + (double)n_mat * kPartW * kPartW * 8.0 +
                              12.0 * nnz;
         if (bytes > kPartMaxBytes) return CAP_ERR_UNSUPPORTED;
     }
     std::vector<double> v((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
-    // the block-diagonal part: in-block off-diagonals in stored order, the
-    // first ew of a row in its ELL slab (column -1 pads), the rest in the
-    // overflow CSR; and, for the coupling solves, the same entries as a
-    // batch with local column indices
+    // the block-diagonal part: in-block off-diagonals in columns i-1 and i-2
+    // summed into the near table (two per row), the others in stored order,
+    // the first ew of a row in its ELL slab (column -1 pads), the rest in the
+    // overflow CSR; and, for the coupling solves, all of them as a batch with
+    // local column indices
     std::vector<int> ell_c((size_t)n * ew, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1), rpd((size_t)n + 1),
         cid;
-    std::vector<double> ell_v((size_t)n * ew, 0.0), ov_v, dg((size_t)n), vd;
+    std::vector<double> ell_v((size_t)n * ew, 0.0), ov_v, dg((size_t)n), vd, nr((size_t)n * 2, 0.0);
     cid.reserve((size_t)nnz);
     vd.reserve((size_t)nnz);
     for (int q = 0; q <= P; ++q) ro[(size_t)q] = (int)std::min<long long>(n, (long long)q * m);
@@ -865,6 +869,12 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         int used = 0;
         for (int j = rp_host[i]; j < d; ++j) {
             if (ci[j] < start) continue;
+            cid.push_back(ci[j] - start);
+            vd.push_back(v[j]);
+            if (ci[j] >= i - 2) {  // column i-1 or i-2
+                nr[(size_t)i * 2 + (size_t)(i - 1 - ci[j])] += v[j];
+                continue;
+            }
             if (used < ew) {
                 ell_c[(size_t)i * ew + used] = ci[j];
                 ell_v[(size_t)i * ew + used] = v[j];
@@ -873,8 +883,6 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
                 ov_v.push_back(v[j]);
             }
             ++used;
-            cid.push_back(ci[j] - start);
-            vd.push_back(v[j]);
         }
         dg[(size_t)i] = 1.0 / v[(size_t)d];
         cid.push_back(i - start);
@@ -946,6 +954,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
         if ((rc = dev_alloc(&ps->ell_c, ell_c.size())) || (rc = dev_alloc(&ps->ell_v, ell_v.size())) ||
             (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
             (rc = dev_alloc(&ps->ov_v, n_ov)) || (rc = dev_alloc(&ps->diag, (size_t)n)) ||
+            (rc = dev_alloc(&ps->near, (size_t)n * 2)) ||
             (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)) || (rc = ensure_part_rhs(p, 1)))
             break;
         if (n_mat > 0 && (rc = dev_alloc(&ps->bm, (size_t)(n_mat * mat)))) break;
@@ -956,7 +965,8 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
             (!ov_c.empty() &&
              (hipMemcpy(ps->ov_c, ov_c.data(), sizeof(int) * ov_c.size(), hipMemcpyHostToDevice) != hipSuccess ||
               hipMemcpy(ps->ov_v, ov_v.data(), sizeof(double) * ov_v.size(), hipMemcpyHostToDevice) != hipSuccess)) ||
-            hipMemcpy(ps->diag, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(ps->diag, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ps->near, nr.data(), sizeof(double) * nr.size(), hipMemcpyHostToDevice) != hipSuccess) {
             rc = CAP_ERR_HIP;
             break;
         }
