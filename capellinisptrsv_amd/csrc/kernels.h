@@ -1459,20 +1459,25 @@ constexpr int kPartEll = 64;
 constexpr int kPartPre = 16;
 
 // A double moved between lanes by DPP (row shifts and row broadcasts of
-// the GFX9 DPP set); lanes with no source, or outside ROW_MASK, read 0.
+// the GFX9 DPP set).  Lanes whose source lies outside their row read 0
+// (bound_ctrl); lanes in rows outside ROW_MASK are left undefined (no old
+// value to set up: two moves fewer per step).
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_d(double v)
 {
     const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, ROW_MASK, 0xf, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Sum over the 64 lanes, in every lane: prefix sums within rows of 16 by
-// row_shr 1, 2, 4, 8 (lane 15 of a row then holds the row's sum), row 0's
-// sum broadcast into row 1 and row 2's into row 3 (row_bcast:15), then lane
-// 31's into rows 2 and 3 (row_bcast:31): lane 63 holds the total.
+// Sum over the 64 lanes, returned in every lane: prefix sums within rows of
+// 16 by row_shr 1, 2, 4, 8 (lane 15 of a row then holds the row's sum), row
+// 0's sum added into row 1 and row 2's into row 3 (row_bcast:15, rows 1 and
+// 3 only), then lane 31's into rows 2 and 3 (row_bcast:31): lane 63 holds
+// the total, which is broadcast.  Only lane 63's value is read, and every
+// value it depends on comes from a row the step wrote, so the rows the masks
+// leave undefined never reach it.
 __device__ __forceinline__ double wave_sum(double v)
 {
     v += dpp_d<0x111, 0xf>(v);
