@@ -1445,8 +1445,10 @@ __global__ void __launch_bounds__(kBlock)
 // y = D^{-1} b: one wavefront per block, rows in order.  A row's in-block
 // entries in columns i-1 and i-2 are kept apart (near: their values, summed
 // per column); the others (columns <= i-3) are stored ELL-style, ew <=
-// kPartEll per row (column, value; column -1 pads), the rest of a longer
-// row in an overflow CSR (ov_rp per row).  Entry l of a row sits in lane l;
+// kPartEll per row (value, and the byte offset of the column's slot in the
+// LDS ring of the block's values: ((col - block start) mod kPartRing) * 8;
+// pads point at slot kPartRing, which holds 0), the rest of a longer row in
+// an overflow CSR (ov_rp per row, absolute columns).  Entry l of a row sits in lane l;
 // its products with the block's earlier values are summed by a DPP tree over
 // the lanes, and
 //   y_i = ((b_i - S_i - w2_i y_{i-2}) - w1_i y_{i-1}) / d_i
@@ -1457,6 +1459,7 @@ __global__ void __launch_bounds__(kBlock)
 // bit-exact), like the whole partitioned solve.
 constexpr int kPartEll = 64;
 constexpr int kPartPre = 16;
+constexpr int kPartRing = 2 * kPartW;  // ring slots (rows' values by row mod kPartRing)
 
 // A double moved between lanes by DPP (row shifts and row broadcasts of
 // the GFX9 DPP set).  Lanes whose source lies outside their row read 0
@@ -1496,13 +1499,16 @@ __global__ void __launch_bounds__(kWave)
                  const double2 *__restrict__ near, const double *__restrict__ b, double *__restrict__ y, int n, int m,
                  int ew, int k)
 {
-    constexpr int kRing = 2 * kPartW;
-    __shared__ double ring[KR][kRing];
+    constexpr int kRing = kPartRing;
+    constexpr int kRow = KR + 3;  // a row's scalars in LDS: b per RHS, 1 / diagonal, near (i-1), near (i-2)
+    __shared__ double ring[KR][kRing + 1];
+    __shared__ double rowd[kPartPre][kRow];
     const int lane = threadIdx.x;
     // right-hand sides q0 .. q0 + KR - 1 (b and y interleaved, k per row)
     const long long q0 = (long long)blockIdx.y * KR;
     const int start = blockIdx.x * m;
     const int end = min(n, start + m);
+    if (lane < KR) ring[lane][kRing] = 0.0;  // the pads' slot
     // a group's row data: entry `lane` of each of its kPartPre rows, and, in
     // lane r, row r's b values, reciprocal diagonal, near values and
     // overflow start (lane kPartPre: the end)
@@ -1514,7 +1520,7 @@ __global__ void __launch_bounds__(kWave)
         for (int r = 0; r < kPartPre; ++r) {
             const long long e = (long long)(g0 + r) * ew + lane;
             const bool in = g0 + r < end && lane < ew;
-            nc[r] = in ? ell_c[e] : -1;
+            nc[r] = in ? ell_c[e] : kRing * 8;
             nv[r] = in ? ell_v[e] : 0.0;
         }
         const int row = min(g0 + lane, end);
@@ -1527,10 +1533,10 @@ __global__ void __launch_bounds__(kWave)
         nw2 = w.y;
         no = lane <= kPartPre ? ov_rp[row] : 0;
     };
-    // The values go to y from the ring (rows' values by row mod kRing), a
-    // group at a time (one store of kPartPre x KR values at the next group's
-    // top): a store per row would make the waits for the prefetch (the
-    // memory counter covers stores too) wait for it.
+    // The values go to y from the ring, a group at a time (one store of
+    // kPartPre x KR values at the next group's top): a store per row would
+    // make the waits for the prefetch (the memory counter covers stores too)
+    // wait for it.
     static_assert(kPartPre * KR <= kWave, "a group's values are stored by one instruction");
     const int wr = lane / KR, wj = lane % KR;  // the (row, RHS) this lane stores
     auto store_group = [&](int s0) {
@@ -1545,15 +1551,20 @@ __global__ void __launch_bounds__(kWave)
     fetch(start);
     for (int g0 = start; g0 < end; g0 += kPartPre) {
         int cc[kPartPre];
-        double cv[kPartPre], bg[KR];
+        double cv[kPartPre];
 #pragma unroll
         for (int r = 0; r < kPartPre; ++r) {
             cc[r] = nc[r];
             cv[r] = nv[r];
         }
+        // the group's row scalars, read back as broadcasts row by row
+        if (lane < kPartPre) {
 #pragma unroll
-        for (int j = 0; j < KR; ++j) bg[j] = nb[j];
-        const double dg = nd, w1g = nw1, w2g = nw2;
+            for (int j = 0; j < KR; ++j) rowd[lane][j] = nb[j];
+            rowd[lane][KR] = nd;
+            rowd[lane][KR + 1] = nw1;
+            rowd[lane][KR + 2] = nw2;
+        }
         const int og = no;
         // FAST: a whole group with no overflow entries (no branch between
         // its rows, so their sums and chains can interleave)
@@ -1562,16 +1573,10 @@ __global__ void __launch_bounds__(kWave)
         auto group = [&](auto fast_tag) {
             constexpr bool FAST = decltype(fast_tag)::value;
             // ring values of row r's ELL entries (all in columns the ring
-            // holds); pad lanes read slot 0 and drop it, so that no lane
-            // mask splits the rows' code
+            // holds; pads read the zero slot)
             auto ring_vals = [&](int r, double (&rv)[KR]) {
-                const bool has = cc[r] >= 0;
-                const int slot = (has ? cc[r] - start : 0) & (kRing - 1);
 #pragma unroll
-                for (int j = 0; j < KR; ++j) {
-                    const double v = ring[j][slot];
-                    rv[j] = has ? v : 0.0;
-                }
+                for (int j = 0; j < KR; ++j) rv[j] = *(const double *)((const char *)ring[j] + cc[r]);
             };
             // S of row r from its ring values (plus its overflow entries)
             auto row_sum = [&](int r, const double (&rv)[KR], double (&s)[KR]) {
@@ -1608,11 +1613,11 @@ __global__ void __launch_bounds__(kWave)
                 // row r + 2's ring values (columns <= i - 1) before y_i is written
                 double rv2[KR];
                 if (r + 2 < kPartPre) ring_vals(r + 2, rv2);
-                const double di = bcast(dg, r), w1 = bcast(w1g, r), w2 = bcast(w2g, r);
+                const double di = rowd[r][KR], w1 = rowd[r][KR + 1], w2 = rowd[r][KR + 2];
                 double mine = 0.0;  // lanes wj = j (lane mod KR) keep right-hand side j's value
 #pragma unroll
                 for (int j = 0; j < KR; ++j) {
-                    double t = bcast(bg[j], r) - s0[j];
+                    double t = rowd[r][j] - s0[j];
                     t = __builtin_fma(-w2, y2[j], t);
                     t = __builtin_fma(-w1, y1[j], t);
                     const double yi = t * di;

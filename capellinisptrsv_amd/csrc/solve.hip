@@ -853,11 +853,12 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
     // the block-diagonal part: in-block off-diagonals in columns i-1 and i-2
     // summed into the near table (two per row), the others in stored order,
-    // the first ew of a row in its ELL slab (column -1 pads), the rest in the
-    // overflow CSR; and, for the coupling solves, all of them as a batch with
-    // local column indices
-    std::vector<int> ell_c((size_t)n * ew, -1), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1), rpd((size_t)n + 1),
-        cid;
+    // the first ew of a row in its ELL slab (as the byte offset of the
+    // column's slot in the block kernel's LDS ring; pads point at the ring's
+    // zero slot), the rest in the overflow CSR (absolute columns); and, for
+    // the coupling solves, all of them as a batch with local column indices
+    std::vector<int> ell_c((size_t)n * ew, kPartRing * 8), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1),
+        rpd((size_t)n + 1), cid;
     std::vector<double> ell_v((size_t)n * ew, 0.0), ov_v, dg((size_t)n), vd, nr((size_t)n * 2, 0.0);
     cid.reserve((size_t)nnz);
     vd.reserve((size_t)nnz);
@@ -876,7 +877,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
                 continue;
             }
             if (used < ew) {
-                ell_c[(size_t)i * ew + used] = ci[j];
+                ell_c[(size_t)i * ew + used] = ((ci[j] - start) & (kPartRing - 1)) * 8;  // ring slot byte offset
                 ell_v[(size_t)i * ew + used] = v[j];
             } else {
                 ov_c.push_back(ci[j]);
