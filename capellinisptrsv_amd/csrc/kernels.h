@@ -1474,22 +1474,34 @@ __device__ __forceinline__ double dpp_d(double v)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Sum over the 64 lanes, returned in every lane: prefix sums within rows of
+// A sum over the 64 lanes, returned in every lane: prefix sums within rows of
 // 16 by row_shr 1, 2, 4, 8 (lane 15 of a row then holds the row's sum), row
 // 0's sum added into row 1 and row 2's into row 3 (row_bcast:15, rows 1 and
 // 3 only), then lane 31's into rows 2 and 3 (row_bcast:31): lane 63 holds
 // the total, which is broadcast.  Only lane 63's value is read, and every
 // value it depends on comes from a row the step wrote, so the rows the masks
-// leave undefined never reach it.
-__device__ __forceinline__ double wave_sum(double v)
+// leave undefined never reach it.  One step of that for N values:
+template <int CTRL, int ROW_MASK, int N>
+__device__ __forceinline__ void wave_sum_step(double (&v)[N])
 {
-    v += dpp_d<0x111, 0xf>(v);
-    v += dpp_d<0x112, 0xf>(v);
-    v += dpp_d<0x114, 0xf>(v);
-    v += dpp_d<0x118, 0xf>(v);
-    v += dpp_d<0x142, 0xa>(v);
-    v += dpp_d<0x143, 0xc>(v);
-    return bcast(v, kWave - 1);
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] += dpp_d<CTRL, ROW_MASK>(v[u]);
+}
+
+// N such sums at once, each step taken for all N before the next, so the N
+// dependent chains interleave (each step's DPP moves must wait for the add
+// before them).
+template <int N>
+__device__ __forceinline__ void wave_sum_n(double (&v)[N])
+{
+    wave_sum_step<0x111, 0xf>(v);
+    wave_sum_step<0x112, 0xf>(v);
+    wave_sum_step<0x114, 0xf>(v);
+    wave_sum_step<0x118, 0xf>(v);
+    wave_sum_step<0x142, 0xa>(v);
+    wave_sum_step<0x143, 0xc>(v);
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] = bcast(v[u], kWave - 1);
 }
 
 template <int KR>
@@ -1511,7 +1523,8 @@ __global__ void __launch_bounds__(kWave)
     if (lane < KR) ring[lane][kRing] = 0.0;  // the pads' slot
     // a group's row data: entry `lane` of each of its kPartPre rows, and, in
     // lane r, row r's b values, reciprocal diagonal, near values and
-    // overflow start (lane kPartPre: the end)
+    // overflow start (lane kPartPre: the end).  Lanes with nothing to load
+    // are masked off (loading a clamped entry instead was slower).
     int nc[kPartPre];
     double nv[kPartPre], nb[KR], nd = 1.0, nw1 = 0.0, nw2 = 0.0;
     int no = 0;
@@ -1578,9 +1591,9 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
                 for (int j = 0; j < KR; ++j) rv[j] = *(const double *)((const char *)ring[j] + cc[r]);
             };
-            // S of row r from its ring values (plus its overflow entries)
-            auto row_sum = [&](int r, const double (&rv)[KR], double (&s)[KR]) {
-                double prod[KR];
+            // the lane products of row r's S, from its ring values (plus its
+            // overflow entries), into prod[0 .. KR)
+            auto row_prod = [&](int r, const double (&rv)[KR], double *prod) {
 #pragma unroll
                 for (int j = 0; j < KR; ++j) prod[j] = cv[r] * rv[j];
                 if constexpr (!FAST) {
@@ -1593,8 +1606,20 @@ __global__ void __launch_bounds__(kWave)
                             prod[j] = __builtin_fma(w, ring[j][(c - start) & (kRing - 1)], prod[j]);
                     }
                 }
+            };
+            // S of rows r and r + 1 (all their right-hand sides), the 2 KR
+            // sums interleaved
+            auto pair_sums = [&](int r, const double (&rva)[KR], const double (&rvb)[KR], double (&sa)[KR],
+                                 double (&sb)[KR]) {
+                double prod[2 * KR];
+                row_prod(r, rva, prod);
+                row_prod(r + 1, rvb, prod + KR);
+                wave_sum_n<2 * KR>(prod);
 #pragma unroll
-                for (int j = 0; j < KR; ++j) s[j] = wave_sum(prod[j]);
+                for (int j = 0; j < KR; ++j) {
+                    sa[j] = prod[j];
+                    sb[j] = prod[KR + j];
+                }
             };
             // the sums of the group's first two rows (columns below g0 - 1)
             double s0[KR], s1[KR], rv0[KR], rv1[KR];
@@ -1602,22 +1627,15 @@ __global__ void __launch_bounds__(kWave)
             ring_vals(1, rv1);
             if (g0 + kPartPre < end) fetch(g0 + kPartPre);
             if (g0 > start) store_group(g0 - kPartPre);
-            row_sum(0, rv0, s0);
-            row_sum(1, rv1, s1);
-#pragma unroll
-            for (int r = 0; r < kPartPre; ++r) {
+            pair_sums(0, rv0, rv1, s0, s1);
+            // y of row r from its sum s
+            auto finish_row = [&](int r, const double (&s)[KR]) {
                 const int i = g0 + r;
-                if constexpr (!FAST) {
-                    if (i >= end) continue;  // uniform: the block's last group
-                }
-                // row r + 2's ring values (columns <= i - 1) before y_i is written
-                double rv2[KR];
-                if (r + 2 < kPartPre) ring_vals(r + 2, rv2);
                 const double di = rowd[r][KR], w1 = rowd[r][KR + 1], w2 = rowd[r][KR + 2];
                 double mine = 0.0;  // lanes wj = j (lane mod KR) keep right-hand side j's value
 #pragma unroll
                 for (int j = 0; j < KR; ++j) {
-                    double t = rowd[r][j] - s0[j];
+                    double t = rowd[r][j] - s[j];
                     t = __builtin_fma(-w2, y2[j], t);
                     t = __builtin_fma(-w1, y1[j], t);
                     const double yi = t * di;
@@ -1628,9 +1646,25 @@ __global__ void __launch_bounds__(kWave)
                 // every lane writes (the lanes of one RHS the same value to
                 // the same slot), so no lane mask splits the rows' code
                 ring[wj][(i - start) & (kRing - 1)] = mine;
+            };
+            // Rows in pairs: row r + 2's ring values (columns <= i - 1) are
+            // read before y_i is written, row r + 3's (columns <= i) before
+            // y_{i+1}; then both sums run, two independent DPP chains.
+            static_assert(kPartPre % 2 == 0, "rows go in pairs");
 #pragma unroll
-                for (int j = 0; j < KR; ++j) s0[j] = s1[j];
-                if (r + 2 < kPartPre) row_sum(r + 2, rv2, s1);
+            for (int r = 0; r < kPartPre; r += 2) {
+                if constexpr (!FAST) {
+                    if (g0 + r >= end) continue;  // uniform: the block's last group
+                }
+                double rv2[KR], rv3[KR];
+                if (r + 2 < kPartPre) ring_vals(r + 2, rv2);
+                finish_row(r, s0);
+                if constexpr (!FAST) {
+                    if (g0 + r + 1 >= end) continue;
+                }
+                if (r + 2 < kPartPre) ring_vals(r + 3, rv3);
+                finish_row(r + 1, s1);
+                if (r + 2 < kPartPre) pair_sums(r + 2, rv2, rv3, s0, s1);
             }
         };
         if (fast)
