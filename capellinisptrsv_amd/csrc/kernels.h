@@ -1753,6 +1753,35 @@ __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const 
     for (int j = 0; j < KQ; ++j) out[j] = (d[j][0] + d[j][1]) + (d[j][2] + d[j][3]);
 }
 
+// The part of y_j = M v_j that columns k0 .. k0 + KLEN - 1 of M contribute,
+// for row c (four partial sums by index mod 4, joined as in part_matvec).
+template <int KQ, int KLEN>
+__device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, const double (*sv)[kPartW], int c, int k0,
+                                                 double (&out)[KQ])
+{
+    double d[KQ][4];
+#pragma unroll
+    for (int j = 0; j < KQ; ++j)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d[j][u] = 0.0;
+#pragma unroll 4
+    for (int kk = k0; kk < k0 + KLEN; kk += 4) {
+        double mv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mv[u] = M[(long long)(kk + u) * kPartW + c];
+#pragma unroll
+        for (int j = 0; j < KQ; ++j)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[j][u] = __builtin_fma(mv[u], sv[j][kk + u], d[j][u]);
+    }
+#pragma unroll
+    for (int j = 0; j < KQ; ++j) out[j] = (d[j][0] + d[j][1]) + (d[j][2] + d[j][3]);
+}
+
+// The scan kernels that split a product's columns into kPartSplit quarters
+// (one thread per row and quarter) and join the partial sums in LDS.
+constexpr int kPartSplit = 4;
+
 // The scan's vectors hold k right-hand sides: tail p of RHS q at (p * k + q)
 // * kPartW.  A workgroup takes the KQ right-hand sides from blockIdx.y * KQ;
 // part_vec gives the j-th of them.
@@ -1762,39 +1791,54 @@ __device__ __forceinline__ long long part_vec(int p, int k, int j)
     return ((long long)p * k + (long long)blockIdx.y * KQ + j) * kPartW;
 }
 
-// Level 1 of the tail scan, one workgroup of kPartW threads per group of
-// `gs` consecutive tails and chunk of KQ right-hand sides: u_first =
-// a_first, u_p = a_p + B_p u_{p-1} (B_p at index p - 1 of bm).  The group's
-// last u also goes to carry[g].
+// Level 1 of the tail scan, one workgroup of kPartSplit x kPartW threads per
+// group of `gs` consecutive tails and chunk of KQ right-hand sides: u_first
+// = a_first, u_p = a_p + B_p u_{p-1} (B_p at index p - 1 of bm).  The group's
+// last u also goes to carry[g].  A step is sequential, so each product is
+// split over the columns (thread: row c, quarter kq) to have four times the
+// loads in flight; the first kPartW threads own the vectors.
 template <int KQ>
-__global__ void __launch_bounds__(kPartW)
+__global__ void __launch_bounds__(kPartSplit * kPartW)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
                       const double *__restrict__ bm, int gs, int q_count, int k)
 {
-    const int g = (int)blockIdx.x, c = (int)threadIdx.x;
+    constexpr int kLen = kPartW / kPartSplit;
+    const int g = (int)blockIdx.x, t = (int)threadIdx.x, c = t % kPartW, kq = t / kPartW;
     const int p0 = g * gs, p1 = min(q_count, p0 + gs);
     __shared__ double sv[KQ][kPartW];
+    __shared__ double red[kPartSplit][KQ][kPartW];
     double v[KQ];
-#pragma unroll
-    for (int j = 0; j < KQ; ++j) {
-        v[j] = a[part_vec<KQ>(p0, k, j) + c];
-        u[part_vec<KQ>(p0, k, j) + c] = v[j];
-    }
-    for (int p = p0 + 1; p < p1; ++p) {
-        __syncthreads();  // the previous matrix-vector product is done with sv
-#pragma unroll
-        for (int j = 0; j < KQ; ++j) sv[j][c] = v[j];
-        __syncthreads();
-        double mv[KQ];
-        part_matvec<KQ>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, mv);
+    if (t < kPartW) {
 #pragma unroll
         for (int j = 0; j < KQ; ++j) {
-            v[j] = a[part_vec<KQ>(p, k, j) + c] + mv[j];
-            u[part_vec<KQ>(p, k, j) + c] = v[j];
+            v[j] = a[part_vec<KQ>(p0, k, j) + c];
+            u[part_vec<KQ>(p0, k, j) + c] = v[j];
         }
     }
+    for (int p = p0 + 1; p < p1; ++p) {
+        __syncthreads();  // the previous product is done with sv and red
+        if (t < kPartW) {
 #pragma unroll
-    for (int j = 0; j < KQ; ++j) carry[part_vec<KQ>(g, k, j) + c] = v[j];
+            for (int j = 0; j < KQ; ++j) sv[j][c] = v[j];
+        }
+        __syncthreads();
+        double mv[KQ];
+        part_matvec_cols<KQ, kLen>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, kq * kLen, mv);
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) red[kq][j][c] = mv[j];
+        __syncthreads();
+        if (t < kPartW) {
+#pragma unroll
+            for (int j = 0; j < KQ; ++j) {
+                v[j] = a[part_vec<KQ>(p, k, j) + c] + ((red[0][j][c] + red[1][j][c]) + (red[2][j][c] + red[3][j][c]));
+                u[part_vec<KQ>(p, k, j) + c] = v[j];
+            }
+        }
+    }
+    if (t < kPartW) {
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) carry[part_vec<KQ>(g, k, j) + c] = v[j];
+    }
 }
 
 // Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
@@ -1836,30 +1880,40 @@ __global__ void __launch_bounds__(kBlock)
     a[g] = y[((long long)(p + 1) * m - kPartW + c) * k + q];
 }
 
-// One round of the log-depth scan over the group carries (one workgroup of
-// kPartW threads per entry and chunk of KQ right-hand sides): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
+// One round of the log-depth scan over the group carries (kPartSplit
+// workgroups of kPartW threads per entry and chunk of KQ right-hand sides,
+// each for a quarter of the rows: the round has only G entries, so each
+// product is spread over four workgroups; thread: row, quarter of the
+// columns, partial sums joined in LDS): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
 // index g - s of the round's matrices), dst_g = src_g below s.
 template <int KQ>
 __global__ void __launch_bounds__(kPartW)
     k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s, int k)
 {
-    const int p = (int)blockIdx.x;
-    const int c = (int)threadIdx.x;
+    constexpr int kRows = kPartW / kPartSplit, kLen = kPartW / kPartSplit;
+    const int p = (int)blockIdx.x / kPartSplit, part = (int)blockIdx.x % kPartSplit;
+    const int t = (int)threadIdx.x, r = t % kRows, kq = t / kRows, c = part * kRows + r;
     __shared__ double sv[KQ][kPartW];
-    double acc[KQ];
-#pragma unroll
-    for (int j = 0; j < KQ; ++j) acc[j] = src[part_vec<KQ>(p, k, j) + c];
+    __shared__ double red[kPartSplit][KQ][kRows];
     if (p >= s) {  // uniform in the workgroup
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) sv[j][c] = src[part_vec<KQ>(p - s, k, j) + c];
+        for (int j = 0; j < KQ; ++j) sv[j][t] = src[part_vec<KQ>(p - s, k, j) + t];
         __syncthreads();
         double mv[KQ];
-        part_matvec<KQ>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, mv);
+        part_matvec_cols<KQ, kLen>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, kq * kLen, mv);
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) acc[j] += mv[j];
+        for (int j = 0; j < KQ; ++j) red[kq][j][r] = mv[j];
+        __syncthreads();
+        if (t < kRows) {
+#pragma unroll
+            for (int j = 0; j < KQ; ++j)
+                dst[part_vec<KQ>(p, k, j) + c] = src[part_vec<KQ>(p, k, j) + c] +
+                                                 ((red[0][j][r] + red[1][j][r]) + (red[2][j][r] + red[3][j][r]));
+        }
+    } else if (t < kRows) {
+#pragma unroll
+        for (int j = 0; j < KQ; ++j) dst[part_vec<KQ>(p, k, j) + c] = src[part_vec<KQ>(p, k, j) + c];
     }
-#pragma unroll
-    for (int j = 0; j < KQ; ++j) dst[part_vec<KQ>(p, k, j) + c] = acc[j];
 }
 
 // x = y - V t_{p-1} in blocks p >= 1 (t: the scan's result), x = y in

@@ -490,11 +490,12 @@ double *launch_part_carries(cap_plan p, int k, hipStream_t st)
     const unsigned chunks = (unsigned)(k / KQ);
     const long long mat = (long long)kPartW * kPartW;
     k_part_tails<<<blocks_for((long long)Q * k * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, k, ps->a);
-    k_part_group_scan<KQ><<<dim3(ps->G, chunks), kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs, Q, k);
+    k_part_group_scan<KQ><<<dim3(ps->G, chunks), kPartSplit * kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs,
+                                                                              Q, k);
     double *cur = ps->c0, *nxt = ps->c1;
     for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<KQ><<<dim3(ps->G, chunks), kPartW, 0, st>>>(cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat,
-                                                                1 << r, k);
+        k_part_scan<KQ><<<dim3(ps->G * kPartSplit, chunks), kPartW, 0, st>>>(
+            cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r, k);
         std::swap(cur, nxt);
     }
     return cur;
