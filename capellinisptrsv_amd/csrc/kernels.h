@@ -1508,8 +1508,8 @@ template <int KR>
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
-                 const double2 *__restrict__ near, const double *__restrict__ b, double *__restrict__ y, int n, int m,
-                 int ew, int k)
+                 const double2 *__restrict__ near, const double *__restrict__ b, double *__restrict__ y,
+                 double *__restrict__ tails, int n, int m, int ew, int k, int q_count)
 {
     constexpr int kRing = kPartRing;
     constexpr int kRow = KR + 3;  // a row's scalars in LDS: b per RHS, 1 / diagonal, near (i-1), near (i-2)
@@ -1673,6 +1673,17 @@ __global__ void __launch_bounds__(kWave)
             group(std::false_type{});
     }
     store_group(start + (end - 1 - start) / kPartPre * kPartPre);  // the last group
+    // a_p = the block's last kPartW values, the tail scan's input (blocks
+    // p < q_count, which are whole: m >= kPartW rows), at (p * k + q) *
+    // kPartW + c, from the ring
+    if ((int)blockIdx.x < q_count) {
+#pragma unroll
+        for (int c = lane; c < kPartW; c += kWave) {
+            const int slot = (end - kPartW + c - start) & (kRing - 1);
+#pragma unroll
+            for (int j = 0; j < KR; ++j) tails[((long long)blockIdx.x * k + q0 + j) * kPartW + c] = ring[j][slot];
+        }
+    }
 }
 
 // Plan-time matrix products C = A * B of the partition's tables, one per
@@ -1867,18 +1878,6 @@ __global__ void __launch_bounds__(kPartW)
     for (int j = 0; j < KQ; ++j) t[part_vec<KQ>(p, k, j) + c] = v[j];
 }
 
-// a_p = the last kPartW values of y in block p (p < q_count), per RHS: the
-// scan's input, (p * k + q) * kPartW + c
-__global__ void __launch_bounds__(kBlock)
-    k_part_tails(const double *__restrict__ y, int m, int q_count, int k, double *__restrict__ a)
-{
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)q_count * k * kPartW) return;
-    const int c = (int)(g % kPartW);
-    const long long pq = g / kPartW;
-    const int p = (int)(pq / k), q = (int)(pq % k);
-    a[g] = y[((long long)(p + 1) * m - kPartW + c) * k + q];
-}
 
 // One round of the log-depth scan over the group carries (kPartSplit
 // workgroups of kPartW threads per entry and chunk of KQ right-hand sides,

@@ -470,8 +470,8 @@ void launch_part_block(cap_plan p, const double *b, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     k_part_block<KR><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
-        ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, (const double2 *)ps->near, b, ps->y, p->n,
-        ps->m, ps->ew, k);
+        ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, (const double2 *)ps->near, b, ps->y, ps->a,
+        p->n, ps->m, ps->ew, k, ps->Q);
 }
 
 // Right-hand sides per workgroup of the group scan and the carry rounds, at
@@ -489,7 +489,6 @@ double *launch_part_carries(cap_plan p, int k, hipStream_t st)
     const int Q = ps->Q;
     const unsigned chunks = (unsigned)(k / KQ);
     const long long mat = (long long)kPartW * kPartW;
-    k_part_tails<<<blocks_for((long long)Q * k * kPartW), kBlock, 0, st>>>(ps->y, ps->m, Q, k, ps->a);
     k_part_group_scan<KQ><<<dim3(ps->G, chunks), kPartSplit * kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs,
                                                                               Q, k);
     double *cur = ps->c0, *nxt = ps->c1;
