@@ -454,6 +454,10 @@ int ensure_part_rhs(cap_plan p, int k)
 
 // Right-hand sides per wavefront of the block solve, at most (they share the
 // row data; each adds an independent reduction chain and 2 KB of LDS ring).
+// Below that cap a solve takes k / 2, so that the block solve runs about two
+// wavefronts per SIMD (P = 1024 blocks): on the 500 k deep band, k = 4 took
+// 0.417 ms at 2 per wavefront against 0.441 at 4, and k = 16 0.975 ms at 4
+// against 1.209 at 2.
 constexpr int kPartBlockRhs = 4;
 
 // The largest power of two up to `cap` that divides k: the chunk of
@@ -527,7 +531,7 @@ int launch_partition(cap_plan p, const double *b, double *x, int k, hipStream_t 
     PartState *ps = p->part;
     if (!ps || k < 1 || k > kWave || ps->m < kPartW) return CAP_ERR_ARG;
     CAP_TRY(ensure_part_rhs(p, k));
-    switch (part_chunk(k, kPartBlockRhs)) {
+    switch (part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)))) {
     case 4: launch_part_block<4>(p, b, k, st); break;
     case 2: launch_part_block<2>(p, b, k, st); break;
     default: launch_part_block<1>(p, b, k, st); break;
