@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline workload only (skip the extra workloads)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the solve is a single-GPU kernel and this build runs on one MI355X "
+        print("bench.py measures one GPU: the solve is a single-GPU kernel, measured on one MI355X "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 

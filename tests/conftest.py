@@ -15,7 +15,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def cuda():
-    """GPU tests fail (not skip) when no GPU is visible: they only run with -m gpu on a GPU box."""
+    """GPU tests fail (not skip) when no GPU is visible: they are meant to run with -m gpu on a machine with one."""
     import torch
     assert torch.cuda.is_available(), "GPU test collected but no GPU is visible"
     import capellinisptrsv_amd as cap

@@ -1,7 +1,7 @@
 """GPU FP32-solve + FP64-refinement and batched multi-matrix / multi-RHS solves.
 
 Refinement bar: relative max-norm error <= 1e-12 against the FP64 oracle
-(north_star's FP32+refinement target).  Batched bar: each matrix's k
+(the FP32-solve + FP64-refinement accuracy target).  Batched bar: each matrix's k
 right-hand sides bit-exact against the oracle's multi-RHS forward
 substitution on that matrix (the same sequential fused order).
 """

@@ -14,7 +14,7 @@ once.  Two loop shapes are simulated:
 
 The uniform shape must always finish with the oracle's values; the sunk
 shape must deadlock as soon as a row depends on another row of its own
-wavefront (the bug this build's kernels are structured against).
+wavefront (the bug the library's kernels are structured against).
 """
 import numpy as np
 import pytest
