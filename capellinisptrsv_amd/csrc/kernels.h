@@ -1391,16 +1391,21 @@ __global__ void __launch_bounds__(kBlock)
 // every tail is fixed up with its in-group product W_p = B_p ... B_first:
 // t_p = u_p + W_p T_{g-1}.  The matrix products are formed once per plan
 // (k_part_gemm); a solve only touches vectors.  The sums are reassociated,
-// so results match the sequential solve to rounding, not bit for bit.  Matrices are column-major kPartW x kPartW (element (c, k) at
-// k * kPartW + c); V is two halves of 64 interleaved columns (n x 64 each).
+// so results match the sequential solve to rounding, not bit for bit.
+// Matrices are column-major kPartW x kPartW (element (c, k) at k * kPartW +
+// c); V is two halves of 64 interleaved columns (n x 64 each).  The tables
+// and the solve's work vectors are FP64 for FP32 plans too: b is widened
+// when the block kernel reads it and x rounded once when the finish writes
+// it.
 constexpr int kPartW = 128;
 
 // Coupling right-hand sides (two zeroed halves of 64 interleaved columns):
 // an entry of row i in block p >= 1 whose column c lies before the block
 // goes to column q = c - (pm - kPartW); duplicated entries add up.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    k_part_coupling(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val, int n,
-                    int m, double *__restrict__ r0, double *__restrict__ r1)
+    k_part_coupling(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val, int n, int m,
+                    double *__restrict__ r0, double *__restrict__ r1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1412,7 +1417,7 @@ __global__ void __launch_bounds__(kBlock)
         const int q = c - base;
         if (c >= start || q < 0) continue;  // inside the block; q < 0 cannot occur (bandwidth checked on the host)
         double *dst = q < 64 ? r0 + (long long)i * 64 + q : r1 + (long long)i * 64 + (q - 64);
-        *dst += val[j];
+        *dst += (double)val[j];
     }
 }
 
@@ -1504,11 +1509,11 @@ __device__ __forceinline__ void wave_sum_n(double (&v)[N])
     for (int u = 0; u < N; ++u) v[u] = bcast(v[u], kWave - 1);
 }
 
-template <int KR>
+template <int KR, typename T>
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
-                 const double2 *__restrict__ near, const double *__restrict__ b, double *__restrict__ y,
+                 const double2 *__restrict__ near, const T *__restrict__ b, double *__restrict__ y,
                  double *__restrict__ tails, int n, int m, int ew, int k, int q_count)
 {
     constexpr int kRing = kPartRing;
@@ -1539,7 +1544,7 @@ __global__ void __launch_bounds__(kWave)
         const int row = min(g0 + lane, end);
         const bool mine = lane < kPartPre && row < end;
 #pragma unroll
-        for (int j = 0; j < KR; ++j) nb[j] = mine ? b[(long long)row * k + q0 + j] : 0.0;
+        for (int j = 0; j < KR; ++j) nb[j] = mine ? (double)b[(long long)row * k + q0 + j] : 0.0;
         nd = mine ? inv_diag[row] : 1.0;
         const double2 w = mine ? near[row] : make_double2(0.0, 0.0);
         nw1 = w.x;
@@ -1919,9 +1924,10 @@ __global__ void __launch_bounds__(kPartW)
 // block 0.  One thread per (row, RHS), g = row * k + q: V column-major, so
 // for each column the lanes read consecutive rows (the k lanes of one row
 // share an element).
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_part_finish(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t, int n,
-                  int m, int k, double *__restrict__ x)
+                  int m, int k, T *__restrict__ x)
 {
     const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= (long long)n * k) return;
@@ -1937,7 +1943,7 @@ __global__ void __launch_bounds__(kBlock)
             for (int u = 0; u < 4; ++u) s[u] = __builtin_fma(vt[(long long)(c + u) * n + i], tp[c + u], s[u]);
         xi -= (s[0] + s[1]) + (s[2] + s[3]);
     }
-    x[gi] = xi;
+    x[gi] = (T)xi;
 }
 
 // The finish for chunks of KQ >= 2 right-hand sides: a workgroup takes
@@ -1946,10 +1952,10 @@ __global__ void __launch_bounds__(kBlock)
 // read as broadcasts), and each thread reads its row of V once for all KQ.
 // The sums run in the single-RHS kernel's order (four partial sums by
 // column mod 4), so each RHS gets that kernel's bits.
-template <int KQ>
+template <int KQ, typename T>
 __global__ void __launch_bounds__(kPartW)
     k_part_finish_multi(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t,
-                        int n, int m, int k, double *__restrict__ x)
+                        int n, int m, int k, T *__restrict__ x)
 {
     __shared__ double st[2][kPartW][KQ];
     const int lo = (int)blockIdx.x * kPartW;
@@ -1987,7 +1993,7 @@ __global__ void __launch_bounds__(kPartW)
         const long long g = (long long)i * k + q0 + j;
         double xi = y[g];
         if (p > 0) xi -= (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
-        x[g] = xi;
+        x[g] = (T)xi;
     }
 }
 

@@ -330,11 +330,7 @@ template <typename T>
 int launch_prepare(cap_plan p, int algo, int sync, void *x, long long count, hipStream_t st)
 {
     typedef typename VBits<T>::U U;
-    if (algo == CAP_ALGO_PARTITION) {
-        // the partitioned solve keeps no readiness state (its tables are FP64)
-        if (sizeof(T) == 8) return CAP_OK;
-        algo = CAP_ALGO_LOCAL;
-    }
+    if (algo == CAP_ALGO_PARTITION) return CAP_OK;  // the partitioned solve keeps no readiness state
     if (algo == CAP_ALGO_LOCAL) sync = CAP_SYNC_VALUE;
     if (sync == CAP_SYNC_VALUE) {
         void *target = x;
@@ -469,11 +465,11 @@ inline int part_chunk(int k, int cap)
     return c;
 }
 
-template <int KR>
-void launch_part_block(cap_plan p, const double *b, int k, hipStream_t st)
+template <int KR, typename T>
+void launch_part_block(cap_plan p, const T *b, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
-    k_part_block<KR><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
+    k_part_block<KR, T><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
         ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, (const double2 *)ps->near, b, ps->y, ps->a,
         p->n, ps->m, ps->ew, k, ps->Q);
 }
@@ -506,17 +502,17 @@ double *launch_part_carries(cap_plan p, int k, hipStream_t st)
 
 // The fix-up of every tail and the finish, KQ right-hand sides per
 // workgroup (KQ | k); `tg`: the scanned group carries.
-template <int KQ>
-void launch_part_fix(cap_plan p, const double *tg, double *x, int k, hipStream_t st)
+template <int KQ, typename T>
+void launch_part_fix(cap_plan p, const double *tg, T *x, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     const unsigned chunks = (unsigned)(k / KQ);
     const long long mat = (long long)kPartW * kPartW;
     k_part_fixup<KQ><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs, ps->t, k);
     if constexpr (KQ == 1)
-        k_part_finish<<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+        k_part_finish<T><<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
     else  // m >= kPartW: a workgroup's kPartW rows span at most two blocks
-        k_part_finish_multi<KQ><<<dim3((unsigned)((p->n + kPartW - 1) / kPartW), chunks), kPartW, 0, st>>>(
+        k_part_finish_multi<KQ, T><<<dim3((unsigned)((p->n + kPartW - 1) / kPartW), chunks), kPartW, 0, st>>>(
             ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
 }
 
@@ -526,15 +522,16 @@ void launch_part_fix(cap_plan p, const double *tg, double *x, int k, hipStream_t
 // over the group carries, then the fix-up; grids over tails x chunks), then
 // x = y - V t.  Every step is on `st`; none of them waits on another
 // workgroup.  Each right-hand side gets the bits of its own one-RHS solve.
-int launch_partition(cap_plan p, const double *b, double *x, int k, hipStream_t st)
+template <typename T>
+int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
 {
     PartState *ps = p->part;
     if (!ps || k < 1 || k > kWave || ps->m < kPartW) return CAP_ERR_ARG;
     CAP_TRY(ensure_part_rhs(p, k));
     switch (part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)))) {
-    case 4: launch_part_block<4>(p, b, k, st); break;
-    case 2: launch_part_block<2>(p, b, k, st); break;
-    default: launch_part_block<1>(p, b, k, st); break;
+    case 4: launch_part_block<4, T>(p, b, k, st); break;
+    case 2: launch_part_block<2, T>(p, b, k, st); break;
+    default: launch_part_block<1, T>(p, b, k, st); break;
     }
     const double *tg = nullptr;
     switch (part_chunk(k, kPartScanRhs)) {
@@ -543,11 +540,11 @@ int launch_partition(cap_plan p, const double *b, double *x, int k, hipStream_t 
     default: tg = launch_part_carries<1>(p, k, st); break;
     }
     switch (part_chunk(k, 16)) {
-    case 16: launch_part_fix<16>(p, tg, x, k, st); break;
-    case 8: launch_part_fix<8>(p, tg, x, k, st); break;
-    case 4: launch_part_fix<4>(p, tg, x, k, st); break;
-    case 2: launch_part_fix<2>(p, tg, x, k, st); break;
-    default: launch_part_fix<1>(p, tg, x, k, st); break;
+    case 16: launch_part_fix<16, T>(p, tg, x, k, st); break;
+    case 8: launch_part_fix<8, T>(p, tg, x, k, st); break;
+    case 4: launch_part_fix<4, T>(p, tg, x, k, st); break;
+    case 2: launch_part_fix<2, T>(p, tg, x, k, st); break;
+    default: launch_part_fix<1, T>(p, tg, x, k, st); break;
     }
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
@@ -558,10 +555,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
 {
     const int n = p->n;
     if (n == 0) return CAP_OK;
-    if (algo == CAP_ALGO_PARTITION) {
-        if constexpr (sizeof(T) == 8) return launch_partition(p, b, x, rhs, st);
-        algo = CAP_ALGO_LOCAL;  // the partition's tables are FP64
-    }
+    if (algo == CAP_ALGO_PARTITION) return launch_partition<T>(p, b, x, rhs, st);
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
     if (algo == CAP_ALGO_WRTFST) {
@@ -817,7 +811,6 @@ int build_partition(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
     if (p->part) return CAP_OK;
-    if (p->vt != CAP_F64) return CAP_ERR_UNSUPPORTED;
     const int n = p->n, nnz = p->nnz;
     const int m = std::max(kPartW, (n + kPartBlocks - 1) / kPartBlocks);
     const int P = n > 0 ? (n + m - 1) / m : 0;
@@ -853,8 +846,14 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
                              12.0 * nnz;
         if (bytes > kPartMaxBytes) return CAP_ERR_UNSUPPORTED;
     }
-    std::vector<double> v((size_t)nnz);
-    CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
+    std::vector<double> v((size_t)nnz);  // FP32 values widen exactly
+    if (p->vt == CAP_F64) {
+        CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> vf((size_t)nnz);
+        CAP_HIP_TRY(hipMemcpy(vf.data(), p->val, sizeof(float) * (size_t)nnz, hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < vf.size(); ++e) v[e] = (double)vf[e];
+    }
     // the block-diagonal part: in-block off-diagonals in columns i-1 and i-2
     // summed into the near table (two per row), the others in stored order,
     // the first ew of a row in its ELL slab (as the byte offset of the
@@ -986,7 +985,10 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
             rc = CAP_ERR_HIP;
             break;
         }
-        k_part_coupling<<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, n, m, r0, r1);
+        if (p->vt == CAP_F64)
+            k_part_coupling<double><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, n, m, r0, r1);
+        else
+            k_part_coupling<float><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const float *)p->val, n, m, r0, r1);
         if (hipGetLastError() != hipSuccess) {
             rc = CAP_ERR_HIP;
             break;

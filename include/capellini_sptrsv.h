@@ -59,11 +59,12 @@ extern "C" {
 #define CAP_ALGO_LOCAL 5   /* one workgroup per matrix, wavefront per row, dependencies
                               resolved inside one compute unit: deep matrices
                               (few rows per level); value protocol on x itself */
-#define CAP_ALGO_PARTITION 6 /* FP64, lower bandwidth <= 128: row blocks solved
+#define CAP_ALGO_PARTITION 6 /* lower bandwidth <= 128: row blocks solved
                               independently, joined by a scan over the 128-value
-                              block tails (tables built at plan creation); sums
-                              are reassociated (within rounding of the
-                              sequential solve, not bit-exact).  Up to 64
+                              block tails (FP64 tables built at plan creation;
+                              FP32 plans widen b and round x once); sums are
+                              reassociated (within rounding of the sequential
+                              solve, not bit-exact).  Up to 64
                               right-hand sides per solve (its work vectors
                               grow on the first solve with a larger count,
                               which waits for the device); the refined solve

@@ -9,8 +9,10 @@ bits each time), rows longer than the 64 entries of an ELL slab, several
 right-hand sides (1 to 64, interleaved; each column against its own
 sequential solve, and the work vectors growing and shrinking between
 solves), refinement (FP64 partitioned solves and one correction, one and
-several right-hand sides), and the refusals (bandwidth above 128, FP32, too
-few rows for two blocks).
+several right-hand sides), FP32 plans (FP64 tables, b widened and x rounded
+once: within the FP32 tolerance, 1e-5, of the oracle's FP32 sequential
+solve, and within FP32 rounding of the FP64 solve of the same FP32 system),
+and the refusals (bandwidth above 128, too few rows for two blocks).
 """
 import numpy as np
 import pytest
@@ -22,6 +24,7 @@ from conftest import random_lower, rel_maxnorm
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
+FP32_TOL = 1e-5
 
 
 def _chain(n):
@@ -132,14 +135,32 @@ def test_partition_refinement(cuda, k):
     plan.close()
 
 
+@pytest.mark.parametrize("name,mat", [CASES[0], CASES[3], CASES[6]], ids=[CASES[i][0] for i in (0, 3, 6)])
+@pytest.mark.parametrize("k", [1, 4])
+def test_partition_fp32(cuda, name, mat, k):
+    import torch
+    rp, ci, v = mat
+    n = len(rp) - 1
+    v32 = v.astype(np.float32)
+    b32 = api.uniform(35, n * k).astype(np.float32)
+    plan = _plan(cuda, rp, ci, v32, algo="partition")
+    assert plan.info()["algo"] == "partition"
+    x = plan.solve(torch.from_numpy(b32.reshape(n, k) if k > 1 else b32).to(cuda)).cpu().numpy().reshape(n, k)
+    assert plan.status() == 0 and x.dtype == np.float32
+    ref32 = oracle.csr_forward(rp, ci, v32, b32, rhs=k, fused=True).reshape(n, k)
+    # the same FP32 system solved in FP64: the partition is that, rounded once
+    ref64 = oracle.csr_forward(rp, ci, v32.astype(np.float64), b32.astype(np.float64), rhs=k, fused=True).reshape(n, k)
+    for q in range(k):
+        assert rel_maxnorm(x[:, q], ref32[:, q]) <= FP32_TOL, (q, rel_maxnorm(x[:, q], ref32[:, q]))
+        assert rel_maxnorm(x[:, q].astype(np.float64), ref64[:, q]) <= 1e-7, q
+    plan.close()
+
+
 def test_partition_refusals(cuda):
     from capellinisptrsv_amd import CapError
     rp, ci, v = api.generate("band_rowcount", n=5000, offdiag=8, band=300, seed=9)
     with pytest.raises(CapError):
         _plan(cuda, rp, ci, v, algo="partition")  # bandwidth 300 > 128
-    rp, ci, v = api.generate("band_rowcount", n=5000, offdiag=8, band=64, seed=9)
-    with pytest.raises(CapError):
-        _plan(cuda, rp, ci, v.astype(np.float32), algo="partition")  # FP64 only
     rp, ci, v = api.generate("band_rowcount", n=128, offdiag=4, band=64, seed=9)
     with pytest.raises(CapError):
         _plan(cuda, rp, ci, v, algo="partition")  # one block
