@@ -90,6 +90,8 @@ SIGNATURES = {
     "sptrsv_csr_solve_s": (c_int, [c_int, c_int, PI, PI, PF, PF, PF, POINTER(cap_opts)]),
     "spts_syncfree_hip_csr_wrt_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_csr_wrt_s": (c_int, [PI, PI, PF, c_int, c_int, c_int, PD, PD, PD]),
+    "spts_syncfree_hip_csr_algo_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, c_int, PD, PD, PD]),
+    "spts_syncfree_hip_csr_algo_s": (c_int, [PI, PI, PF, c_int, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_csr_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_csr_s": (c_int, [PI, PI, PF, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_mix_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, PD, PD, PD, PD, PD, PD]),

@@ -173,12 +173,21 @@ def spts_syncfree_hip_mix(row_ptr, col_idx, val, m: int, n: int, nnzL: int):
     return _bench("spts_syncfree_hip_mix", row_ptr, col_idx, val, m, n, nnzL, mix=True)
 
 
-def _bench(name, row_ptr, col_idx, val, m, n, nnzL, mix):
+def spts_syncfree_hip_csr_algo(row_ptr, col_idx, val, m: int, n: int, nnzL: int, algo: str = "auto"):
+    """The benchmark-compatible wrapper with a chosen algorithm (auto, wrtfst,
+    twopart, csc, local, partition).  Returns (ret, solve_time_ms, gflops, bandwith)."""
+    if algo not in N.ALGO or algo == "mix":
+        raise ValueError(f"algorithm {algo!r}: one of auto, wrtfst, twopart, csc, local, partition")
+    return _bench("spts_syncfree_hip_csr_algo", row_ptr, col_idx, val, m, n, nnzL, mix=False, algo=N.ALGO[algo])
+
+
+def _bench(name, row_ptr, col_idx, val, m, n, nnzL, mix, algo=None):
     val = np.ascontiguousarray(val)
     sfx = _suffix(val.dtype)
     rp, ci = _i32(row_ptr), _i32(col_idx)
     out = [c_double() for _ in range(6 if mix else 3)]
-    ret = getattr(N.lib(), f"{name}_{sfx}")(_pi(rp), _pi(ci), _pv(val), m, n, nnzL, *[byref(o) for o in out])
+    extra = [] if algo is None else [algo]
+    ret = getattr(N.lib(), f"{name}_{sfx}")(_pi(rp), _pi(ci), _pv(val), m, n, nnzL, *extra, *[byref(o) for o in out])
     return (ret, *[o.value for o in out])
 
 

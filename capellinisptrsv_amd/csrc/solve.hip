@@ -790,6 +790,8 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // afterwards) and is stored column-major; the scan's matrices are formed
 // round by round.
 constexpr int kPartBlocks = 1024;
+// largest |x - 1| accepted when the tables solve L x = L 1 at plan creation
+constexpr double kPartStableTol = 1e-10;
 constexpr int kPartGroup = 8;  // tails per group of the scan's first level
 // Largest table set a partitioned plan may hold (device; the host builds the
 // same tables once): larger matrices are refused rather than risk exhausting
@@ -1014,6 +1016,31 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     for (double *t : {r0, r1, v0, v1})
         if (t) (void)hipFree(t);
     if (d_ops) (void)hipFree(d_ops);
+    // Stability check: solve L x = L 1 (row sums in FP64) with the tables
+    // just built.  The block solves and V multiply by block inverses whose
+    // entries grow without bound when the blocks are not diagonally dominant
+    // (unit values in a band, as the reference's .mtx reader makes them);
+    // the sequential solve never forms them.  Such matrices are refused.
+    if (rc == CAP_OK) {
+        std::vector<double> rs((size_t)n, 0.0), xs((size_t)n);
+        for (int i = 0; i < n; ++i)
+            for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) rs[(size_t)i] += v[(size_t)j];
+        double *db = nullptr, *dx = nullptr;
+        if ((rc = dev_alloc(&db, (size_t)n)) == CAP_OK && (rc = dev_alloc(&dx, (size_t)n)) == CAP_OK) {
+            if (hipMemcpy(db, rs.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+                rc = CAP_ERR_HIP;
+            if (rc == CAP_OK) rc = launch_partition<double>(p, db, dx, 1, st);
+            if (rc == CAP_OK && (hipStreamSynchronize(st) != hipSuccess ||
+                                 hipMemcpy(xs.data(), dx, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost) !=
+                                     hipSuccess))
+                rc = CAP_ERR_HIP;
+        }
+        (void)hipFree(db);
+        (void)hipFree(dx);
+        double err = 0.0;
+        for (double xv : xs) err = std::max(err, std::fabs(xv - 1.0));
+        if (rc == CAP_OK && !(err <= kPartStableTol)) rc = CAP_ERR_UNSUPPORTED;  // NaN fails too
+    }
     if (rc != CAP_OK) free_partition(p);
     return rc;
 }
@@ -1035,8 +1062,11 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             apply_choice(p, it->second);
             if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, it->second.border);
             if (p->algo == CAP_ALGO_LOCAL) return ensure_segments(p, nullptr);
-            if (p->algo == CAP_ALGO_PARTITION) return build_partition(p, rp_host, st);
-            return CAP_OK;
+            if (p->algo != CAP_ALGO_PARTITION) return CAP_OK;
+            // the pick was cached for this structure; these values may not
+            // keep the partition stable, and then the plan is tuned afresh
+            const int rc = build_partition(p, rp_host, st);
+            if (rc != CAP_ERR_UNSUPPORTED) return rc;
         }
     }
     void *b = nullptr, *x = nullptr;
@@ -2215,6 +2245,18 @@ int spts_syncfree_hip_csr_wrt_s(const int *rp, const int *ci, const float *v, in
                                 double *gf, double *bw)
 {
     return bench_csr<float>(CAP_ALGO_WRTFST, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_csr_algo_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, int algo,
+                                 double *t, double *gf, double *bw)
+{
+    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_PARTITION) return CAP_ERR_ARG;
+    return bench_csr<double>(algo, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
+}
+int spts_syncfree_hip_csr_algo_s(const int *rp, const int *ci, const float *v, int m, int n, int nnzL, int algo,
+                                 double *t, double *gf, double *bw)
+{
+    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_PARTITION) return CAP_ERR_ARG;
+    return bench_csr<float>(algo, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
 }
 int spts_syncfree_hip_csr_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,
                             double *gf, double *bw)

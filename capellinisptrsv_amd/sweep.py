@@ -6,7 +6,7 @@ program on every file of a directory, each run appending one row.  The
 matrix list of the paper's collection is matrix/matrix_name.txt (one path
 per line).  This driver does the same with the gfx950 program:
 
-    python -m capellinisptrsv_amd.sweep DIR_OR_LIST [--root DIR] [-a wrtfst,mix,2part,csc]
+    python -m capellinisptrsv_amd.sweep DIR_OR_LIST [--root DIR] [-a wrtfst,mix,2part,csc,auto,local,partition]
            [--f32] [--values] [--out-dir DIR] [--keep] [--dry-run]
 
 DIR_OR_LIST is a directory (every *.mtx in it, sorted) or a text file of
@@ -25,7 +25,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ALGOS = ("wrtfst", "2part", "mix", "csc")
+ALGOS = ("wrtfst", "2part", "mix", "csc", "auto", "local", "partition")
 
 
 def matrices(source: str, root: str | None = None) -> list[str]:

@@ -257,6 +257,20 @@ int spts_syncfree_hip_mix_s(const int *csrRowPtrL, const int *csrColIdxL,
                             double *pre_time, double *warp_occupy,
                             double *element_occupy);
 
+/* No counterpart in the reference: the same benchmark-compatible wrapper
+ * (get_x_b, validation, CAP_BENCH_REPEAT timed solves, the same outputs)
+ * with the algorithm chosen by the caller: CAP_ALGO_AUTO, WRTFST, TWOPART,
+ * CSC, LOCAL or PARTITION (for MIX use the mix wrapper, which also reports
+ * the chunk table).  It lets the drop-in driver run the autotuned and the
+ * deep-matrix solvers with the same result.csv columns.  A matrix the
+ * algorithm refuses returns its CAP_ERR_* code. */
+int spts_syncfree_hip_csr_algo_d(const int *csrRowPtrL, const int *csrColIdxL,
+                                 const double *csrValL, int m, int n, int nnzL, int algo,
+                                 double *solve_time, double *gflops, double *bandwith);
+int spts_syncfree_hip_csr_algo_s(const int *csrRowPtrL, const int *csrColIdxL,
+                                 const float *csrValL, int m, int n, int nnzL, int algo,
+                                 double *solve_time, double *gflops, double *bandwith);
+
 /* replaces sptrsv_syncfree_cuda, CUDA/SyncFree_csc/spts_syncfree_cuda2.h:89-102
  * (CSC input, x output; validation = relative L1 error < 1e-4, :245-264) */
 int sptrsv_syncfree_hip_csc_d(const int *cscColPtr, const int *cscRowIdx,
@@ -377,6 +391,14 @@ static inline int spts_syncfree_hip_csr(const int *rp, const int *ci, const VALU
     return sizeof(VALUE_TYPE) == 8
                ? spts_syncfree_hip_csr_d(rp, ci, (const double *)(const void *)v, m, n, nnzL, t, gf, bw)
                : spts_syncfree_hip_csr_s(rp, ci, (const float *)(const void *)v, m, n, nnzL, t, gf, bw);
+}
+static inline int spts_syncfree_hip_csr_algo(const int *rp, const int *ci, const VALUE_TYPE *v,
+                                             int m, int n, int nnzL, int algo, double *t,
+                                             double *gf, double *bw)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? spts_syncfree_hip_csr_algo_d(rp, ci, (const double *)(const void *)v, m, n, nnzL, algo, t, gf, bw)
+               : spts_syncfree_hip_csr_algo_s(rp, ci, (const float *)(const void *)v, m, n, nnzL, algo, t, gf, bw);
 }
 static inline int spts_syncfree_hip_mix(const int *rp, const int *ci, const VALUE_TYPE *v,
                                         int m, int n, int nnzL, double *t, double *gf,

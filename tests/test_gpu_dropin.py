@@ -86,6 +86,30 @@ def test_driver_appends_result_csv(cuda, tmp_path):
         assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
 
 
+def test_driver_autotuned_and_deep_matrix_solvers(cuda, tmp_path):
+    """-a auto / local / partition (this library's additions to the driver):
+    the same result.csv columns as wrtfst, none failing the validation, on a
+    diagonally dominant band of 3000 rows (bandwidth 100, values kept)."""
+    mtx = tmp_path / "band.mtx"
+    rng = np.random.default_rng(2)
+    n = 3000
+    lines = ["%%MatrixMarket matrix coordinate real general", "% generated", f"{n} {n} {4 * n}"]
+    for i in range(n):
+        for c in rng.integers(max(0, i - 100), max(1, i), size=3):
+            lines.append(f"{i + 1} {int(c) + 1} {rng.uniform(-1, 1):.6f}")
+        lines.append(f"{i + 1} {i + 1} 4.0")
+    mtx.write_text("\n".join(lines) + "\n")
+    for algo in ("auto", "local", "partition"):
+        r = subprocess.run([os.path.join(ROOT, "capellinisptrsv_amd", "sptrsv_main"), str(mtx), "-a", algo, "--values"],
+                           cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (algo, r.stdout, r.stderr)
+    rows = (tmp_path / "result.csv").read_text().splitlines()
+    assert len(rows) == 6
+    for row in rows[1::2]:
+        f = [s.strip() for s in row.split(",")]
+        assert f[0] == str(mtx) and int(f[1]) == n and float(f[7]) > 0, row
+
+
 def test_sweep_over_a_matrix_directory(cuda, tmp_path):
     """The run.sh-style sweep: every valid fixture under two algorithms, one
     result.csv row (after its header line) per run, none failing validation."""

@@ -12,7 +12,8 @@ solves), refinement (FP64 partitioned solves and one correction, one and
 several right-hand sides), FP32 plans (FP64 tables, b widened and x rounded
 once: within the FP32 tolerance, 1e-5, of the oracle's FP32 sequential
 solve, and within FP32 rounding of the FP64 solve of the same FP32 system),
-and the refusals (bandwidth above 128, too few rows for two blocks).
+and the refusals (bandwidth above 128, too few rows for two blocks, blocks
+whose inverses grow: the plan-creation check that solves L x = L 1).
 """
 import numpy as np
 import pytest
@@ -164,6 +165,33 @@ def test_partition_refusals(cuda):
     rp, ci, v = api.generate("band_rowcount", n=128, offdiag=4, band=64, seed=9)
     with pytest.raises(CapError):
         _plan(cuda, rp, ci, v, algo="partition")  # one block
+
+
+def _unstable(n):
+    """Unit values in columns i-1 and i-3 and on the diagonal: the recurrence
+    x_i = -(x_{i-1} + x_{i-3}) has a root of modulus 1.47, so the block
+    inverses grow like 1.47^m while the sequential solve of L x = L 1 stays
+    exact."""
+    rp, ci = [0], []
+    for i in range(n):
+        ci += [c for c in (i - 3, i - 1) if c >= 0] + [i]
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.ones(len(ci))
+
+
+def test_partition_refuses_growing_blocks_and_auto_avoids_it(cuda):
+    import torch
+    from capellinisptrsv_amd import CapError
+    rp, ci, v = _unstable(3000)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v, algo="partition")
+    plan = _plan(cuda, rp, ci, v)  # autotuned: the partition is no candidate
+    assert plan.info()["algo"] != "partition"
+    b = api.uniform(36, len(rp) - 1)
+    x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+    assert plan.status() == 0
+    assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, fused=True))
+    plan.close()
 
 
 def test_autotuner_on_a_deep_band(cuda):
