@@ -317,6 +317,19 @@ def extra_workloads(args):
             res[name]["rhs16"] = {"ms_per_solve": round(ms_k, 4), "ms_per_rhs": round(ms_k / k, 4),
                                   "rel_err_vs_oracle": err / float(np.max(np.abs(ref)))}
         p.close()
+        if cfg == 2:
+            # the same matrix and b in FP32 (autotuned plan), against the FP32 oracle
+            v32 = v.astype(np.float32)
+            b32 = b.to(torch.float32)
+            x32 = torch.empty_like(b32)
+            p32 = Plan(d[0], d[1], torch.from_numpy(v32).to(dev))
+            ms32 = _time_gpu(lambda: p32.solve(b32, out=x32), 20, warmup=2)
+            assert p32.status() == 0
+            ref32 = oracle.csr_forward(rp, ci, v32, b32.cpu().numpy(), fused=True)
+            err32 = float(np.max(np.abs(x32.cpu().numpy().astype(np.float64) - ref32)))
+            res[name]["fp32"] = {"algo": p32.info()["algo"], "ms_per_solve": round(ms32, 4),
+                                 "rel_err_vs_fp32_oracle": err32 / float(np.max(np.abs(ref32)))}
+            p32.close()
 
     # differentiable solve: n = 1,000,000 level matrix, forward + adjoint
     log("extra workload adjoint_1m")
