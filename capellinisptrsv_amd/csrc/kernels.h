@@ -1453,9 +1453,9 @@ __global__ void __launch_bounds__(kBlock)
 // kPartEll per row (value, and the byte offset of the column's slot in the
 // LDS ring of the block's values: ((col - block start) mod kPartRing) * 8;
 // pads point at slot kPartRing, which holds 0), the rest of a longer row in
-// an overflow CSR (ov_rp per row, absolute columns).  Entry l of a row sits in lane l;
-// its products with the block's earlier values are summed by a DPP tree over
-// the lanes, and
+// an overflow CSR (ov_rp per row, absolute columns).  Entry l of a row sits
+// in lane l; its products with the block's earlier values are summed through
+// LDS (8 lanes add 8 products each, then an 8-lane butterfly), and
 //   y_i = ((b_i - S_i - w2_i y_{i-2}) - w1_i y_{i-1}) / d_i
 // (as a multiplication by the reciprocal diagonal).  S_i needs only values
 // up to y_{i-3}, so it is summed two rows ahead, off the chain from one row
@@ -1466,10 +1466,10 @@ constexpr int kPartEll = 64;
 constexpr int kPartPre = 16;
 constexpr int kPartRing = 2 * kPartW;  // ring slots (rows' values by row mod kPartRing)
 
-// A double moved between lanes by DPP (row shifts and row broadcasts of
-// the GFX9 DPP set).  Lanes whose source lies outside their row read 0
-// (bound_ctrl); lanes in rows outside ROW_MASK are left undefined (no old
-// value to set up: two moves fewer per step).
+// A double moved between lanes by DPP (the GFX9 DPP set).  Lanes whose
+// source lies outside their row read 0 (bound_ctrl); lanes in rows outside
+// ROW_MASK are left undefined (no old value to set up: two moves fewer per
+// step).
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_d(double v)
 {
@@ -1479,35 +1479,18 @@ __device__ __forceinline__ double dpp_d(double v)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// A sum over the 64 lanes, returned in every lane: prefix sums within rows of
-// 16 by row_shr 1, 2, 4, 8 (lane 15 of a row then holds the row's sum), row
-// 0's sum added into row 1 and row 2's into row 3 (row_bcast:15, rows 1 and
-// 3 only), then lane 31's into rows 2 and 3 (row_bcast:31): lane 63 holds
-// the total, which is broadcast.  Only lane 63's value is read, and every
-// value it depends on comes from a row the step wrote, so the rows the masks
-// leave undefined never reach it.  One step of that for N values:
-template <int CTRL, int ROW_MASK, int N>
-__device__ __forceinline__ void wave_sum_step(double (&v)[N])
+// The sum of v over each group of 8 lanes, in all 8 of them: butterflies by
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1] and row_half_mirror (lane i and
+// 7 - i of each half row).  Each lane adds the same two halves, in either
+// order, so all 8 hold the same bits.
+__device__ __forceinline__ double part_sum8(double v)
 {
-#pragma unroll
-    for (int u = 0; u < N; ++u) v[u] += dpp_d<CTRL, ROW_MASK>(v[u]);
+    v += dpp_d<0xB1, 0xf>(v);
+    v += dpp_d<0x4E, 0xf>(v);
+    v += dpp_d<0x141, 0xf>(v);
+    return v;
 }
 
-// N such sums at once, each step taken for all N before the next, so the N
-// dependent chains interleave (each step's DPP moves must wait for the add
-// before them).
-template <int N>
-__device__ __forceinline__ void wave_sum_n(double (&v)[N])
-{
-    wave_sum_step<0x111, 0xf>(v);
-    wave_sum_step<0x112, 0xf>(v);
-    wave_sum_step<0x114, 0xf>(v);
-    wave_sum_step<0x118, 0xf>(v);
-    wave_sum_step<0x142, 0xa>(v);
-    wave_sum_step<0x143, 0xc>(v);
-#pragma unroll
-    for (int u = 0; u < N; ++u) v[u] = bcast(v[u], kWave - 1);
-}
 
 template <int KR, typename T>
 __global__ void __launch_bounds__(kWave)
@@ -1520,6 +1503,7 @@ __global__ void __launch_bounds__(kWave)
     constexpr int kRow = KR + 3;  // a row's scalars in LDS: b per RHS, 1 / diagonal, near (i-1), near (i-2)
     __shared__ double ring[KR][kRing + 1];
     __shared__ double rowd[kPartPre][kRow];
+    __shared__ double red[2 * KR][kWave];  // a row pair's lane products, transposed for the sums
     const int lane = threadIdx.x;
     // right-hand sides q0 .. q0 + KR - 1 (b and y interleaved, k per row)
     const long long q0 = (long long)blockIdx.y * KR;
@@ -1612,18 +1596,28 @@ __global__ void __launch_bounds__(kWave)
                     }
                 }
             };
-            // S of rows r and r + 1 (all their right-hand sides), the 2 KR
-            // sums interleaved
+            // S of rows r and r + 1 (all their right-hand sides): the 2 KR
+            // sums of 64 lane products go through LDS, 8 lanes per sum add 8
+            // products each (a fixed tree), and three butterfly steps join
+            // the 8 lanes (part_sum8).  Every sum is formed the same way
+            // whatever KR, so each right-hand side keeps its one-RHS bits.
             auto pair_sums = [&](int r, const double (&rva)[KR], const double (&rvb)[KR], double (&sa)[KR],
                                  double (&sb)[KR]) {
-                double prod[2 * KR];
+                constexpr int N = 2 * KR;
+                double prod[N];
                 row_prod(r, rva, prod);
                 row_prod(r + 1, rvb, prod + KR);
-                wave_sum_n<2 * KR>(prod);
+#pragma unroll
+                for (int u = 0; u < N; ++u) red[u][lane] = prod[u];
+                const double *src = &red[min(lane >> 3, N - 1)][(lane & 7) * 8];  // lanes past 8 N: unused
+                double a[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) a[t] = src[t];
+                const double v = part_sum8(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
 #pragma unroll
                 for (int j = 0; j < KR; ++j) {
-                    sa[j] = prod[j];
-                    sb[j] = prod[KR + j];
+                    sa[j] = bcast(v, 8 * j);
+                    sb[j] = bcast(v, 8 * (KR + j));
                 }
             };
             // the sums of the group's first two rows (columns below g0 - 1)
