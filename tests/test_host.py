@@ -142,6 +142,19 @@ def test_validate_codes():
     assert api.validate_csr(n, len(ci) + 1, rp, ci, v)[0] == _native.CAP_ERR_ROWPTR
 
 
+def test_algorithm_wrapper_refuses_bad_algorithms():
+    """The CSR wrapper with a chosen algorithm returns CAP_ERR_ARG for mix
+    (which has its own wrapper) and for codes outside CAP_ALGO_*, before any
+    device work; the Python mirror raises for names it does not take."""
+    rp, ci, v = random_lower(200, max_off=3, band=20, seed=3)
+    n = len(rp) - 1
+    for algo in (_native.ALGO["mix"], -1, 7):
+        ret = api._bench("spts_syncfree_hip_csr_algo", rp, ci, v, n, n, len(ci), mix=False, algo=algo)[0]
+        assert ret == _native.CAP_ERR_ARG, algo
+    with pytest.raises(ValueError):
+        api.spts_syncfree_hip_csr_algo(rp, ci, v, n, n, len(ci), algo="mix")
+
+
 def test_datasize_formula():
     assert api.datasize(10, 10, 37, 8) == oracle.datasize(10, 10, 37, 8)
     assert api.datasize(10, 10, 37, 4, 5) == oracle.datasize(10, 10, 37, 4, 5)
