@@ -1739,7 +1739,7 @@ __global__ void __launch_bounds__(256) k_part_gemm(const int3 *__restrict__ ops,
 // LDS: thread c returns row c of each (four partial sums, by index mod 4, in
 // the same order for every KQ, so a right-hand side gets the same bits
 // whichever chunk it is solved in).  M is read once for all KQ vectors.
-template <int KQ>
+template <int KQ, bool NT>
 __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const double (*sv)[kPartW], int c,
                                             double (&out)[KQ])
 {
@@ -1753,7 +1753,8 @@ __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const 
     for (int kk = 0; kk < kPartW; kk += 4) {
         double mv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) mv[u] = M[(long long)(kk + u) * kPartW + c];
+        for (int u = 0; u < 4; ++u)
+            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * kPartW + c) : M[(long long)(kk + u) * kPartW + c];
 #pragma unroll
         for (int j = 0; j < KQ; ++j)
 #pragma unroll
@@ -1765,7 +1766,7 @@ __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const 
 
 // The part of y_j = M v_j that columns k0 .. k0 + KLEN - 1 of M contribute,
 // for row c (four partial sums by index mod 4, joined as in part_matvec).
-template <int KQ, int KLEN>
+template <int KQ, int KLEN, bool NT>
 __device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, const double (*sv)[kPartW], int c, int k0,
                                                  double (&out)[KQ])
 {
@@ -1778,7 +1779,8 @@ __device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, c
     for (int kk = k0; kk < k0 + KLEN; kk += 4) {
         double mv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) mv[u] = M[(long long)(kk + u) * kPartW + c];
+        for (int u = 0; u < 4; ++u)
+            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * kPartW + c) : M[(long long)(kk + u) * kPartW + c];
 #pragma unroll
         for (int j = 0; j < KQ; ++j)
 #pragma unroll
@@ -1807,7 +1809,7 @@ __device__ __forceinline__ long long part_vec(int p, int k, int j)
 // last u also goes to carry[g].  A step is sequential, so each product is
 // split over the columns (thread: row c, quarter kq) to have four times the
 // loads in flight; the first kPartW threads own the vectors.
-template <int KQ>
+template <int KQ, bool NT>
 __global__ void __launch_bounds__(kPartSplit * kPartW)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
                       const double *__restrict__ bm, int gs, int q_count, int k)
@@ -1833,7 +1835,7 @@ __global__ void __launch_bounds__(kPartSplit * kPartW)
         }
         __syncthreads();
         double mv[KQ];
-        part_matvec_cols<KQ, kLen>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, kq * kLen, mv);
+        part_matvec_cols<KQ, kLen, NT>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, kq * kLen, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) red[kq][j][c] = mv[j];
         __syncthreads();
@@ -1853,7 +1855,7 @@ __global__ void __launch_bounds__(kPartSplit * kPartW)
 
 // Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
 // p - gs of wm, T the scanned group carries), t_p = u_p in group 0.
-template <int KQ>
+template <int KQ, bool NT>
 __global__ void __launch_bounds__(kPartW)
     k_part_fixup(const double *__restrict__ u, const double *__restrict__ tg, const double *__restrict__ wm, int gs,
                  double *__restrict__ t, int k)
@@ -1869,7 +1871,7 @@ __global__ void __launch_bounds__(kPartW)
         for (int j = 0; j < KQ; ++j) sv[j][c] = tg[part_vec<KQ>(g - 1, k, j) + c];
         __syncthreads();
         double mv[KQ];
-        part_matvec<KQ>(wm + (long long)(p - gs) * kPartW * kPartW, sv, c, mv);
+        part_matvec<KQ, NT>(wm + (long long)(p - gs) * kPartW * kPartW, sv, c, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) v[j] += mv[j];
     }
@@ -1884,7 +1886,7 @@ __global__ void __launch_bounds__(kPartW)
 // product is spread over four workgroups; thread: row, quarter of the
 // columns, partial sums joined in LDS): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
 // index g - s of the round's matrices), dst_g = src_g below s.
-template <int KQ>
+template <int KQ, bool NT>
 __global__ void __launch_bounds__(kPartW)
     k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s, int k)
 {
@@ -1898,7 +1900,7 @@ __global__ void __launch_bounds__(kPartW)
         for (int j = 0; j < KQ; ++j) sv[j][t] = src[part_vec<KQ>(p - s, k, j) + t];
         __syncthreads();
         double mv[KQ];
-        part_matvec_cols<KQ, kLen>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, kq * kLen, mv);
+        part_matvec_cols<KQ, kLen, NT>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, kq * kLen, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) red[kq][j][r] = mv[j];
         __syncthreads();
@@ -1934,7 +1936,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll 8
         for (int c = 0; c < kPartW; c += 4)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = __builtin_fma(vt[(long long)(c + u) * n + i], tp[c + u], s[u]);
+            for (int u = 0; u < 4; ++u)
+                s[u] = __builtin_fma(__builtin_nontemporal_load(vt + (long long)(c + u) * n + i), tp[c + u], s[u]);
         xi -= (s[0] + s[1]) + (s[2] + s[3]);
     }
     x[gi] = (T)xi;
@@ -1945,8 +1948,10 @@ __global__ void __launch_bounds__(kBlock)
 // blockIdx.y * KQ + j.  It stages both blocks' t_{p-1} in LDS (RHS fastest,
 // read as broadcasts), and each thread reads its row of V once for all KQ.
 // The sums run in the single-RHS kernel's order (four partial sums by
-// column mod 4), so each RHS gets that kernel's bits.
-template <int KQ, typename T>
+// column mod 4), so each RHS gets that kernel's bits.  NT: V is read with
+// non-temporal loads (one chunk covers every RHS, so V is read once; with
+// several chunks, cached reads let later chunks reuse some of it).
+template <int KQ, typename T, bool NT>
 __global__ void __launch_bounds__(kPartW)
     k_part_finish_multi(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t,
                         int n, int m, int k, T *__restrict__ x)
@@ -1975,7 +1980,8 @@ __global__ void __launch_bounds__(kPartW)
         for (int c = 0; c < kPartW; c += 4) {
             double mv[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) mv[u] = vt[(long long)(c + u) * n + i];
+            for (int u = 0; u < 4; ++u)
+                mv[u] = NT ? __builtin_nontemporal_load(vt + (long long)(c + u) * n + i) : vt[(long long)(c + u) * n + i];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll

@@ -481,23 +481,32 @@ void launch_part_block(cap_plan p, const T *b, int k, hipStream_t st)
 constexpr int kPartScanRhs = 4;
 
 // The tail scan over the group carries, KQ right-hand sides per workgroup
-// (KQ | k); returns the buffer that holds the scanned carries.
-template <int KQ>
-double *launch_part_carries(cap_plan p, int k, hipStream_t st)
+// (KQ | k); returns the buffer that holds the scanned carries.  NT: the
+// matrices are read with non-temporal loads, when one chunk covers every
+// RHS and each matrix is read once (with several chunks, cached reads let
+// later chunks reuse them).
+template <int KQ, bool NT>
+double *part_carries(cap_plan p, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     const int Q = ps->Q;
     const unsigned chunks = (unsigned)(k / KQ);
     const long long mat = (long long)kPartW * kPartW;
-    k_part_group_scan<KQ><<<dim3(ps->G, chunks), kPartSplit * kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs,
+    k_part_group_scan<KQ, NT><<<dim3(ps->G, chunks), kPartSplit * kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs,
                                                                               Q, k);
     double *cur = ps->c0, *nxt = ps->c1;
     for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<KQ><<<dim3(ps->G * kPartSplit, chunks), kPartW, 0, st>>>(
+        k_part_scan<KQ, NT><<<dim3(ps->G * kPartSplit, chunks), kPartW, 0, st>>>(
             cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r, k);
         std::swap(cur, nxt);
     }
     return cur;
+}
+
+template <int KQ>
+double *launch_part_carries(cap_plan p, int k, hipStream_t st)
+{
+    return k == KQ ? part_carries<KQ, true>(p, k, st) : part_carries<KQ, false>(p, k, st);
 }
 
 // The fix-up of every tail and the finish, KQ right-hand sides per
@@ -508,12 +517,21 @@ void launch_part_fix(cap_plan p, const double *tg, T *x, int k, hipStream_t st)
     const PartState *ps = p->part;
     const unsigned chunks = (unsigned)(k / KQ);
     const long long mat = (long long)kPartW * kPartW;
-    k_part_fixup<KQ><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs, ps->t, k);
+    if (chunks == 1)  // non-temporal matrix reads when each W is read once
+        k_part_fixup<KQ, true><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
+                                                                      ps->t, k);
+    else
+        k_part_fixup<KQ, false><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
+                                                                       ps->t, k);
     if constexpr (KQ == 1)
         k_part_finish<T><<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
-    else  // m >= kPartW: a workgroup's kPartW rows span at most two blocks
-        k_part_finish_multi<KQ, T><<<dim3((unsigned)((p->n + kPartW - 1) / kPartW), chunks), kPartW, 0, st>>>(
-            ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+    else {  // m >= kPartW: a workgroup's kPartW rows span at most two blocks
+        const dim3 grid((unsigned)((p->n + kPartW - 1) / kPartW), chunks);
+        if (chunks == 1)
+            k_part_finish_multi<KQ, T, true><<<grid, kPartW, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+        else
+            k_part_finish_multi<KQ, T, false><<<grid, kPartW, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+    }
 }
 
 // One solve of a partitioned plan (k FP64 right-hand sides, interleaved:
