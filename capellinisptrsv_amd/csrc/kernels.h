@@ -1491,6 +1491,16 @@ __device__ __forceinline__ double part_sum8(double v)
     return v;
 }
 
+// v of lane `src` (0..63, may differ per lane; every lane active), through
+// the LDS crossbar (ds_bpermute): no LDS memory and no VALU work
+__device__ __forceinline__ double lane_read(double v, int src)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src * 4, (int)u);
+    const int hi = __builtin_amdgcn_ds_bpermute(src * 4, (int)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 
 template <int KR, typename T>
 __global__ void __launch_bounds__(kWave)
@@ -1545,11 +1555,11 @@ __global__ void __launch_bounds__(kWave)
         const int i = s0 + wr;
         if (lane < kPartPre * KR && i < end) y[(long long)i * k + q0 + wj] = ring[wj][(i - start) & (kRing - 1)];
     };
-    // y_{i-1}, y_{i-2} per right-hand side (0 before the block's first row,
-    // where the near values are 0 too)
-    double y1[KR], y2[KR];
-#pragma unroll
-    for (int j = 0; j < KR; ++j) y1[j] = y2[j] = 0.0;
+    // Each lane carries the chain of one right-hand side, wj = lane mod KR:
+    // y_{i-1}, y_{i-2} of that RHS (0 before the block's first row, where the
+    // near values are 0 too).  Every lane still forms the products of its
+    // ELL entry for all KR right-hand sides (the sums need them).
+    double y1 = 0.0, y2 = 0.0;
     fetch(start);
     for (int g0 = start; g0 < end; g0 += kPartPre) {
         int cc[kPartPre];
@@ -1601,8 +1611,7 @@ __global__ void __launch_bounds__(kWave)
             // products each (a fixed tree), and three butterfly steps join
             // the 8 lanes (part_sum8).  Every sum is formed the same way
             // whatever KR, so each right-hand side keeps its one-RHS bits.
-            auto pair_sums = [&](int r, const double (&rva)[KR], const double (&rvb)[KR], double (&sa)[KR],
-                                 double (&sb)[KR]) {
+            auto pair_sums = [&](int r, const double (&rva)[KR], const double (&rvb)[KR], double &sa, double &sb) {
                 constexpr int N = 2 * KR;
                 double prod[N];
                 row_prod(r, rva, prod);
@@ -1614,37 +1623,36 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
                 for (int t = 0; t < 8; ++t) a[t] = src[t];
                 const double v = part_sum8(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
-#pragma unroll
-                for (int j = 0; j < KR; ++j) {
-                    sa[j] = bcast(v, 8 * j);
-                    sb[j] = bcast(v, 8 * (KR + j));
+                // sum u sits in lanes 8u .. 8u + 7: this lane's RHS's two
+                if constexpr (KR == 1) {
+                    sa = bcast(v, 0);
+                    sb = bcast(v, 8);
+                } else {
+                    sa = lane_read(v, 8 * wj);
+                    sb = lane_read(v, 8 * (KR + wj));
                 }
             };
             // the sums of the group's first two rows (columns below g0 - 1)
-            double s0[KR], s1[KR], rv0[KR], rv1[KR];
+            double s0, s1, rv0[KR], rv1[KR];
             ring_vals(0, rv0);
             ring_vals(1, rv1);
             if (g0 + kPartPre < end) fetch(g0 + kPartPre);
             if (g0 > start) store_group(g0 - kPartPre);
             pair_sums(0, rv0, rv1, s0, s1);
             // y of row r from its sum s
-            auto finish_row = [&](int r, const double (&s)[KR]) {
+            // y of row r for this lane's RHS from its sum s
+            auto finish_row = [&](int r, double s) {
                 const int i = g0 + r;
                 const double di = rowd[r][KR], w1 = rowd[r][KR + 1], w2 = rowd[r][KR + 2];
-                double mine = 0.0;  // lanes wj = j (lane mod KR) keep right-hand side j's value
-#pragma unroll
-                for (int j = 0; j < KR; ++j) {
-                    double t = rowd[r][j] - s[j];
-                    t = __builtin_fma(-w2, y2[j], t);
-                    t = __builtin_fma(-w1, y1[j], t);
-                    const double yi = t * di;
-                    y2[j] = y1[j];
-                    y1[j] = yi;
-                    if (wj == j) mine = yi;
-                }
+                double t = rowd[r][wj] - s;
+                t = __builtin_fma(-w2, y2, t);
+                t = __builtin_fma(-w1, y1, t);
+                const double yi = t * di;
+                y2 = y1;
+                y1 = yi;
                 // every lane writes (the lanes of one RHS the same value to
                 // the same slot), so no lane mask splits the rows' code
-                ring[wj][(i - start) & (kRing - 1)] = mine;
+                ring[wj][(i - start) & (kRing - 1)] = yi;
             };
             // Rows in pairs: row r + 2's ring values (columns <= i - 1) are
             // read before y_i is written, row r + 3's (columns <= i) before
