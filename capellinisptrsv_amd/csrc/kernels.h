@@ -1528,12 +1528,29 @@ __global__ void __launch_bounds__(kWave)
     double nv[kPartPre], nb[KR], nd = 1.0, nw1 = 0.0, nw2 = 0.0;
     int no = 0;
     auto fetch = [&](int g0) {
+        if (g0 + kPartPre <= end) {  // uniform: a whole group, one lane mask for all its loads
+            if (lane < ew) {
 #pragma unroll
-        for (int r = 0; r < kPartPre; ++r) {
-            const long long e = (long long)(g0 + r) * ew + lane;
-            const bool in = g0 + r < end && lane < ew;
-            nc[r] = in ? ell_c[e] : kRing * 8;
-            nv[r] = in ? ell_v[e] : 0.0;
+                for (int r = 0; r < kPartPre; ++r) {
+                    const long long e = (long long)(g0 + r) * ew + lane;
+                    nc[r] = ell_c[e];
+                    nv[r] = ell_v[e];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < kPartPre; ++r) {
+                    nc[r] = kRing * 8;
+                    nv[r] = 0.0;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kPartPre; ++r) {
+                const long long e = (long long)(g0 + r) * ew + lane;
+                const bool in = g0 + r < end && lane < ew;
+                nc[r] = in ? ell_c[e] : kRing * 8;
+                nv[r] = in ? ell_v[e] : 0.0;
+            }
         }
         const int row = min(g0 + lane, end);
         const bool mine = lane < kPartPre && row < end;
