@@ -33,6 +33,8 @@ SYNC = {"auto": 0, "value": 1, "flag": 2}
 ADJOINT = {"auto": 0, "scatter": 1, "gather": 2}
 READINESS = {"auto": 0, "staged": 1, "in_place": 2}
 READINESS_NAME = {v: k for k, v in READINESS.items()}
+# fused: fma per term; reference: rounded product then rounded sum (the reference's serial executor)
+ARITH = {"fused": 0, "reference": 1}
 SYNC_NAME = {v: k for k, v in SYNC.items()}
 
 
@@ -51,7 +53,7 @@ class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
                 ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
-                ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int)]
+                ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int), ("arith", c_int)]
 
 
 class cap_gen_params(Structure):
@@ -184,5 +186,7 @@ def default_opts(**kw) -> cap_opts:
             v = ADJOINT[v]
         if k == "readiness" and isinstance(v, str):
             v = READINESS[v]
+        if k == "arith" and isinstance(v, str):
+            v = ARITH[v]
         setattr(o, k, int(v))
     return o

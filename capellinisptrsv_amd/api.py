@@ -206,11 +206,12 @@ def sptrsv_syncfree_hip_csc(csc_col_ptr, csc_row_idx, csc_val, m: int, n: int, n
 
 
 def sptrsv_csr_solve(row_ptr, col_idx, val, b, algo: str = "wrtfst", sync: str = "value", border: int = 0,
-                     spin_limit: int = 0, upper: bool = False, unit_diagonal: bool = False):
+                     spin_limit: int = 0, upper: bool = False, unit_diagonal: bool = False, arith: str = "fused"):
     """One-shot host solve T x = b on the GPU; raises CapError on failure.  T is
     lower triangular with the diagonal last in each row, or with upper=True
     upper triangular with the diagonal first; unit_diagonal=True: no diagonal
-    is stored and it is taken as 1."""
+    is stored and it is taken as 1.  arith="reference": each term rounded as
+    the reference's serial executor rounds it (product, then sum)."""
     val = np.ascontiguousarray(val)
     sfx = _suffix(val.dtype)
     rp, ci = _i32(row_ptr), _i32(col_idx)
@@ -218,7 +219,7 @@ def sptrsv_csr_solve(row_ptr, col_idx, val, b, algo: str = "wrtfst", sync: str =
     b = np.ascontiguousarray(b, dtype=val.dtype)
     x = np.zeros(n, dtype=val.dtype)
     o = N.default_opts(algo=algo, sync=sync, border=border, spin_limit=spin_limit, upper=int(upper),
-                       unit_diag=int(unit_diagonal))
+                       unit_diag=int(unit_diagonal), arith=arith)
     N.check(getattr(N.lib(), f"sptrsv_csr_solve_{sfx}")(n, int(rp[-1]), _pi(rp), _pi(ci), _pv(val), _pv(b), _pv(x),
                                                          byref(o)), "sptrsv_csr_solve")
     return x

@@ -6,10 +6,12 @@
 // solve runs on the GPU.
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -61,7 +63,7 @@ struct Cursor {
         if (!skip()) return false;
         char *end = nullptr;
         long v = std::strtol(p, &end, 10);
-        if (end == p) return false;
+        if (end == p || v < INT_MIN || v > INT_MAX) return false;
         p = end;
         *out = (int)v;
         return true;
@@ -78,14 +80,18 @@ struct Cursor {
 };
 
 template <typename T>
-int read_mtx_impl(const char *path, int *m_out, int *n_out, int *nnz_out, int **rp_out, int **ci_out, T **v_out)
+int read_mtx_body(const char *path, int *m_out, int *n_out, int *nnz_out, int **rp_out, int **ci_out, T **v_out)
 {
     FILE *f = std::fopen(path, "rb");
     if (!f) return -1;
     std::fseek(f, 0, SEEK_END);
     long size = std::ftell(f);
     std::fseek(f, 0, SEEK_SET);
-    std::vector<char> buf((size_t)(size > 0 ? size : 0) + 1, '\0');
+    if (size < 0) {
+        std::fclose(f);
+        return -1;
+    }
+    std::vector<char> buf((size_t)size + 1, '\0');
     size_t got = size > 0 ? std::fread(buf.data(), 1, (size_t)size, f) : 0;
     std::fclose(f);
     const char *p = buf.data(), *e = buf.data() + got;
@@ -108,10 +114,13 @@ int read_mtx_impl(const char *path, int *m_out, int *n_out, int *nnz_out, int **
     } while (!line.empty() && line[0] == '%');
     int m, n, nr;
     if (std::sscanf(line.c_str(), "%d %d %d", &m, &n, &nr) != 3 || m < 0 || n < 0 || nr < 0) return -4;
+    // every entry takes at least four bytes ("i j" and a separator): a count
+    // the rest of the file cannot hold is malformed, not an allocation size
+    if ((long long)nr > (long long)(e - p) / 4 + 1) return CAP_ERR_IO;
 
     std::vector<int> ri((size_t)nr), cj((size_t)nr);
     std::vector<T> vv((size_t)nr);
-    std::vector<int> cnt((size_t)m + 1, 0);
+    std::vector<long long> cnt((size_t)m + 1, 0);
     Cursor cur{p, e};
     for (int k = 0; k < nr; ++k) {
         int a, b;
@@ -134,6 +143,11 @@ int read_mtx_impl(const char *path, int *m_out, int *n_out, int *nnz_out, int **
         cnt[a]++;
         if (t.symmetric && a != b) cnt[b]++;  // mirrored entry (read_mtx.h:110-117)
     }
+    // mirrored symmetric entries can take the total past int32 even when the
+    // entry count fits: refuse it before any index is formed
+    long long total = 0;
+    for (int i = 0; i < m; ++i) total += cnt[i];
+    if (total > (long long)INT_MAX) return CAP_ERR_IO;
     int *rp = (int *)std::malloc(sizeof(int) * ((size_t)m + 1));
     if (!rp) return CAP_ERR_ALLOC;
     rp[0] = 0;
@@ -162,6 +176,19 @@ int read_mtx_impl(const char *path, int *m_out, int *n_out, int *nnz_out, int **
     *m_out = m; *n_out = n; *nnz_out = nnz;
     *rp_out = rp; *ci_out = ci; *v_out = v;
     return 0;
+}
+
+// The reader's working vectors are sized from the file header: an
+// allocation failure is reported as CAP_ERR_ALLOC, never thrown across the
+// C ABI.
+template <typename T>
+int read_mtx_impl(const char *path, int *m_out, int *n_out, int *nnz_out, int **rp_out, int **ci_out, T **v_out)
+{
+    try {
+        return read_mtx_body<T>(path, m_out, n_out, nnz_out, rp_out, ci_out, v_out);
+    } catch (const std::bad_alloc &) {
+        return CAP_ERR_ALLOC;
+    }
 }
 
 // change2tran (read_mtx.h:194-257): for row i < m keep the entries with

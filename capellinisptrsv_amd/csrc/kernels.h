@@ -87,13 +87,27 @@ __device__ __forceinline__ void atomic_add_agent(V *p, V v)
 //  into x with plain stores.  nullptr = x itself is polled.  The library
 //  places it in uncached device memory, where a poll goes straight to the
 //  memory side instead of through an L2 that agent-scope loads must miss.
+//  unfused: accumulate s + v*x as a rounded product and a rounded sum (the
+//  arithmetic of the reference's serial executor,
+//  openCL/SynFree_csc/sptrsv_syncfree_serialref.h:67-68, compiled without
+//  contraction) instead of one fused multiply-add.
 struct SyncCtx {
     int *flags;
     int epoch;
     int spin_limit;
     int *err;
     void *ready;
+    int unfused;
 };
+
+// One accumulation step of a row's sum, in the plan's arithmetic.  The
+// operand order and the stored-order chain are the same in both forms; only
+// the rounding of the product differs.
+template <typename T>
+__device__ __forceinline__ T acc(const SyncCtx &sc, T v, T x, T s)
+{
+    return sc.unfused ? s + v * x : fmac(v, x, s);
+}
 
 template <typename T>
 __device__ __forceinline__ typename VBits<T>::U *ready_array(const SyncCtx &sc, T *x)
@@ -321,7 +335,7 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
             const unsigned long long ready = __ballot(rdy);
             const int prefix = ~ready ? __builtin_ctzll(~ready) : kWave;
             const int upto = min(prefix, cnt);
-            for (int t = used; t < upto; ++t) s = fmac(bcast(v, t), bcast(xv, t), s);
+            for (int t = used; t < upto; ++t) s = acc(sc, bcast(v, t), bcast(xv, t), s);
             if (upto > used) {
                 idle = 0;
             } else if (give_up(sc, ++idle)) {  // uniform: idle is the same in every lane
@@ -379,11 +393,11 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
             if (j < d && x0 == kS) x0 = win.has(c0) ? win.load(c0, q) : ld_agent(xr + (long long)c0 * k + q);
             if (j + 1 < d && x1 == kS) x1 = win.has(c1) ? win.load(c1, q) : ld_agent(xr + (long long)c1 * k + q);
             if (j < d && x0 != kS) {
-                s = fmac(v0, as_val(x0), s);
+                s = acc(sc, v0, as_val(x0), s);
                 ++j;
                 prog = true;
                 if (j < d && x1 != kS) {  // both consumed: a fresh window
-                    s = fmac(v1, as_val(x1), s);
+                    s = acc(sc, v1, as_val(x1), s);
                     ++j;
                     x0 = kS;
                     if (j < d) {
@@ -482,7 +496,7 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 #pragma unroll
                 for (int t = 0; t < kWin; ++t)
                     if (t == used && t < cnt && rdy[t]) {
-                        s = fmac(vw[t], as_val(xw[t]), s);
+                        s = acc(sc, vw[t], as_val(xw[t]), s);
                         ++used;
                     }
                 if (used != before) {
@@ -742,7 +756,7 @@ __global__ void __launch_bounds__(kBlock)
         for (; j < d; ++j) {  // rows of earlier wavefronts: blocking waits
             const int c = ci[j];
             if (c >= wave_begin) break;
-            s = fmac(val[j], wait_value<T, SYNC>(x, sc, c), s);
+            s = acc(sc, val[j], wait_value<T, SYNC>(x, sc, c), s);
         }
         wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, j, d, 1, 0, s);
     });
@@ -959,12 +973,12 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                             const T x0 = as_val(xs[t]), x1 = as_val(xs[t + 1]), x2 = as_val(xs[t + 2]),
                                     x3 = as_val(xs[t + 3]);
                             const T v0 = vs[t], v1 = vs[t + 1], v2 = vs[t + 2], v3 = vs[t + 3];
-                            s = fmac(v0, x0, s);
-                            s = fmac(v1, x1, s);
-                            s = fmac(v2, x2, s);
-                            s = fmac(v3, x3, s);
+                            s = acc(sc, v0, x0, s);
+                            s = acc(sc, v1, x1, s);
+                            s = acc(sc, v2, x2, s);
+                            s = acc(sc, v3, x3, s);
                         }
-                        for (; t < upto; ++t) s = fmac(vs[t], as_val(xs[t]), s);
+                        for (; t < upto; ++t) s = acc(sc, vs[t], as_val(xs[t]), s);
                         used = upto;
                     }
                 } else
@@ -987,7 +1001,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                                 }
                                 xr[g] = poll(cu[g]);
                             }
-                            s = fmac(bcast(cv[u], u0 + g), as_val(xr[g]), s);
+                            s = acc(sc, bcast(cv[u], u0 + g), as_val(xr[g]), s);
                         }
                     }
                 }
@@ -1069,7 +1083,7 @@ __device__ __forceinline__ void local_rows_global(const int *__restrict__ rp, co
                             xr[g] = __hip_atomic_load(xu + (long long)cu[g] * k + qk, __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
-                        s = fmac(bcast(vl, u0 + g), as_val(xr[g]), s);
+                        s = acc(sc, bcast(vl, u0 + g), as_val(xr[g]), s);
                     }
                 }
             }

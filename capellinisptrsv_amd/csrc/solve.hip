@@ -93,6 +93,7 @@ struct cap_plan_s {
     int wblock = kBlock;     // threads per workgroup of k_wrtfst (128 or 256)
     int rblock = 0;          // of k_refine2; 0 = not chosen yet (autotuned plans time it)
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
+    int unfused = 0;         // CAP_ARITH_REFERENCE: rounded products, then rounded sums
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
     int n_sched = 0, sched_rows = 0;
@@ -415,7 +416,7 @@ int local_ring_log2(const cap_plan p, int rhs, size_t *bytes)
 template <typename T>
 int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
 {
-    *sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    *sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
     if (sync == CAP_SYNC_VALUE && p->staged) {
         if (!p->ready || p->ready_bytes < sizeof(typename VBits<T>::U) * (size_t)count) return CAP_ERR_ARG;
         sc->ready = p->ready;
@@ -568,14 +569,19 @@ int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
     return CAP_OK;
 }
 
+// fused: accumulate with fused multiply-adds whatever the plan's arithmetic
+// (the refinement's FP32 corrections, which belong to this library's own
+// method and follow the oracle's fused steps).
 template <typename T>
-int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x, int rhs, hipStream_t st)
+int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x, int rhs, hipStream_t st,
+                  bool fused = false)
 {
     const int n = p->n;
     if (n == 0) return CAP_OK;
     if (algo == CAP_ALGO_PARTITION) return launch_partition<T>(p, b, x, rhs, st);
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
+    if (fused) sc.unfused = 0;
     if (algo == CAP_ALGO_WRTFST) {
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
@@ -732,12 +738,17 @@ int ensure_csc(cap_plan p, hipStream_t st)
 }
 
 // ----------------------------------------------------------- autotuner
+// A pick is cached per matrix shape: size, value type, readiness, the
+// dependency depth (level count) and a hash of sampled row pointers and
+// column indices.  Two matrices with one row-length profile but different
+// dependencies (a chain and a wide-level matrix) never share a pick.
 struct TuneKey {
-    int n, nnz, vt, staged;
+    int n, nnz, vt, staged, unfused, levels;
     unsigned long long fp;
     bool operator<(const TuneKey &o) const
     {
-        return std::tie(n, nnz, vt, staged, fp) < std::tie(o.n, o.nnz, o.vt, o.staged, o.fp);
+        return std::tie(n, nnz, vt, staged, unfused, levels, fp) <
+               std::tie(o.n, o.nnz, o.vt, o.staged, o.unfused, o.levels, o.fp);
     }
 };
 std::mutex g_tune_mu;
@@ -746,13 +757,15 @@ struct TuneChoice {
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 
-unsigned long long fingerprint(const std::vector<int> &rp)
+unsigned long long fingerprint(const std::vector<int> &rp, const std::vector<int> &ci)
 {
     unsigned long long h = 1469598103934665603ULL;
-    const size_t n = rp.size();
-    for (int s = 0; s < 257 && n; ++s) {
-        h ^= (unsigned long long)(unsigned)rp[(n - 1) * (size_t)s / 256];
-        h *= 1099511628211ULL;
+    for (const std::vector<int> *a : {&rp, &ci}) {
+        const size_t n = a->size();
+        for (int s = 0; s < 257 && n; ++s) {
+            h ^= (unsigned long long)(unsigned)(*a)[(n - 1) * (size_t)s / 256];
+            h *= 1099511628211ULL;
+        }
     }
     return h;
 }
@@ -808,8 +821,9 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 // afterwards) and is stored column-major; the scan's matrices are formed
 // round by round.
 constexpr int kPartBlocks = 1024;
-// largest |x - 1| accepted when the tables solve L x = L 1 at plan creation
-constexpr double kPartStableTol = 1e-10;
+// largest relative max-norm difference from the sequential solve accepted
+// when the tables are checked at plan creation (the FP64 bar is 1e-12)
+constexpr double kPartStableTol = 1e-13;
 constexpr int kPartGroup = 8;  // tails per group of the scan's first level
 // Largest table set a partitioned plan may hold (device; the host builds the
 // same tables once): larger matrices are refused rather than risk exhausting
@@ -1034,30 +1048,54 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     for (double *t : {r0, r1, v0, v1})
         if (t) (void)hipFree(t);
     if (d_ops) (void)hipFree(d_ops);
-    // Stability check: solve L x = L 1 (row sums in FP64) with the tables
-    // just built.  The block solves and V multiply by block inverses whose
-    // entries grow without bound when the blocks are not diagonally dominant
-    // (unit values in a band, as the reference's .mtx reader makes them);
-    // the sequential solve never forms them.  Such matrices are refused.
+    // Accuracy check against the sequential solve.  The block solves and V
+    // multiply by block inverses, which the sequential solve never forms;
+    // when the blocks are not diagonally dominant their entries grow along a
+    // block (unit values in a band, as the reference's .mtx reader makes
+    // them, grow as 1.47^m) and the finish cancels.  The tables solve two
+    // right-hand sides -- b = L 1 (row sums in FP64) and a seeded U[-1,1] b
+    // -- and each must agree with a sequential FP64 forward substitution of
+    // the same system on the host to kPartStableTol relative max-norm, ten
+    // times inside the FP64 bar of 1e-12; otherwise the matrix is refused.
     if (rc == CAP_OK) {
-        std::vector<double> rs((size_t)n, 0.0), xs((size_t)n);
-        for (int i = 0; i < n; ++i)
-            for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) rs[(size_t)i] += v[(size_t)j];
+        std::vector<double> bh((size_t)n * 2), xs((size_t)n * 2), xr((size_t)n * 2);
+        cap_uniform_d(0x5eedULL, n, -1.0, 1.0, xr.data());  // reused below as the reference solution
+        for (int i = 0; i < n; ++i) {
+            double rs = 0.0;
+            for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) rs += v[(size_t)j];
+            bh[(size_t)i * 2] = rs;
+            bh[(size_t)i * 2 + 1] = xr[(size_t)i];
+        }
+        for (int i = 0; i < n; ++i) {
+            const int d = rp_host[i + 1] - 1;
+            for (int q = 0; q < 2; ++q) {
+                double sum = 0.0;
+                for (int j = rp_host[i]; j < d; ++j) sum = std::fma(v[(size_t)j], xr[(size_t)ci[j] * 2 + q], sum);
+                xr[(size_t)i * 2 + q] = (bh[(size_t)i * 2 + q] - sum) / v[(size_t)d];
+            }
+        }
         double *db = nullptr, *dx = nullptr;
-        if ((rc = dev_alloc(&db, (size_t)n)) == CAP_OK && (rc = dev_alloc(&dx, (size_t)n)) == CAP_OK) {
-            if (hipMemcpy(db, rs.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        if ((rc = ensure_part_rhs(p, 2)) == CAP_OK && (rc = dev_alloc(&db, (size_t)n * 2)) == CAP_OK &&
+            (rc = dev_alloc(&dx, (size_t)n * 2)) == CAP_OK) {
+            if (hipMemcpy(db, bh.data(), sizeof(double) * bh.size(), hipMemcpyHostToDevice) != hipSuccess)
                 rc = CAP_ERR_HIP;
-            if (rc == CAP_OK) rc = launch_partition<double>(p, db, dx, 1, st);
+            if (rc == CAP_OK) rc = launch_partition<double>(p, db, dx, 2, st);
             if (rc == CAP_OK && (hipStreamSynchronize(st) != hipSuccess ||
-                                 hipMemcpy(xs.data(), dx, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost) !=
+                                 hipMemcpy(xs.data(), dx, sizeof(double) * xs.size(), hipMemcpyDeviceToHost) !=
                                      hipSuccess))
                 rc = CAP_ERR_HIP;
         }
         (void)hipFree(db);
         (void)hipFree(dx);
-        double err = 0.0;
-        for (double xv : xs) err = std::max(err, std::fabs(xv - 1.0));
-        if (rc == CAP_OK && !(err <= kPartStableTol)) rc = CAP_ERR_UNSUPPORTED;  // NaN fails too
+        for (int q = 0; q < 2 && rc == CAP_OK; ++q) {
+            double err = 0.0, ref = 0.0;
+            for (int i = 0; i < n; ++i) {
+                err = std::max(err, std::fabs(xs[(size_t)i * 2 + q] - xr[(size_t)i * 2 + q]));
+                ref = std::max(ref, std::fabs(xr[(size_t)i * 2 + q]));
+            }
+            // NaN or inf anywhere (err or ref) fails the test too
+            if (!(err <= kPartStableTol * ref) || !std::isfinite(ref)) rc = CAP_ERR_UNSUPPORTED;
+        }
     }
     if (rc != CAP_OK) free_partition(p);
     return rc;
@@ -1072,7 +1110,10 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
 // it in every candidate.
 int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
 {
-    TuneKey key{p->n, p->nnz, p->vt, p->staged, fingerprint(rp_host)};
+    std::vector<int> ci((size_t)p->nnz);
+    CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    const int levels = p->n ? cap_matrix_layer(p->n, rp_host.data(), ci.data(), nullptr, nullptr, nullptr) : 1;
+    const TuneKey key{p->n, p->nnz, p->vt, p->staged, p->unfused, levels, fingerprint(rp_host, ci)};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -1104,15 +1145,13 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     const double avg = (double)p->nnz / std::max(1, p->n);
     // one workgroup per matrix pays off only with few rows per level
     {
-        std::vector<int> ci((size_t)p->nnz);
-        CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
-        const int levels = p->n ? cap_matrix_layer(p->n, rp_host.data(), ci.data(), nullptr, nullptr, nullptr) : 1;
         // timed first: on such matrices it sets a low bound that lets the
         // slower candidates stop after their warm-up
         if ((double)p->n / std::max(1, levels) <= kLocalMaxWidth) {
             cands.insert(cands.begin(), {CAP_ALGO_LOCAL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
             // deep and narrow-banded: the partitioned solve, timed before all
-            if (build_partition(p, rp_host, st) == CAP_OK)
+            // (reassociated sums: not a candidate in the reference arithmetic)
+            if (!p->unfused && build_partition(p, rp_host, st) == CAP_OK)
                 cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
         }
     }
@@ -1216,6 +1255,9 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     if (!o.unit_diag && nnz < n) return CAP_ERR_ARG;
     if (o.blocks_per_cu < 0 || o.blocks_per_cu > kMaxBlocksPerCu) return CAP_ERR_ARG;
     if (o.row_threads != 0 && o.row_threads != kBlock && o.row_threads != kBlockSmall) return CAP_ERR_ARG;
+    if (o.arith != CAP_ARITH_FUSED && o.arith != CAP_ARITH_REFERENCE) return CAP_ERR_ARG;
+    if (o.arith == CAP_ARITH_REFERENCE && (o.algo == CAP_ALGO_CSC || o.algo == CAP_ALGO_PARTITION))
+        return CAP_ERR_UNSUPPORTED;
     *out = nullptr;
     if (o.upper || o.unit_diag) {
         // convert on the host, then create a plan over the owned lower matrix
@@ -1302,6 +1344,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         if (o.row_threads > 0) p->wblock = o.row_threads;
         p->lds = o.lds_window > 0;
+        p->unfused = o.arith == CAP_ARITH_REFERENCE;
         p->staged = o.readiness != CAP_READY_IN_PLACE;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
@@ -1542,7 +1585,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     CAP_HIP_TRY(hipMemcpyAsync(p->cnt, p->cnt0, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
     k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
                                                                     p->ticket, kTickets);
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
     k_adjoint<<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
         p->rp, p->ci, (const double *)p->val, (const double *)g, (const double *)x, p->cnt, p->adj_ls,
         (double *)grad_b, (double *)grad_val, n, sc, p->ticket);
@@ -1645,7 +1688,7 @@ int refine_step(cap_plan p, const double *b, double *x, int rhs, bool first, hip
     // partitioned ones included) or WRTFST
     const int algo = p->algo == CAP_ALGO_LOCAL || p->algo == CAP_ALGO_PARTITION ? CAP_ALGO_LOCAL : CAP_ALGO_WRTFST;
     CAP_TRY(launch_prepare<float>(p, algo, CAP_SYNC_VALUE, p->d32, len, st));
-    CAP_TRY(launch_kernel<float>(p, algo, CAP_SYNC_VALUE, p->val32, p->r32, p->d32, rhs, st));
+    CAP_TRY(launch_kernel<float>(p, algo, CAP_SYNC_VALUE, p->val32, p->r32, p->d32, rhs, st, true));
     k_accumulate<<<stream_blocks(p, len), kBlock, 0, st>>>(x, p->d32, len, p->norms, first ? 1 : 0);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
@@ -1734,7 +1777,11 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
 {
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
     if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
-    if (p->algo == CAP_ALGO_PARTITION && p->part && max_iter >= 2)
+    // every refined solve of a partitioned plan (any max_iter) runs FP64
+    // partitioned solves: the plan has no segment table for LOCAL FP32
+    // corrections, and building one inside the graph capture below would
+    // allocate there
+    if (p->algo == CAP_ALGO_PARTITION && p->part)
         return refined_partition(p, b, x, rhs, tol, max_iter, iters, last_rel, st);
     CAP_TRY(ensure_refine(p, rhs, st));
     // the staged readiness array must exist before capture (no allocation

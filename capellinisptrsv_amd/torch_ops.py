@@ -44,12 +44,16 @@ class Plan:
     upper=True: upper triangular, diagonal first in each row; unit_diagonal=
     True: no diagonal stored (taken as 1).  Both convert the matrix into a
     plan-owned copy; such plans solve, but have no adjoint or refinement.
+    arith: "fused" (one fused multiply-add per term) or "reference" (a
+    rounded product then a rounded sum per term: bit-identical to the
+    reference's serial CPU executor; not with csc or partition).
     """
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
-                 upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, stream=None):
+                 upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, arith="fused",
+                 stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -62,7 +66,7 @@ class Plan:
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
-                              unit_diag=int(unit_diagonal), row_threads=row_threads)
+                              unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -150,17 +154,18 @@ class Batch:
     """Many matrices (concatenated, local column indices) with k interleaved RHS."""
 
     def __init__(self, row_off: torch.Tensor, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor,
-                 k: int, sync="value", validate: bool = True, algo="auto", blocks_per_cu: int = 0, stream=None):
+                 k: int, sync="value", validate: bool = True, algo="auto", blocks_per_cu: int = 0,
+                 arith="fused", stream=None):
         """algo: auto (both timed at creation) | wrtfst (level-scheduled
         thread-per-row over all matrices) | local (one workgroup per matrix).
         blocks_per_cu: persistent workgroups per compute unit (0: chosen
-        at creation)."""
+        at creation).  arith: "fused" or "reference" (as for Plan)."""
         self.row_off, self.row_ptr, self.col_idx, self.values = row_off, row_ptr, col_idx, values
         self.k = k
         self.n = row_ptr.numel() - 1
         self.dtype = values.dtype
         opts = N.default_opts(algo=algo, sync=sync, validate=int(validate), host_inputs=0,
-                              blocks_per_cu=blocks_per_cu)
+                              blocks_per_cu=blocks_per_cu, arith=arith)
         h = c_void_p()
         N.check(N.lib().cap_batch_create(byref(h), row_off.numel() - 1, _ptr(row_off), _ptr(row_ptr),
                                          _ptr(col_idx), _ptr(values), _vt(values.dtype), k, byref(opts),

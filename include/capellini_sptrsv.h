@@ -88,6 +88,20 @@ extern "C" {
 #define CAP_READY_STAGED 1
 #define CAP_READY_IN_PLACE 2
 
+/* accumulation arithmetic (cap_opts.arith).  Both add a row's terms in its
+ * stored order; they differ in the rounding of each term.
+ * CAP_ARITH_FUSED: s = fma(v, x, s), one rounding per term (the default).
+ * CAP_ARITH_REFERENCE: s = s + v*x with the product rounded, then the sum --
+ * the arithmetic of the reference's serial CPU executor
+ * (openCL/SynFree_csc/sptrsv_syncfree_serialref.h:67-68, built with g++ -O3
+ * and no FMA contraction), so results are bit-identical to it.  Available
+ * for the thread-per-row, two-part, mix and one-workgroup algorithms (and
+ * the batch and gather adjoint built on them); CSC (atomic, order set by
+ * scheduling) and PARTITION (reassociated) return CAP_ERR_UNSUPPORTED, and
+ * the autotuner leaves them out. */
+#define CAP_ARITH_FUSED 0
+#define CAP_ARITH_REFERENCE 1
+
 /* readiness protocol (cap_opts.sync) */
 #define CAP_SYNC_AUTO 0
 #define CAP_SYNC_VALUE 1 /* the value is the flag: x (or the staged readiness
@@ -121,6 +135,7 @@ typedef struct cap_opts {
                           and the fused refinement sweep: 128 or 256 rows
                           polling per compute unit at one RHS
                           (0 = tuned / default 256) */
+    int arith;         /* CAP_ARITH_* (0 = fused) */
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */

@@ -13,7 +13,8 @@ several right-hand sides), FP32 plans (FP64 tables, b widened and x rounded
 once: within the FP32 tolerance, 1e-5, of the oracle's FP32 sequential
 solve, and within FP32 rounding of the FP64 solve of the same FP32 system),
 and the refusals (bandwidth above 128, too few rows for two blocks, blocks
-whose inverses grow: the plan-creation check that solves L x = L 1).
+whose inverses grow: the plan-creation check that compares the tables'
+solve of two right-hand sides with the sequential solve).
 """
 import numpy as np
 import pytest
@@ -191,6 +192,38 @@ def test_partition_refuses_growing_blocks_and_auto_avoids_it(cuda):
     x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
     assert plan.status() == 0
     assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, fused=True))
+    plan.close()
+
+
+def _growing_bidiagonal(n, a=33.0 / 32.0):
+    """x_i = b_i + a x_{i-1}: unit diagonal, sub-diagonal -a.  With a = 33/32
+    and b = L 1 every step of the sequential solve is exact (x = 1), while the
+    block inverses hold a^k (a^127 = 50) and the scan over the block tails
+    compounds them: on four blocks the partitioned solve is off by about
+    1e-11 -- past the FP64 bar of 1e-12, yet inside the 1e-10 absolute test
+    on L x = L 1 the plan-creation check used to apply."""
+    rp = np.r_[0, np.arange(1, 2 * n, 2)].astype(np.int32)
+    ci = np.r_[0, np.c_[np.arange(0, n - 1), np.arange(1, n)].reshape(-1)].astype(np.int32)
+    return rp, ci, np.r_[1.0, np.tile([-a, 1.0], n - 1)]
+
+
+@pytest.mark.parametrize("n", [512, 640])
+def test_partition_refuses_moderate_growth_and_auto_avoids_it(cuda, n):
+    """The plan-creation check holds the partition to 1e-13 relative of the
+    sequential solve (two right-hand sides: L 1 and a seeded U[-1,1] b)."""
+    import torch
+    from capellinisptrsv_amd import CapError
+    rp, ci, v = _growing_bidiagonal(n)
+    with pytest.raises(CapError) as e:
+        _plan(cuda, rp, ci, v, algo="partition")
+    assert e.value.code == -10  # CAP_ERR_UNSUPPORTED
+    plan = _plan(cuda, rp, ci, v)
+    assert plan.info()["algo"] != "partition"
+    b = np.asarray([v[rp[i]:rp[i + 1]].sum() for i in range(n)])  # L 1: the sequential solve gives x = 1
+    x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+    assert plan.status() == 0
+    assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, fused=True))
+    assert np.array_equal(x, np.ones(n))
     plan.close()
 
 

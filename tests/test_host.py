@@ -96,6 +96,20 @@ def test_read_mtx_malformed_entry(tmp_path):
     assert api.read_mtx(str(p))[0] == _native.CAP_ERR_IO  # truncated
 
 
+def test_read_mtx_oversized_header_counts(tmp_path):
+    """A header whose entry count the file cannot hold is malformed (no
+    allocation of that size, no int32 row-pointer overflow from mirrored
+    symmetric entries); index values beyond int32 are malformed too."""
+    p = tmp_path / "big.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n3 3 2000000000\n2 1\n3 1\n")
+    assert api.read_mtx(str(p))[0] == _native.CAP_ERR_IO
+    p.write_text("%%MatrixMarket matrix coordinate real general\n3 3 1\n1 4294967297 1.0\n")
+    assert api.read_mtx(str(p))[0] == _native.CAP_ERR_IO
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n3 3 2\n2 1\n3 1\n")
+    ret, m, n, nnz, rp, ci, v = api.read_mtx(str(p))
+    assert ret == 0 and nnz == 4 and list(rp) == [0, 2, 3, 4]
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_preprocessing_random_matches_oracle(seed):
     rng = np.random.default_rng(seed)
