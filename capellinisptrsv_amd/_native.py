@@ -53,7 +53,8 @@ class cap_opts(Structure):
     _fields_ = [("algo", c_int), ("sync", c_int), ("border", c_int), ("validate", c_int),
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
                 ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
-                ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int), ("arith", c_int)]
+                ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int), ("arith", c_int),
+                ("wait_limit_ms", c_int)]
 
 
 class cap_gen_params(Structure):

@@ -79,9 +79,10 @@ __device__ __forceinline__ void atomic_add_agent(V *p, V v)
 }
 
 // Readiness protocol state shared by the solve kernels.
-//  flags/epoch: SYNC FLAG only.  spin_limit: polls without progress after
-//  which a lane gives up, publishes NaN (so every dependent row, and with it
-//  the whole grid, still finishes) and sets bit 0 of *err; <= 0 = unbounded.
+//  flags/epoch: SYNC FLAG only.  spin_limit / wait_ticks: polls without
+//  progress / wall time without progress after which a lane gives up,
+//  publishes NaN (so every dependent row, and with it the whole grid, still
+//  finishes) and sets bit 0 of *err; <= 0 = no bound of that kind.
 //  ready: SYNC VALUE only -- the array that is polled and published (same
 //  shape as x, sentinel-filled before the solve); results are also stored
 //  into x with plain stores.  nullptr = x itself is polled.  The library
@@ -98,6 +99,7 @@ struct SyncCtx {
     int *err;
     void *ready;
     int unfused;
+    int wait_ticks;  // wall-time bound of one wait, in 10 ns ticks (<= 0: none)
 };
 
 // One accumulation step of a row's sum, in the plan's arithmetic.  The
@@ -122,13 +124,34 @@ __device__ __forceinline__ const typename VBits<T>::U *ready_array(const SyncCtx
     return sc.ready ? reinterpret_cast<const U *>(sc.ready) : reinterpret_cast<const U *>(x);
 }
 
-__device__ __forceinline__ bool give_up(const SyncCtx &sc, int idle)
+// Watchdog state of one wait: polls since the last progress and the
+// real-time counter (100 MHz, s_memrealtime) read at the kWatchCheck-th of
+// them.  A wait gives up after sc.spin_limit polls (when > 0) or once
+// sc.wait_ticks of wall time have passed since that read (when > 0), so a
+// kernel drains in bounded wall time however long each poll takes under
+// memory contention.  The counter is read once every kWatchCheck polls.
+constexpr unsigned kWatchCheck = 1024;
+struct Idle {
+    unsigned n = 0, t0 = 0;
+    __device__ __forceinline__ void progress() { n = 0; }
+};
+__device__ __forceinline__ bool expired(const SyncCtx &sc, Idle &w)
 {
-    if (sc.spin_limit > 0 && idle > sc.spin_limit) {
-        atomicOr(sc.err, 1);
-        return true;
+    ++w.n;
+    if (sc.spin_limit > 0 && w.n > (unsigned)sc.spin_limit) return true;
+    if ((w.n & (kWatchCheck - 1)) != 0 || sc.wait_ticks <= 0) return false;
+    const unsigned now = (unsigned)__builtin_amdgcn_s_memrealtime();
+    if (w.n == kWatchCheck) {
+        w.t0 = now;
+        return false;
     }
-    return false;
+    return now - w.t0 > (unsigned)sc.wait_ticks;
+}
+__device__ __forceinline__ bool give_up(const SyncCtx &sc, Idle &w)
+{
+    if (!expired(sc, w)) return false;
+    atomicOr(sc.err, 1);
+    return true;
 }
 
 // Persistent ticket loop: the workgroup takes `per_block` work items at a time
@@ -255,16 +278,16 @@ __device__ __forceinline__ T wait_value(const T *x, const SyncCtx &sc, long long
 {
     typedef typename VBits<T>::U U;
     const U *xu = reinterpret_cast<const U *>(x);
-    int idle = 0;
+    Idle idle;
     if (SYNC == 1) {
         const U *xr = ready_array(sc, x);
         U raw;
         while ((raw = ld_agent(xr + o)) == VBits<T>::kSentinel)
-            if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
+            if (give_up(sc, idle)) return as_val(VBits<T>::kQNaN);
         return as_val(raw);
     } else {
         while (ld_agent(sc.flags + o) != sc.epoch)
-            if (give_up(sc, ++idle)) return as_val(VBits<T>::kQNaN);
+            if (give_up(sc, idle)) return as_val(VBits<T>::kQNaN);
         return as_val(ld_agent(xu + o));  // issued after the flag was seen
     }
 }
@@ -313,7 +336,7 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
     const int d = rp[row + 1] - 1;
     const T brow = b[row], vd = val[d];  // loaded now, off the critical path
     T s = (T)0;
-    int idle = 0;
+    Idle idle;
     bool failed = false;
     for (int j0 = rp[row]; j0 < d; j0 += kWave) {
         const int cnt = min(kWave, d - j0);
@@ -337,8 +360,8 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
             const int upto = min(prefix, cnt);
             for (int t = used; t < upto; ++t) s = acc(sc, bcast(v, t), bcast(xv, t), s);
             if (upto > used) {
-                idle = 0;
-            } else if (give_up(sc, ++idle)) {  // uniform: idle is the same in every lane
+                idle.progress();
+            } else if (give_up(sc, idle)) {  // uniform: idle is the same in every lane
                 failed = true;
                 break;
             }
@@ -385,7 +408,7 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
         c1 = n_rev ? n_rev - 1 - ci[j + 1] : ci[j + 1];
         v1 = val[j + 1];
     }
-    int idle = 0;
+    Idle idle;
     bool done = false;
     do {  // wave-uniform exit: see wrtfst_row
         if (!done) {
@@ -416,8 +439,8 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
                 }
             }
             if (prog) {
-                idle = 0;
-            } else if (j < d && give_up(sc, ++idle)) {
+                idle.progress();
+            } else if (j < d && give_up(sc, idle)) {
                 s = as_val(VBits<T>::kQNaN);  // poison this row; it still publishes below
                 j = d;
             }
@@ -459,7 +482,8 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
     T vw[kWin];
     U xw[kWin];
     bool rdy[kWin];
-    int cnt = min(kWin, d - j), used = 0, idle = 0;
+    int cnt = min(kWin, d - j), used = 0;
+    Idle idle;
 #pragma unroll
     for (int t = 0; t < kWin; ++t) {
         rdy[t] = false;
@@ -500,8 +524,8 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
                         ++used;
                     }
                 if (used != before) {
-                    idle = 0;
-                } else if (give_up(sc, ++idle)) {
+                    idle.progress();
+                } else if (give_up(sc, idle)) {
                     s = as_val(VBits<T>::kQNaN);  // poison this row; it still publishes below
                     used = cnt;
                     j = d;
@@ -641,7 +665,7 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
         c1 = ci[j + 1];
         v1 = val[j + 1];
     }
-    int idle = 0;
+    Idle idle;
     bool done = false;
     do {  // wave-uniform exit: see wrtfst_row
         if (!done) {
@@ -672,8 +696,8 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
                 }
             }
             if (prog) {
-                idle = 0;
-            } else if (j < d && give_up(sc, ++idle)) {
+                idle.progress();
+            } else if (j < d && give_up(sc, idle)) {
                 s0 = s1 = __builtin_nanf("");  // poison this row; it still publishes below
                 s = __builtin_nan("");
                 j = d;
@@ -931,7 +955,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
             if (row < hi) {
                 const int a = ca[u], d = ce[u];
                 const int cnt = d - a;
-                int idle = 0;
+                Idle idle;
                 T s = (T)0;
                 if constexpr (K1) {
                     // one right-hand side: lane t polls entry t; values that
@@ -955,7 +979,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                         // (measured 110 -> 104 ms on the deep band)
                         if (cnt - upto <= 2) __builtin_amdgcn_s_setprio(1);
                         if (upto == used) {
-                            if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                            if (expired(sc, idle)) {
                                 failed = true;
                                 s = as_val(VBits<T>::kQNaN);
                                 break;
@@ -967,7 +991,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                         // wavefronts that are still polling (reset after the
                         // publish); measured 117 -> 109 ms on the deep band
                         if (upto == cnt) __builtin_amdgcn_s_setprio(2);
-                        idle = 0;
+                        idle.progress();
                         int t = used;
                         for (; t + 4 <= upto; t += 4) {
                             const T x0 = as_val(xs[t]), x1 = as_val(xs[t + 1]), x2 = as_val(xs[t + 2]),
@@ -994,7 +1018,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                     for (int g = 0; g < kLocalGroup; ++g) {
                         if (u0 + g < cnt) {
                             while (xr[g] == kS) {
-                                if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                                if (expired(sc, idle)) {
                                     failed = true;
                                     xr[g] = VBits<T>::kQNaN;
                                     break;
@@ -1007,7 +1031,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                 }
                 __builtin_amdgcn_s_setprio(2);  // division and publish: the critical path
                 // the slot's previous row and all its readers must be done
-                idle = 0;
+                idle.progress();
                 while (row - R + bw >= front) {
                     int f = hi;
 #pragma unroll
@@ -1015,7 +1039,7 @@ __device__ __forceinline__ void local_rows_ring(const int *__restrict__ rp, cons
                         f = min(f, __hip_atomic_load(s_cur + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
                     front = f;
                     if (row - R + bw < front) break;
-                    if (sc.spin_limit > 0 && ++idle > sc.spin_limit) {
+                    if (expired(sc, idle)) {
                         failed = true;
                         break;
                     }
@@ -1056,7 +1080,7 @@ __device__ __forceinline__ void local_rows_global(const int *__restrict__ rp, co
     const int qk = min(lane, k - 1);
     for (int row = sg.x + w; row < sg.y; row += kWaves) {
         const int a = rp[row], d = rp[row + 1] - 1;
-        int idle = 0;
+        Idle idle;
         T s = (T)0;
         for (int j0 = a; j0 < d; j0 += kWave) {
             const int cnt = min(kWave, d - j0);
@@ -1076,7 +1100,7 @@ __device__ __forceinline__ void local_rows_global(const int *__restrict__ rp, co
                 for (int g = 0; g < kLocalGroup; ++g) {
                     if (u0 + g < cnt) {
                         while (xr[g] == kS) {
-                            if (give_up(sc, ++idle)) {
+                            if (give_up(sc, idle)) {
                                 xr[g] = VBits<T>::kQNaN;
                                 break;
                             }
@@ -1182,11 +1206,11 @@ __global__ void __launch_bounds__(kBlock)
             bc = b[col];
             dg = cv[dpos];
         }
-        int idle = 0;
+        Idle idle;
         do {  // wave-uniform exit
             if (!done) {
                 const bool ready = ld_agent(deg + col) == 1;
-                const bool failed = !ready && give_up(sc, ++idle);
+                const bool failed = !ready && give_up(sc, idle);
                 if (ready || failed) {
                     const T ls = as_val(ld_agent(reinterpret_cast<const U *>(left_sum) + col));
                     const T xi = failed ? as_val(VBits<T>::kQNaN) : (bc - ls) / dg;
@@ -1228,12 +1252,12 @@ __global__ void __launch_bounds__(kBlock)
         const int i = n - 1 - t;
         const int a = rp[i], d = rp[i + 1] - 1;
         const double gi = g[i], vd = val[d];  // before the wait, off the critical path
-        int idle = 0;
+        Idle idle;
         bool done = false;
         do {  // wave-uniform exit, as in wrtfst_row
             if (!done) {
                 const bool ready = ld_agent(cnt + i) == 0;
-                const bool failed = !ready && give_up(sc, ++idle);
+                const bool failed = !ready && give_up(sc, idle);
                 if (ready || failed) {
                     // ordered as in k_csc: left_sum read with an agent-scope
                     // load after the count was seen; the count decrements

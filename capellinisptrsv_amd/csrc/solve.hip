@@ -29,7 +29,13 @@ namespace {
 // 10 for 32-lane warps (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:188); this
 // default is the value measured best on MI355X for the banded workloads.
 constexpr int kMixBorderDefault = 16;
-constexpr int kSpinLimitDefault = 1 << 24;  // polls without progress before the watchdog fires
+// A wait gives up after this much wall time without progress (10 ns ticks of
+// the real-time counter): far above any legitimate wait -- a lane waits only
+// for rows the persistent grid already holds, tens of milliseconds at most
+// on the deepest matrices -- and small enough that a grid that cannot make
+// progress drains in seconds.  No poll-count bound by default.
+constexpr int kWaitMsDefault = 10000;
+constexpr int kWaitMsMax = 40000;  // the 32-bit tick difference wraps at 42.9 s
 // Persistent grids hold at most 4 workgroups per compute unit.  A batch
 // solve with 8 (2048 resident workgroups, one right-hand side) stopped
 // making progress, so larger values are refused.
@@ -88,7 +94,8 @@ struct cap_plan_s {
     unsigned char *ctrl = nullptr;
     unsigned *ticket = nullptr;
     int *err = nullptr;
-    int spin_limit = kSpinLimitDefault;
+    int spin_limit = 0;                   // polls without progress (0: no count bound)
+    int wait_ticks = kWaitMsDefault * 100000;  // wall time without progress, 10 ns ticks
     int ncu = 256, bpc = 1;  // persistent grid: bpc workgroups per compute unit
     int wblock = kBlock;     // threads per workgroup of k_wrtfst (128 or 256)
     int rblock = 0;          // of k_refine2; 0 = not chosen yet (autotuned plans time it)
@@ -416,7 +423,7 @@ int local_ring_log2(const cap_plan p, int rhs, size_t *bytes)
 template <typename T>
 int sync_ctx(cap_plan p, int sync, long long count, SyncCtx *sc)
 {
-    *sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
+    *sc = SyncCtx{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
     if (sync == CAP_SYNC_VALUE && p->staged) {
         if (!p->ready || p->ready_bytes < sizeof(typename VBits<T>::U) * (size_t)count) return CAP_ERR_ARG;
         sc->ready = p->ready;
@@ -579,7 +586,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     const int n = p->n;
     if (n == 0) return CAP_OK;
     if (algo == CAP_ALGO_PARTITION) return launch_partition<T>(p, b, x, rhs, st);
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
     if (fused) sc.unfused = 0;
     if (algo == CAP_ALGO_WRTFST) {
@@ -981,6 +988,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     o.host_inputs = 1;
     o.validate = 0;
     o.spin_limit = p->spin_limit;
+    o.wait_limit_ms = p->wait_ticks > 0 ? p->wait_ticks / 100000 : -1;
     o.readiness = CAP_READY_IN_PLACE;  // plan-time only: keep the uncached pool at the solves' size
     cap_batch b64 = nullptr;
     double *r0 = nullptr, *r1 = nullptr, *v0 = nullptr, *v1 = nullptr;
@@ -1256,6 +1264,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     if (o.blocks_per_cu < 0 || o.blocks_per_cu > kMaxBlocksPerCu) return CAP_ERR_ARG;
     if (o.row_threads != 0 && o.row_threads != kBlock && o.row_threads != kBlockSmall) return CAP_ERR_ARG;
     if (o.arith != CAP_ARITH_FUSED && o.arith != CAP_ARITH_REFERENCE) return CAP_ERR_ARG;
+    if (o.wait_limit_ms > kWaitMsMax) return CAP_ERR_ARG;
     if (o.arith == CAP_ARITH_REFERENCE && (o.algo == CAP_ALGO_CSC || o.algo == CAP_ALGO_PARTITION))
         return CAP_ERR_UNSUPPORTED;
     *out = nullptr;
@@ -1341,6 +1350,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         }
         if ((rc = init_ctrl(p)) != CAP_OK) break;
         if (o.spin_limit > 0) p->spin_limit = o.spin_limit;
+        if (o.wait_limit_ms != 0) p->wait_ticks = o.wait_limit_ms > 0 ? o.wait_limit_ms * 100000 : 0;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         if (o.row_threads > 0) p->wblock = o.row_threads;
         p->lds = o.lds_window > 0;
@@ -1585,7 +1595,7 @@ int cap_plan_adjoint(cap_plan p, const void *g, const void *x, void *grad_b, voi
     CAP_HIP_TRY(hipMemcpyAsync(p->cnt, p->cnt0, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
     k_prepare<unsigned long long><<<blocks_for(n), kBlock, 0, st>>>((unsigned long long *)p->adj_ls, n, 0ULL,
                                                                     p->ticket, kTickets);
-    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused};
+    SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
     k_adjoint<<<grid_for(p, ((long long)n + kBlock - 1) / kBlock), kBlock, 0, st>>>(
         p->rp, p->ci, (const double *)p->val, (const double *)g, (const double *)x, p->cnt, p->adj_ls,
         (double *)grad_b, (double *)grad_val, n, sc, p->ticket);
@@ -1657,7 +1667,7 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     const int2 *sched = use_sched ? p->sched : nullptr;
     const int n_sched = use_sched ? p->n_sched : 0;
     const unsigned nb = grid_for(p, use_sched ? n_sched : ((long long)p->n + rows_per_block - 1) / rows_per_block);
-    const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr};
+    const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, 0, p->wait_ticks};
     const double *val = (const double *)p->val;
     auto go = [&](auto kernel) {
         kernel<<<nb, block, 0, st>>>(p->rp, p->ci, val, b, x, pr, sc, p->n, rhs, l, p->ticket, sched, n_sched,

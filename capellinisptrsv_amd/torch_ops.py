@@ -44,6 +44,8 @@ class Plan:
     upper=True: upper triangular, diagonal first in each row; unit_diagonal=
     True: no diagonal stored (taken as 1).  Both convert the matrix into a
     plan-owned copy; such plans solve, but have no adjoint or refinement.
+    spin_limit / wait_limit_ms: a dependency wait gives up after that many
+    polls / that much wall time without progress (0: no poll bound, 10 s).
     arith: "fused" (one fused multiply-add per term) or "reference" (a
     rounded product then a rounded sum per term: bit-identical to the
     reference's serial CPU executor; not with csc or partition).
@@ -53,7 +55,7 @@ class Plan:
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
                  upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, arith="fused",
-                 stream=None):
+                 wait_limit_ms: int = 0, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -66,7 +68,8 @@ class Plan:
         opts = N.default_opts(algo=algo, sync=sync, border=border, validate=int(validate), host_inputs=0,
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
-                              unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith)
+                              unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith,
+                              wait_limit_ms=wait_limit_ms)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")

@@ -43,8 +43,8 @@ extern "C" {
 #define CAP_ERR_UNSUPPORTED (-10) /* option combination not available */
 #define CAP_ERR_IO (-11)          /* Matrix Market file could not be read */
 #define CAP_ERR_NOT_CONVERGED (-12) /* refinement did not reach its tolerance */
-#define CAP_ERR_WATCHDOG (-13)    /* a dependency wait exceeded spin_limit polls;
-                                     the rows involved hold NaN */
+#define CAP_ERR_WATCHDOG (-13)    /* a dependency wait exceeded spin_limit polls
+                                     or wait_limit_ms; the rows involved hold NaN */
 
 /* value types */
 #define CAP_F64 0
@@ -119,8 +119,8 @@ typedef struct cap_opts {
     int validate;     /* 1 = check the CSR layout before the first solve */
     int host_inputs;  /* 1 = row_ptr/col_idx/val are host pointers (copied) */
     int spin_limit;   /* polls without progress before a wait gives up
-                         (0 = default 2^24); the watchdog never lets a
-                         kernel spin forever on a malformed input */
+                         (0 = no count bound, the default; the wall-time
+                         bound below still applies) */
     int blocks_per_cu; /* persistent grid size in workgroups per compute unit
                           (0 = tuned / default 1; at most 4) */
     int adjoint;       /* CAP_ADJOINT_* (0 = gather) */
@@ -136,6 +136,11 @@ typedef struct cap_opts {
                           polling per compute unit at one RHS
                           (0 = tuned / default 256) */
     int arith;         /* CAP_ARITH_* (0 = fused) */
+    int wait_limit_ms; /* wall time without progress after which a wait gives
+                          up, publishes NaN and sets the watchdog code
+                          (0 = default 10000 ms, < 0 = none, at most 40000):
+                          every kernel drains in bounded time, whatever a
+                          poll costs under memory contention */
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */

@@ -166,12 +166,12 @@ def test_reference_arithmetic_gather_adjoint(cuda):
     gb = gb.cpu().numpy()
     assert plan.status() == 0
     plan.close()
-    cr, cp, cv = oracle.transpose(n, n, nnz, rp, ci, v)
-    indeg = oracle.csc_analyser(cr, n, nnz)
+    # the CSR arrays of L are the CSC arrays of L^T: BACKWARD substitution on them
+    indeg = oracle.csc_analyser(ci, n, nnz)
     if oracle.ref_available("d64"):
-        want = oracle.ref_csc_executor(cp, cr, cv, indeg, n, n, 1, 1, g)
+        want = oracle.ref_csc_executor(rp, ci, v, indeg, n, n, 1, 1, g)
     else:
-        want = oracle.csc_executor(cp, cr, cv, indeg, n, n, 1, 1, g)
+        want = oracle.csc_executor(rp, ci, v, indeg, n, n, 1, 1, g)
     assert np.array_equal(gb, want)
     assert np.array_equal(want, oracle.adjoint(rp, ci, v, g, fused=False))
 

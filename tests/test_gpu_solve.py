@@ -223,6 +223,40 @@ def test_watchdog_forced_from_host(cuda):
     assert torch.isnan(x).any()
 
 
+@pytest.mark.parametrize("algo", ["wrtfst", "mix", "local"])
+def test_watchdog_wall_time_bound(cuda, algo):
+    """No poll-count bound, a 1 ms wall-time bound: a dependency that can never
+    be published (row 10 of a chain also reads row 4000, which depends on row
+    10; layout validation off) makes its wait give up once 1 ms has passed
+    without progress, the row publishes NaN, and every kernel drains --
+    the bound that holds whatever a poll costs."""
+    import time
+    import torch
+    from capellinisptrsv_amd import Plan
+    n = 5000
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        cols = ([i - 1] if i else []) + ([4000] if i == 10 else [])
+        ci += cols + [i]
+        v += [0.5] * len(cols) + [2.0]
+        rp.append(len(ci))
+    rp, ci, v = np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+    d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+    # a poll-count bound far beyond the time bound stays as a backstop (tens
+    # of seconds of polls): the solve must end well before it could fire
+    plan = Plan(*d, algo=algo, sync="value", validate=False, spin_limit=1 << 26, wait_limit_ms=1, border=1)
+    b = torch.ones(n, dtype=torch.float64, device=cuda)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = plan.solve(b)
+    assert plan.status() == -13
+    elapsed = time.perf_counter() - t0
+    assert elapsed < 2.0, elapsed
+    x = x.cpu().numpy()
+    assert not np.isnan(x[:10]).any() and np.isnan(x[10:]).all()
+    plan.close()
+
+
 def test_autotuner_picks_a_correct_path(cuda):
     # deep band (few rows per level): the one-workgroup kernel and, the band
     # being narrow, the partitioned solve are candidates

@@ -254,6 +254,22 @@ def test_plan_create_refuses_bad_grid_options():
         assert not h.value
 
 
+def test_plan_create_checks_the_arithmetic_option():
+    """arith takes CAP_ARITH_FUSED or CAP_ARITH_REFERENCE; the reference
+    arithmetic is refused by the algorithms whose order is not the sequential
+    one (CSC, partition) -- all before any device call."""
+    from ctypes import byref, c_void_p
+    rp, ci, v = random_lower(50, seed=3)
+    h = c_void_p()
+    for arith, algo, want in ((2, "auto", _native.CAP_ERR_ARG), (-1, "auto", _native.CAP_ERR_ARG),
+                              (1, "csc", _native.CAP_ERR_UNSUPPORTED), (1, "partition", _native.CAP_ERR_UNSUPPORTED)):
+        o = _native.default_opts(host_inputs=1, arith=arith, algo=algo)
+        rc = _native.lib().cap_plan_create(byref(h), 50, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                           _native.CAP_F64, byref(o), None)
+        assert rc == want, (arith, algo, rc)
+        assert not h.value
+
+
 def test_sweep_lists_matrices_and_commands(tmp_path):
     """The sweep driver (run.sh equivalent): directory and list-file sources,
     lookup by name under --root, one command per (matrix, algorithm)."""
@@ -282,3 +298,30 @@ def test_bench_refuses_more_than_one_gpu(argv, env):
                        env=e, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_host_preprocessing_under_address_and_ub_sanitizers(tmp_path):
+    """host_prep.cpp built with -fsanitize=address,undefined (aborting on the
+    first report, leaks checked) and driven over every fixture, hostile
+    Matrix Market headers, layout violations and every generator kind
+    (tests/sanitize_host.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "sanitize_host")
+    build = subprocess.run(
+        [gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+         "-fno-omit-frame-pointer", "-I" + os.path.join(root, "include"),
+         os.path.join(root, "tests", "sanitize_host.cpp"),
+         os.path.join(root, "capellinisptrsv_amd", "csrc", "host_prep.cpp"), "-o", exe],
+        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert build.returncode == 0, build.stdout
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    env.pop("LD_PRELOAD", None)
+    run = subprocess.run([exe, os.path.join(root, "tests", "golden", "mtx"), str(tmp_path)], env=env,
+                         stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert run.returncode == 0, run.stdout[-4000:]
+    assert "0 failure(s)" in run.stdout
