@@ -91,6 +91,8 @@ SIGNATURES = {
     "cap_batch_destroy": (c_int, [c_void_p]),
     "sptrsv_csr_solve_d": (c_int, [c_int, c_int, PI, PI, PD, PD, PD, POINTER(cap_opts)]),
     "sptrsv_csr_solve_s": (c_int, [c_int, c_int, PI, PI, PF, PF, PF, POINTER(cap_opts)]),
+    "sptrsv_csr_solve_noalloc_d": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
     "spts_syncfree_hip_csr_wrt_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_csr_wrt_s": (c_int, [PI, PI, PF, c_int, c_int, c_int, PD, PD, PD]),
     "spts_syncfree_hip_csr_algo_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, c_int, PD, PD, PD]),

@@ -2310,6 +2310,85 @@ int sptrsv_csr_solve_s(int n, int nnz, const int *rp, const int *ci, const float
 {
     return oneshot<float>(n, nnz, rp, ci, v, b, x, o);
 }
+}  // extern "C"
+
+// ------------------------------------------------- allocation-free solve
+// The one-shot solve above allocates: a plan (control block with the
+// tickets and the watchdog word, a staged readiness array) and device copies
+// of the inputs.  sptrsv_csr_solve_noalloc_d runs the same write-first
+// kernel on the caller's device arrays with nothing allocated: x itself is
+// the readiness array (sentinel-filled by the reset launch), and the tickets
+// and the watchdog word live in `scratch`, one static array per device that
+// every call on that device shares (calls that overlap on one device race).
+// Its layout: kTickets ticket words, then the watchdog word.
+constexpr size_t kScratchBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static_assert((kTickets + 1) * sizeof(unsigned) <= kScratchBytes, "tickets and the watchdog word fit scratch");
+
+namespace {
+
+struct ScratchDev {
+    unsigned char *ptr = nullptr;  // device address of this device's copy of scratch
+    int ncu = 0;
+};
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+ScratchDev g_scratch[kMaxDevices];
+
+// Set-up on a device's first call, with that device current: checks that it
+// is a gfx950, takes the device address of its copy of `scratch` (the host
+// name is only a stand-in for it) and its compute-unit count.
+int scratch_of(int dev, ScratchDev *out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return CAP_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    ScratchDev &sd = g_scratch[dev];
+    if (!sd.ptr) {
+        int cur = -1;
+        CAP_TRY(check_device(&cur));
+        if (cur != dev) return CAP_ERR_NO_DEVICE;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "capellini: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return CAP_ERR_HIP;
+        }
+        int ncu = 0;
+        CAP_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        sd.ncu = ncu > 0 ? ncu : 256;
+        sd.ptr = static_cast<unsigned char *>(p);
+    }
+    *out = sd;
+    return CAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sptrsv_csr_solve_noalloc_d(int n, int nnz, const int *row_ptr, const int *col_idx, const double *val,
+                               const double *b, double *x, void *stream)
+{
+    typedef unsigned long long U;
+    if (n < 0 || nnz < n || !row_ptr || (n > 0 && (!col_idx || !val || !b || !x))) return CAP_ERR_ARG;
+    int dev = 0;
+    CAP_HIP_TRY(hipGetDevice(&dev));
+    ScratchDev sd;
+    CAP_TRY(scratch_of(dev, &sd));
+    if (n == 0) return CAP_OK;
+    hipStream_t st = S(stream);
+    unsigned *ticket = reinterpret_cast<unsigned *>(sd.ptr);
+    int *err = reinterpret_cast<int *>(sd.ptr + kTickets * sizeof(unsigned));
+    // one reset launch: x <- sentinel, tickets and the watchdog word <- 0
+    const unsigned nbp = std::max(1u, std::min(blocks_for(n), 2048u));
+    k_prepare<U><<<nbp, kBlock, 0, st>>>(reinterpret_cast<U *>(x), n, VBits<double>::kSentinel, ticket, kTickets + 1);
+    const SyncCtx sc{nullptr, 0, 0, err, nullptr, 0, kWaitMsDefault * 100000};
+    const unsigned nb = (unsigned)std::max(1LL, std::min<long long>(sd.ncu, ((long long)n + kBlock - 1) / kBlock));
+    k_wrtfst<double, 1, 0, kBlock><<<nb, kBlock, 0, st>>>(row_ptr, col_idx, val, b, x, sc, n, 1, 0, ticket, 0,
+                                                            nullptr, 0);
+    CAP_HIP_TRY(hipGetLastError());
+    return CAP_OK;
+}
 
 int spts_syncfree_hip_csr_wrt_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,
                                 double *gf, double *bw)

@@ -234,6 +234,25 @@ int sptrsv_csr_solve_s(int n, int nnz, const int *row_ptr, const int *col_idx,
                        const float *val, const float *b, float *x,
                        const cap_opts *opts);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the one-shot solve above (one right-hand side, FP64,
+ * thread-per-row write-first kernel, fused arithmetic) on the caller's
+ * DEVICE arrays, with nothing allocated.  x itself is the readiness array;
+ * the tickets and the watchdog word live in a static 256-byte array of the
+ * library on each device (held until the process ends) that every call on
+ * that device shares: calls that overlap on one device race, so serialize
+ * them (one stream, or events).  Limits: the kernel's own (n < 2^31 rows);
+ * the scratch does not grow with the input.  Asynchronous on `stream`; the
+ * CSR layout is not validated (no host copy is made) -- a wait that exceeds
+ * the default 10 s wall-time bound gives up and leaves NaN in the rows
+ * involved.  Returns CAP_OK, CAP_ERR_ARG, CAP_ERR_NO_DEVICE or CAP_ERR_HIP.
+ * The first call on a device does its set-up (device check, the scratch
+ * address), which is not an allocation either. */
+int sptrsv_csr_solve_noalloc_d(int n, int nnz, const int *row_ptr,
+                               const int *col_idx, const double *val,
+                               const double *b, double *x, void *stream);
+
 /* -------------------------------------------- benchmark-compatible entries
  * Same parameter lists and semantics as the reference's wrappers: build
  * x_ref = 1 and b = L*x_ref internally (get_x_b), run a warm-up plus
