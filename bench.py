@@ -8,8 +8,9 @@ resident in HBM.  A step is the per-solve reset plus the solve kernel.
 
 Prints one JSON line:
   value        GFLOPS = 2 * nnzL / step time (nnzL includes the diagonal)
-  roofline     solve kernel alone: algorithmic bytes / mean kernel time vs
-               8 TB/s; traffic from the committed rocprofv3 counter summary
+  roofline     solve kernel alone: algorithmic bytes (the reference's
+               dataSize) / median MALL-flushed kernel time vs 8 TB/s (warm
+               beside it); traffic from the committed rocprofv3 counters
   cpu_baseline the reference's serial executor (serialref.h, compiled from
                its own sources into oracle/_ref) on one host core, same matrix
   extras       FP32 solve, FP32 solve + FP64 refinement to 1e-12 (error vs
@@ -91,22 +92,30 @@ def main():
         torch.cuda.synchronize()
     synced_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
-    # the solve kernel alone, HIP events on the stream it is launched on
+    # the solve kernel alone, HIP events on the stream it is launched on:
+    # MALL-warm (back to back, as the reference measures) and MALL-flushed
+    # (a 512 MB fill before each launch), medians of >= 20 launches each
     stream = torch.cuda.current_stream()
-    kms = []
-    for _ in range(max(args.steps, 20)):
-        plan.reset(x)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        plan.launch(b, x)
-        e1.record(stream)
-        e1.synchronize()
-        kms.append(e0.elapsed_time(e1))
-    kernel_ms = float(np.mean(kms))
+    flush = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+
+    def kernel_times(cold):
+        ts = []
+        for _ in range(max(args.steps, 20)):
+            plan.reset(x)
+            if cold:
+                flush.fill_(1.0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.launch(b, x)
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+    kernel_ms_warm = kernel_times(False)
+    kernel_ms = kernel_times(True)
     assert torch.equal(x, x64)
 
-    # cold: MALL (256 MB) flushed before every step, each step timed alone
-    flush = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    # cold step: MALL flushed before every step, each step timed alone
     cold = []
     for _ in range(10):
         flush.fill_(1.0)
@@ -117,6 +126,34 @@ def main():
         e1.synchronize()
         cold.append(e0.elapsed_time(e1))
     del flush
+
+    # spin counter: the same tuning in the counting build (poll loads per row)
+    spin = None
+    if info["algo"] == "wrtfst" and info["sync"] == "value":
+        pc = Plan(d_rp, d_ci, d_v, algo="wrtfst", sync="value", lds_window=info["lds_window"],
+                  blocks_per_cu=info["blocks_per_cu"], readiness=info["readiness"], row_threads=info["row_threads"],
+                  count_polls=True)
+        xc_ = torch.empty_like(b)
+        pc.solve(b, out=xc_)
+        pc.poll_count()
+        polls = []
+        for _ in range(5):
+            pc.solve(b, out=xc_)
+            polls.append(pc.poll_count())
+        assert torch.equal(xc_, x64)
+        pc.close()
+        polls = float(np.median(polls))
+        spin = {"polls_per_row": round(polls / n, 3), "polls_per_dependency": round(polls / (nnz - n), 3),
+                "polls_per_solve": int(polls)}
+
+    # the reference arithmetic (rounded product, then rounded sum): same tuning
+    pr = Plan(d_rp, d_ci, d_v, algo="wrtfst", sync="value", lds_window=info["lds_window"],
+              blocks_per_cu=info["blocks_per_cu"], readiness=info["readiness"], row_threads=info["row_threads"],
+              arith="reference")
+    x_refarith = torch.empty_like(b)
+    refarith_ms = _time_gpu(lambda: pr.solve(b, out=x_refarith), args.steps, warmup=3)
+    assert pr.status() == 0
+    pr.close()
 
     # FP32 solve, and FP32 solves inside FP64 refinement
     plan32 = Plan(d_rp, d_ci, d_v.float(), algo=info["algo"] if info["algo"] != "csc" else "wrtfst")
@@ -140,11 +177,14 @@ def main():
     torch.cuda.synchronize()
     refined_ms = (time.perf_counter() - t0) * 1e3 / reps
 
-    # algorithmic bytes of one solve: row_ptr, col_idx, values, b read once, x written once
+    # algorithmic bytes of one solve (the reference's dataSize,
+    # spts_syncfree_cuda_csr_wrt.h:208): row_ptr, col_idx and values read
+    # once, a 4-byte readiness word per row, b read once, x written once
     vb = 8
-    alg_bytes = 4 * (n + 1) + (4 + vb) * nnz + 2 * vb * n
+    alg_bytes = 4 * (n + 1) + (4 + vb) * nnz + 4 * n + 2 * vb * n
     ref_datasize = api.datasize(n, n, nnz, vb)  # the reference's formula (includes its int flag array)
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    achieved_warm = alg_bytes / (kernel_ms_warm * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_wrtfst_wide_level_2M.json")
     if os.path.exists(tpath):
@@ -168,8 +208,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_wrtfst" if info["algo"] == "wrtfst" else "k_" + info["algo"],
-                     "kernel_ms": round(kernel_ms, 5), "algorithmic_bytes": alg_bytes},
+                     "kernel_ms": round(kernel_ms, 5), "kernel_ms_is": "median, MALL flushed before each launch",
+                     "kernel_ms_warm": round(kernel_ms_warm, 5), "achieved_warm": round(achieved_warm, 2),
+                     "frac_warm": round(achieved_warm / HBM_PEAK_GBS, 5), "algorithmic_bytes": alg_bytes},
         "per_level_ns": round(kernel_ms * 1e6 / levels, 1),
+        "spin_per_row": spin,
+        "reference_arith_ms_per_step": round(refarith_ms, 5),
         # latency check: the per-level time 40 % of the HBM roofline would need
         "per_level_ns_for_40pct_roofline": round(alg_bytes / (0.4 * HBM_PEAK_GBS * 1e9) / levels * 1e9, 1),
         "cold_ms_per_step": round(float(np.median(cold)), 5),
@@ -214,11 +258,14 @@ def main():
         ref = oracle.csr_forward(rp, ci, v, b_h, fused=False)
     den = float(np.max(np.abs(ref)))
     out["fp64_err_vs_cpu"] = float(np.max(np.abs(x64.cpu().numpy() - ref)) / den)
+    # the reference arithmetic: bit-identical to the reference's executor
+    out["fp64_err_vs_cpu_reference_arith"] = float(np.max(np.abs(x_refarith.cpu().numpy() - ref)) / den)
+    out["reference_arith_bit_identical"] = bool(np.array_equal(x_refarith.cpu().numpy(), ref))
     out["refined_err_vs_cpu_fp64"] = float(np.max(np.abs(xr.cpu().numpy() - ref)) / den)
     out["fp32_err_vs_cpu_fp64"] = float(np.max(np.abs(x32.double().cpu().numpy() - ref)) / den)
     plan.close()
     plan32.close()
-    del d_rp, d_ci, d_v, b, x, xr, x32, b32
+    del d_rp, d_ci, d_v, b, x, xr, x32, b32, x_refarith
     if not args.quick:
         out["extra_workloads"] = extra_workloads(args)
     print(json.dumps(out), flush=True)
@@ -353,7 +400,6 @@ def extra_workloads(args):
     # batched: 256 matrices, n log-uniform in [2,000, 200,000], nnz/row U{1..16}
     # (diagonal included), band U[64, 4096], k = 16 interleaved RHS
     log("extra workload batched_256x16")
-    rng = np.random.default_rng(0)
     mats, row_off, rp_all, ci_all, v_all = api.make_batch(256, seed=0)
     k = 16
     N = int(row_off[-1])
@@ -371,19 +417,20 @@ def extra_workloads(args):
         holder["r"] = bt.solve_refined(bd, tol=1e-12, out=xr)
     ref_ms = _time_gpu(refined, 3, warmup=1)
     _, iters, rel = holder["r"]
-    # error against the FP64 oracle on a sample of matrices
+    # error against the FP64 oracle, every matrix
     worst64, worstr = 0.0, 0.0
-    for m in rng.choice(256, size=6, replace=False):
+    x64h, xrh = x64.reshape(-1).cpu().numpy(), xr.reshape(-1).cpu().numpy()
+    for m in range(256):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
         o = oracle.csr_forward(*mats[m], bb[lo:hi], rhs=k, fused=True)
         den = float(np.max(np.abs(o)))
-        worst64 = max(worst64, float(np.max(np.abs(x64.reshape(-1)[lo:hi].cpu().numpy() - o))) / den)
-        worstr = max(worstr, float(np.max(np.abs(xr.reshape(-1)[lo:hi].cpu().numpy() - o))) / den)
+        worst64 = max(worst64, float(np.max(np.abs(x64h[lo:hi] - o))) / den)
+        worstr = max(worstr, float(np.max(np.abs(xrh[lo:hi] - o))) / den)
     res["batched_256x16"] = {
         "matrices": 256, "rows": N, "nnzL": int(len(ci_all)), "k": k,
         "fp64_ms": round(fp64_ms, 3), "fp64_gflops": round(2 * len(ci_all) * k / (fp64_ms * 1e6), 2),
         "fp32_refined_ms": round(ref_ms, 3), "refined_iterations": iters, "refined_over_fp64": round(ref_ms / fp64_ms, 3),
-        "fp64_rel_err_sampled": worst64, "refined_rel_err_sampled": worstr,
+        "fp64_rel_err_max_over_matrices": worst64, "refined_rel_err_max_over_matrices": worstr,
     }
     bt.close()
     res["rocsparse_csrsv"] = rocsparse_comparison()

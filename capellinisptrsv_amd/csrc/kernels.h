@@ -100,7 +100,11 @@ struct SyncCtx {
     void *ready;
     int unfused;
     int wait_ticks;  // wall-time bound of one wait, in 10 ns ticks (<= 0: none)
+    // kPollCounters counters of the polls the value-protocol write-first
+    // solve issued (its counting instantiation only; nullptr otherwise)
+    unsigned long long *polls;
 };
+constexpr int kPollCounters = 64;
 
 // One accumulation step of a row's sum, in the plan's arithmetic.  The
 // operand order and the stored-order chain are the same in both forms; only
@@ -384,11 +388,13 @@ __device__ __forceinline__ void wave_row(const int *__restrict__ rp, const int *
 // (entries j and j+1) is polled at once and slides by the number consumed,
 // so most rows (one or two off-diagonals) are covered by a single window and
 // the loop body stays short.  Consumption is in stored order (fma chain).
-template <typename T>
+// COUNT: add the number of poll loads issued to *np (the per-row spin
+// counter of the counting instantiation).
+template <typename T, bool COUNT = false>
 __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, const T *__restrict__ val,
                                                  const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
                                                  int i, int j, int d, int k, int q, T s, int n_rev,
-                                                 const LdsWin<typename VBits<T>::U> &win)
+                                                 const LdsWin<typename VBits<T>::U> &win, unsigned *np = nullptr)
 {
     typedef typename VBits<T>::U U;
     constexpr U kS = VBits<T>::kSentinel;
@@ -413,6 +419,7 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
     do {  // wave-uniform exit: see wrtfst_row
         if (!done) {
             bool prog = false;
+            if (COUNT) *np += (j < d && x0 == kS) + (j + 1 < d && x1 == kS);
             if (j < d && x0 == kS) x0 = win.has(c0) ? win.load(c0, q) : ld_agent(xr + (long long)c0 * k + q);
             if (j + 1 < d && x1 == kS) x1 = win.has(c1) ? win.load(c1, q) : ld_agent(xr + (long long)c1 * k + q);
             if (j < d && x0 != kS) {
@@ -463,14 +470,15 @@ __device__ __forceinline__ void wrtfst_row_value(const int *__restrict__ ci, con
 // carries a partial sum when the caller already consumed a prefix (two-part).
 // n_rev > 0 stores row r at position n_rev-1-r of b and x (the adjoint solves
 // L^T in reversed row order, which is lower triangular).
-template <typename T, int SYNC>
+template <typename T, int SYNC, bool COUNT = false>
 __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *__restrict__ val,
                                            const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
                                            int i, int j, int d, int k, int q, T s, int n_rev = 0,
-                                           LdsWin<typename VBits<T>::U> win = {nullptr, 0, 0, 1})
+                                           LdsWin<typename VBits<T>::U> win = {nullptr, 0, 0, 1},
+                                           unsigned *np = nullptr)
 {
     if constexpr (SYNC == 1) {
-        wrtfst_row_value<T>(ci, val, b, x, sc, i, j, d, k, q, s, n_rev, win);
+        wrtfst_row_value<T, COUNT>(ci, val, b, x, sc, i, j, d, k, q, s, n_rev, win, np);
         return;
     }
     // flag protocol: a fixed window of up to kWin entries, refilled when consumed
@@ -565,17 +573,22 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 // host orders the blocks so that every row comes after the rows it depends
 // on (each matrix of a batch keeps its row order), which keeps the
 // forward-progress argument of the ticket loop.
-template <typename T, int SYNC, int LDS, int BLOCK>
+// COUNT (value protocol only): every lane counts the poll loads it issues;
+// each wavefront adds its total to one of sc.polls' kPollCounters counters
+// when the grid drains (one atomic per wavefront).
+template <typename T, int SYNC, int LDS, int BLOCK, bool COUNT = false>
 __global__ void __launch_bounds__(BLOCK)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
              unsigned *__restrict__ ticket, int n_rev, const int2 *__restrict__ sched, int n_sched)
 {
     typedef typename VBits<T>::U U;
+    static_assert(!COUNT || SYNC == 1, "the poll counter is for the value protocol");
     __shared__ U s_x[LDS ? BLOCK : 1];
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
     const int rows = BLOCK >> lpr_log2;
+    unsigned np = 0;
     ticket_loop(
         ticket, sched ? 1 : rows, sched ? n_sched : n,
         [&](int) {
@@ -591,8 +604,16 @@ __global__ void __launch_bounds__(BLOCK)
             const int i = lo + r;
             LdsWin<U> win{s_x, lo, (LDS && !n_rev) ? len : 0, k};
             if (r < len && i < n && q < k)
-                wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win);
+                wrtfst_row<T, SYNC, COUNT>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win,
+                                           &np);
         });
+    if constexpr (COUNT) {
+        unsigned long long t = np;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) t += (unsigned long long)__shfl_xor((long long)t, o, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            atomic_add_agent(sc.polls + (blockIdx.x * (BLOCK / kWave) + threadIdx.x / kWave) % kPollCounters, t);
+    }
 }
 
 // ------------------------------------------------------------ refinement

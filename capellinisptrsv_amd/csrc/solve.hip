@@ -90,7 +90,7 @@ struct cap_plan_s {
     long long flags_len = 0;
     int epoch = 0;
     // control block (one 4 KiB allocation): tickets at +0, watchdog word at
-    // +256, refinement norms at +512
+    // +256, refinement norms at +512, poll counters at +1024
     unsigned char *ctrl = nullptr;
     unsigned *ticket = nullptr;
     int *err = nullptr;
@@ -101,6 +101,7 @@ struct cap_plan_s {
     int rblock = 0;          // of k_refine2; 0 = not chosen yet (autotuned plans time it)
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
     int unfused = 0;         // CAP_ARITH_REFERENCE: rounded products, then rounded sums
+    int count_polls = 0;     // the write-first solve counts its poll loads (ctrl + 1024)
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
     int n_sched = 0, sched_rows = 0;
@@ -603,7 +604,13 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
             kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched, n_sched);
         };
         const bool small = p->wblock == kBlockSmall;
-        if (sync == CAP_SYNC_VALUE && p->lds)
+        if (sync == CAP_SYNC_VALUE && p->count_polls) {  // the counting instantiations
+            sc.polls = reinterpret_cast<unsigned long long *>(p->ctrl + 1024);
+            if (p->lds)
+                small ? go(k_wrtfst<T, 1, 1, kBlockSmall, true>) : go(k_wrtfst<T, 1, 1, kBlock, true>);
+            else
+                small ? go(k_wrtfst<T, 1, 0, kBlockSmall, true>) : go(k_wrtfst<T, 1, 0, kBlock, true>);
+        } else if (sync == CAP_SYNC_VALUE && p->lds)
             small ? go(k_wrtfst<T, 1, 1, kBlockSmall>) : go(k_wrtfst<T, 1, 1, kBlock>);
         else if (sync == CAP_SYNC_VALUE)
             small ? go(k_wrtfst<T, 1, 0, kBlockSmall>) : go(k_wrtfst<T, 1, 0, kBlock>);
@@ -1355,6 +1362,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.row_threads > 0) p->wblock = o.row_threads;
         p->lds = o.lds_window > 0;
         p->unfused = o.arith == CAP_ARITH_REFERENCE;
+        p->count_polls = o.count_polls != 0;
         p->staged = o.readiness != CAP_READY_IN_PLACE;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
@@ -1542,6 +1550,23 @@ int cap_plan_status(cap_plan p, void *stream)
     CAP_HIP_TRY(hipMemcpy(&e, p->err, sizeof(int), hipMemcpyDeviceToHost));
     if (e) CAP_HIP_TRY(hipMemset(p->err, 0, sizeof(int)));
     return e ? CAP_ERR_WATCHDOG : CAP_OK;
+}
+
+int cap_plan_poll_count(cap_plan p, unsigned long long *polls, void *stream)
+{
+    if (!p || !polls) return CAP_ERR_ARG;
+    if (!p->count_polls) return CAP_ERR_UNSUPPORTED;
+    hipStream_t st = S(stream);
+    unsigned long long h[kPollCounters];
+    unsigned long long *d = reinterpret_cast<unsigned long long *>(p->ctrl + 1024);
+    CAP_HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+    CAP_HIP_TRY(hipStreamSynchronize(st));
+    k_prepare<unsigned long long><<<1, kBlock, 0, st>>>(d, kPollCounters, 0ULL, p->ticket, 0);
+    CAP_HIP_TRY(hipGetLastError());
+    unsigned long long sum = 0;
+    for (unsigned long long v : h) sum += v;
+    *polls = sum;
+    return CAP_OK;
 }
 
 int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint, int *readiness,

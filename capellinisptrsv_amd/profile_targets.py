@@ -14,6 +14,9 @@
     python -m capellinisptrsv_amd.profile_targets refine [--steps N]
         the wide-level matrix: N FP64 solves, then N FP32 + FP64 refined
         solves (fused first two steps) to 1e-12
+    python -m capellinisptrsv_amd.profile_targets band [--steps N] [--algo auto]
+        the n = 100,000 band (3 off-diagonals in the previous 1,000 rows,
+        723 levels), N reset+solve steps
     python -m capellinisptrsv_amd.profile_targets deep [--steps N] [--algo auto] [--rhs K]
         the n = 500,000 bandwidth-128 matrix (~38 nnz/row, 197,032 levels);
         --algo partition builds the partitioned solve without the autotuner;
@@ -25,7 +28,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched", "deep", "refine"])
+    ap.add_argument("target", choices=["solve", "copy", "adjoint", "batched", "deep", "refine", "band"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--algo", default="auto")
     ap.add_argument("--mib", type=int, default=1024)
@@ -61,7 +64,7 @@ def main(argv=None):
         print(f"batched: n={n} nnz={len(ci)} k={k} steps={a.steps} refined iterations={it}", flush=True)
         bt.close()
         return 0
-    cfg = {"solve": 1, "refine": 1, "adjoint": 4, "deep": 2}[a.target]
+    cfg = {"solve": 1, "refine": 1, "adjoint": 4, "deep": 2, "band": 0}[a.target]
     rp, ci, v = api.make_config(cfg, seed=0)
     n = len(rp) - 1
     d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
@@ -69,7 +72,8 @@ def main(argv=None):
     if a.target == "deep" and a.rhs > 1:
         b = torch.from_numpy(api.uniform(1, n * a.rhs).reshape(n, a.rhs)).to(dev)
     x = torch.empty_like(b)
-    plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep", "refine") else Plan(*d, algo="wrtfst", sync="value")
+    plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep", "refine", "band") else \
+        Plan(*d, algo="wrtfst", sync="value")
     for _ in range(a.steps):
         plan.solve(b, out=x)
         if a.target == "adjoint":

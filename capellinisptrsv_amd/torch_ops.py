@@ -55,7 +55,7 @@ class Plan:
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
                  upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, arith="fused",
-                 wait_limit_ms: int = 0, stream=None):
+                 wait_limit_ms: int = 0, count_polls: bool = False, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -69,7 +69,7 @@ class Plan:
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
                               unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith,
-                              wait_limit_ms=wait_limit_ms)
+                              wait_limit_ms=wait_limit_ms, count_polls=int(count_polls))
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -88,6 +88,14 @@ class Plan:
                     warp_occupy=wo.value, element_occupy=eo.value, blocks_per_cu=bpc.value,
                     lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"),
                     readiness=N.READINESS_NAME.get(rdy.value, "?"), row_threads=thr.value)
+
+    def poll_count(self, stream=None) -> int:
+        """Poll loads issued by the thread-per-row solves since the last call
+        (plans created with count_polls=True); synchronizes and clears."""
+        from ctypes import c_ulonglong
+        v = c_ulonglong()
+        N.check(N.lib().cap_plan_poll_count(self._h, byref(v), _stream(stream)), "cap_plan_poll_count")
+        return v.value
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

@@ -136,6 +136,9 @@ typedef struct cap_opts {
                           polling per compute unit at one RHS
                           (0 = tuned / default 256) */
     int arith;         /* CAP_ARITH_* (0 = fused) */
+    int count_polls;   /* 1 = the thread-per-row solve (value protocol) runs its
+                          counting build: every poll load is counted, read
+                          with cap_plan_poll_count (evidence, not speed) */
     int wait_limit_ms; /* wall time without progress after which a wait gives
                           up, publishes NaN and sets the watchdog code
                           (0 = default 10000 ms, < 0 = none, at most 40000):
@@ -168,6 +171,13 @@ int cap_plan_launch(cap_plan plan, const void *b, void *x, int rhs, void *stream
 /* Synchronizes `stream`, then reads and clears the plan's watchdog word:
  * CAP_OK, or CAP_ERR_WATCHDOG if any wait gave up since the last call. */
 int cap_plan_status(cap_plan plan, void *stream);
+/* No counterpart in the reference, whose d_while_profiler
+ * (CUDA/SyncFree_csc/spts_syncfree_cuda2.h:180-184) was allocated but never
+ * written: synchronizes `stream`, returns the number of poll loads the
+ * plan's thread-per-row solves issued since the last call (plans created
+ * with cap_opts.count_polls = 1; CAP_ERR_UNSUPPORTED otherwise) and clears
+ * it.  Divided by rows x right-hand sides it is the spin count per row. */
+int cap_plan_poll_count(cap_plan plan, unsigned long long *polls, void *stream);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table. */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);

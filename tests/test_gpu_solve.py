@@ -326,3 +326,29 @@ def test_local_long_rows_use_global_mode_bit_exact(cuda, k):
     x, info = _solve(cuda, rp, ci, v, b, "local", "value", k=k)
     assert info["algo"] == "local"
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("lds", [False, True])
+def test_poll_counter_build(cuda, lds):
+    """The counting build of the write-first solve gives the same bits and
+    counts at least one poll per off-diagonal entry (every dependency is read
+    at least once); reading clears the count; plans without it refuse."""
+    import torch
+    from capellinisptrsv_amd import CapError, Plan
+    rp, ci, v = api.make_config(1, seed=0, scale=0.05)
+    n = len(rp) - 1
+    d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+    b = api.uniform(71, n)
+    plan = Plan(*d, algo="wrtfst", sync="value", lds_window=lds, count_polls=True)
+    for _ in range(2):
+        x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+        assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, fused=True))
+        polls = plan.poll_count()
+        assert polls >= len(ci) - n, polls
+        assert polls < 1000 * len(ci), polls
+        assert plan.poll_count() == 0
+    plan.close()
+    plain = Plan(*d, algo="wrtfst", sync="value")
+    with pytest.raises(CapError):
+        plain.poll_count()
+    plain.close()
