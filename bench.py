@@ -362,7 +362,8 @@ def extra_workloads(args):
             ref = oracle.csr_forward(rp, ci, v, bh, rhs=k, fused=True).reshape(n, k)
             err = float(np.max(np.abs(xk.cpu().numpy() - ref)))
             res[name]["rhs16"] = {"ms_per_solve": round(ms_k, 4), "ms_per_rhs": round(ms_k / k, 4),
-                                  "rel_err_vs_oracle": err / float(np.max(np.abs(ref)))}
+                                  "rel_err_vs_oracle": err / float(np.max(np.abs(ref))),
+                                  "rhs_block": p.rhs_block(k)}
         p.close()
         if cfg == 2:
             # the same matrix and b in FP32 (autotuned plan), against the FP32 oracle
@@ -377,6 +378,29 @@ def extra_workloads(args):
             res[name]["fp32"] = {"algo": p32.info()["algo"], "ms_per_solve": round(ms32, 4),
                                  "rel_err_vs_fp32_oracle": err32 / float(np.max(np.abs(ref32)))}
             p32.close()
+
+    # 16 right-hand sides on the wide-level matrix: the autotuned plan picks
+    # its RHS block (workgroups per CU, threads per workgroup) on the first
+    # 16-RHS solve; the fixed plan keeps one 256-thread workgroup per CU
+    log("extra workload wide_level_2m_rhs16")
+    rp, ci, v = api.make_config(1, seed=0)
+    n, nnz, k = len(rp) - 1, len(ci), 16
+    d = [torch.from_numpy(a).to(dev) for a in (rp, ci, v)]
+    bh = api.uniform(6, n * k)
+    bk = torch.from_numpy(bh.reshape(n, k)).to(dev)
+    xk = torch.empty_like(bk)
+    ref = oracle.csr_forward(rp, ci, v, bh, rhs=k, fused=True).reshape(n, k)
+    r = {"n": n, "nnzL": nnz, "k": k}
+    for name, kw in (("tuned", {}), ("fixed", dict(algo="wrtfst", sync="value", blocks_per_cu=1, row_threads=256))):
+        p = Plan(*d, **kw)
+        ms_k = _time_gpu(lambda: p.solve(bk, out=xk), 10, warmup=2)
+        assert p.status() == 0
+        r[name] = {"ms_per_solve": round(ms_k, 4), "ms_per_rhs": round(ms_k / k, 4),
+                   "gflops": round(2 * nnz * k / (ms_k * 1e6), 2),
+                   "bit_exact_vs_oracle": bool(np.array_equal(xk.cpu().numpy(), ref)),
+                   "rhs_block": p.rhs_block(k), "plan": p.info()["algo"]}
+        p.close()
+    res["wide_level_2m_rhs16"] = r
 
     # differentiable solve: n = 1,000,000 level matrix, forward + adjoint
     log("extra workload adjoint_1m")

@@ -78,6 +78,13 @@ struct PartState {
     double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction (n * kcap)
 };
 
+// The RHS-block pick of an autotuned plan for one right-hand-side count:
+// thread-per-row workgroups per CU and threads per workgroup, or the
+// partitioned block solve's right-hand sides per wavefront (0: the plan's).
+struct RhsPick {
+    int bpc = 0, block = 0, kr = 0;
+};
+
 struct cap_plan_s {
     int n = 0, nnz = 0, vt = CAP_F64, vbytes = 8;
     int algo = CAP_ALGO_WRTFST, sync = CAP_SYNC_VALUE, border = kMixBorderDefault;
@@ -102,6 +109,11 @@ struct cap_plan_s {
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
     int unfused = 0;         // CAP_ARITH_REFERENCE: rounded products, then rounded sums
     int count_polls = 0;     // the write-first solve counts its poll loads (ctrl + 1024)
+    // multi-RHS solves of an autotuned plan: the RHS block is timed per
+    // right-hand-side count on the first cap_plan_solve with that count
+    bool tune_rhs = false;
+    std::map<int, RhsPick> rhs_pick;
+    int part_kr = 0;  // partition block solve: right-hand sides per wavefront (0: the fixed rule)
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
     int n_sched = 0, sched_rows = 0;
@@ -555,7 +567,12 @@ int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
     PartState *ps = p->part;
     if (!ps || k < 1 || k > kWave || ps->m < kPartW) return CAP_ERR_ARG;
     CAP_TRY(ensure_part_rhs(p, k));
-    switch (part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)))) {
+    // right-hand sides per wavefront of the block solve: the plan's tuned
+    // pick for k when it has one, else the largest power of two up to
+    // min(4, k / 2) that divides k
+    int kr = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
+    if (p->part_kr > 0 && p->part_kr <= kPartBlockRhs && k % p->part_kr == 0) kr = p->part_kr;
+    switch (kr) {
     case 4: launch_part_block<4, T>(p, b, k, st); break;
     case 2: launch_part_block<2, T>(p, b, k, st); break;
     default: launch_part_block<1, T>(p, b, k, st); break;
@@ -577,6 +594,27 @@ int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
     return CAP_OK;
 }
 
+// Applies the plan's RHS-block pick for k right-hand sides for the length
+// of one launch (and puts the plan's own settings back).
+struct RhsScope {
+    cap_plan p;
+    int bpc, block, kr;
+    RhsScope(cap_plan q, int k) : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr)
+    {
+        const auto it = k > 1 ? p->rhs_pick.find(k) : p->rhs_pick.end();
+        if (it == p->rhs_pick.end()) return;
+        if (it->second.bpc) p->bpc = it->second.bpc;
+        if (it->second.block) p->wblock = it->second.block;
+        if (it->second.kr) p->part_kr = it->second.kr;
+    }
+    ~RhsScope()
+    {
+        p->bpc = bpc;
+        p->wblock = block;
+        p->part_kr = kr;
+    }
+};
+
 // fused: accumulate with fused multiply-adds whatever the plan's arithmetic
 // (the refinement's FP32 corrections, which belong to this library's own
 // method and follow the oracle's fused steps).
@@ -586,6 +624,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
 {
     const int n = p->n;
     if (n == 0) return CAP_OK;
+    const RhsScope scope(p, rhs);
     if (algo == CAP_ALGO_PARTITION) return launch_partition<T>(p, b, x, rhs, st);
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
@@ -814,6 +853,54 @@ int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, int r
     (void)hipEventDestroy(e1);
     *ms = best;
     return rc;
+}
+
+// The RHS block of an autotuned plan for k > 1 right-hand sides, timed on
+// the caller's b and x (x is overwritten by the solve that follows) the first
+// time cap_plan_solve sees that k: thread-per-row plans time 1, 2 and 4
+// workgroups per CU at 256 and 128 threads (how many rows, each with k
+// lanes, poll at once); partitioned plans time 1, 2 and 4 right-hand sides
+// per wavefront of the block solve (the ones that divide k).  Every
+// candidate gives the same bits.  The plan's own settings are timed first and
+// another must be 2 % faster to win.  Waits for the device (events).
+int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
+{
+    std::vector<RhsPick> cands;
+    if (p->algo == CAP_ALGO_WRTFST) {
+        cands.push_back({p->bpc, p->wblock, 0});
+        for (int blk : {kBlock, kBlockSmall})
+            for (int bpc : {1, 2, 4})
+                if (!(bpc == p->bpc && blk == p->wblock)) cands.push_back({bpc, blk, 0});
+    } else if (p->algo == CAP_ALGO_PARTITION) {
+        const int dflt = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
+        cands.push_back({0, 0, dflt});
+        for (int kr : {1, 2, 4})
+            if (kr != dflt && k % kr == 0) cands.push_back({0, 0, kr});
+    } else {
+        p->rhs_pick[k] = RhsPick{};
+        return CAP_OK;
+    }
+    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr;
+    float best = 1e30f;
+    RhsPick pick = cands[0];
+    int rc = CAP_OK;
+    for (size_t c = 0; c < cands.size() && rc == CAP_OK; ++c) {
+        if (cands[c].bpc) p->bpc = cands[c].bpc;
+        if (cands[c].block) p->wblock = cands[c].block;
+        p->part_kr = cands[c].kr;
+        float t = 0;
+        rc = time_candidate(p, p->algo, p->sync, b, x, k, st, best, &t);
+        if (rc == CAP_OK && t < (c ? 0.98f : 1.0f) * best) {
+            best = t;
+            pick = cands[c];
+        }
+        p->bpc = bpc0;
+        p->wblock = blk0;
+        p->part_kr = kr0;
+    }
+    if (rc != CAP_OK) return rc;
+    p->rhs_pick[k] = pick;
+    return CAP_OK;
 }
 
 void apply_choice(cap_plan p, const TuneChoice &c)
@@ -1363,6 +1450,9 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         p->lds = o.lds_window > 0;
         p->unfused = o.arith == CAP_ARITH_REFERENCE;
         p->count_polls = o.count_polls != 0;
+        // the caller fixed neither the grid nor the workgroup size: multi-RHS
+        // solves get their RHS block timed per right-hand-side count
+        p->tune_rhs = o.algo == CAP_ALGO_AUTO && o.blocks_per_cu == 0 && o.row_threads == 0;
         p->staged = o.readiness != CAP_READY_IN_PLACE;
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
@@ -1500,6 +1590,8 @@ extern "C" {
 int cap_plan_solve(cap_plan p, const void *b, void *x, int rhs, void *stream)
 {
     if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
+    if (p->tune_rhs && rhs > 1 && rhs <= kWave && !p->reversed && !p->rhs_pick.count(rhs))
+        CAP_TRY(tune_rhs_block(p, b, x, rhs, S(stream)));
     if (!p->reversed) return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
     CAP_TRY(ensure_reversal(p, rhs));
     CAP_TRY(reverse_rows(p, b, p->rev_b, rhs, S(stream)));
@@ -1566,6 +1658,17 @@ int cap_plan_poll_count(cap_plan p, unsigned long long *polls, void *stream)
     unsigned long long sum = 0;
     for (unsigned long long v : h) sum += v;
     *polls = sum;
+    return CAP_OK;
+}
+
+int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, int *rhs_per_wave)
+{
+    if (!p || k < 1) return CAP_ERR_ARG;
+    const auto it = p->rhs_pick.find(k);
+    if (it == p->rhs_pick.end()) return CAP_ERR_UNSUPPORTED;
+    if (blocks_per_cu) *blocks_per_cu = it->second.bpc ? it->second.bpc : p->bpc;
+    if (row_threads) *row_threads = it->second.block ? it->second.block : p->wblock;
+    if (rhs_per_wave) *rhs_per_wave = it->second.kr;
     return CAP_OK;
 }
 

@@ -97,6 +97,16 @@ class Plan:
         N.check(N.lib().cap_plan_poll_count(self._h, byref(v), _stream(stream)), "cap_plan_poll_count")
         return v.value
 
+    def rhs_block(self, k: int):
+        """The RHS block the autotuner picked for k right-hand sides (after a
+        solve with k): dict(blocks_per_cu, row_threads, rhs_per_wave), or None."""
+        bpc, thr, kr = c_int(), c_int(), c_int()
+        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr))
+        if rc == N.CAP_ERR_UNSUPPORTED:
+            return None
+        N.check(rc, "cap_plan_rhs_block")
+        return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value)
+
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""
         return N.lib().cap_plan_status(self._h, _stream(stream))

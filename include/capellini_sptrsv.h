@@ -178,6 +178,17 @@ int cap_plan_status(cap_plan plan, void *stream);
  * with cap_opts.count_polls = 1; CAP_ERR_UNSUPPORTED otherwise) and clears
  * it.  Divided by rows x right-hand sides it is the spin count per row. */
 int cap_plan_poll_count(cap_plan plan, unsigned long long *polls, void *stream);
+/* The RHS block an autotuned plan (algo AUTO, blocks_per_cu and row_threads
+ * left 0) picked for k > 1 right-hand sides.  The first cap_plan_solve with
+ * a new k times the candidates on its own b and x and waits for the device:
+ * thread-per-row plans 1/2/4 workgroups per CU at 128/256 threads,
+ * partitioned plans 1/2/4 right-hand sides per wavefront of the block solve
+ * (*rhs_per_wave; 0 for the others).  Every candidate gives the same bits.
+ * CAP_ERR_UNSUPPORTED until a solve with that k has run.  (The reference's
+ * OPT_WARP_RHS / OPT_WARP_AUTO switches, openCL/WrtFst/common.h:28-30, set
+ * this by hand at compile time.) */
+int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
+                       int *row_threads, int *rhs_per_wave);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table. */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);

@@ -352,3 +352,34 @@ def test_poll_counter_build(cuda, lds):
     with pytest.raises(CapError):
         plain.poll_count()
     plain.close()
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 0.05), (2, 0.02)])
+def test_rhs_block_tuned_per_count(cuda, cfg, scale):
+    """An autotuned plan times its RHS block on the first solve with each new
+    right-hand-side count (thread-per-row: workgroups per CU and threads per
+    workgroup; partitioned: right-hand sides per wavefront); every solve,
+    tuning or not, gives the oracle's results."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = api.make_config(cfg, seed=4, scale=scale)
+    n = len(rp) - 1
+    plan = Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, v)))
+    algo = plan.info()["algo"]
+    for k in (8, 16, 8):
+        assert plan.rhs_block(k) is None or k == 8
+        b = api.uniform(80 + k, n * k)
+        x = plan.solve(torch.from_numpy(b.reshape(n, k)).to(cuda)).cpu().numpy().reshape(-1)
+        assert plan.status() == 0
+        ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)
+        if algo == "partition":
+            assert rel_maxnorm(x, ref) <= FP64_TOL
+        else:
+            assert np.array_equal(x, ref), (algo, k)
+        pick = plan.rhs_block(k)
+        assert pick is not None
+        if algo == "partition":
+            assert pick["rhs_per_wave"] in (1, 2, 4) and k % pick["rhs_per_wave"] == 0
+        elif algo == "wrtfst":
+            assert pick["blocks_per_cu"] in (1, 2, 4) and pick["row_threads"] in (128, 256)
+    plan.close()
