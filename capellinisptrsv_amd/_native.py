@@ -120,6 +120,13 @@ SIGNATURES = {
     "cap_generate_d": (c_int, [POINTER(cap_gen_params), PI, PI, PI, PD]),
     "cap_generate_s": (c_int, [POINTER(cap_gen_params), PI, PI, PI, PF]),
     "cap_uniform_d": (None, [c_ulonglong, c_longlong, c_double, c_double, PD]),
+    "cap_serial_analyser": (c_int, [PI, c_int, c_int, c_int, PI]),
+    "cap_serial_executor_d": (c_int, [PI, PI, PD, PI, c_int, c_int, c_int, c_int, PD, PD]),
+    "cap_serial_executor_s": (c_int, [PI, PI, PF, PI, c_int, c_int, c_int, c_int, PF, PF]),
+    "sptrsv_syncfree_serialref_d": (c_int, [PI, PI, PD, c_int, c_int, c_int, c_int, c_int, PD, PD, PD]),
+    "sptrsv_syncfree_serialref_s": (c_int, [PI, PI, PF, c_int, c_int, c_int, c_int, c_int, PF, PF, PF]),
+    "cap_serial_csr_d": (c_int, [c_int, c_int, PI, PI, PD, c_int, PD, PD, c_int]),
+    "cap_serial_csr_s": (c_int, [c_int, c_int, PI, PI, PF, c_int, PF, PF, c_int]),
     "cap_datasize": (c_double, [c_int, c_int, c_int, c_int, c_int]),
     "cap_device_count": (c_int, []),
     "cap_strerror": (c_char_p, [c_int]),

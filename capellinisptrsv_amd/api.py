@@ -153,6 +153,65 @@ def validate_csr(n: int, nnz: int, row_ptr, col_idx, val):
     return code, bad.value
 
 
+# ------------------------------------------------------------- serial CPU solves
+FORWARD, BACKWARD = 0, 1  # SUBSTITUTION_FORWARD / _BACKWARD (openCL/SynFree_csc/common.h:25-26)
+
+
+def serial_analyser(csc_row_idx, m: int, nnz: int):
+    """In-degree of every row of a CSC matrix (sptrsv_syncfree_serialref.h:6-23)."""
+    deg = np.zeros(max(m, 1), np.int32)
+    N.check(N.lib().cap_serial_analyser(_pi(_i32(csc_row_idx)), m, m, nnz, _pi(deg)), "cap_serial_analyser")
+    return deg[:m]
+
+
+def serial_executor(csc_col_ptr, csc_row_idx, csc_val, in_degree, m: int, n: int, substitution: int, rhs: int, b):
+    """Column-driven serial substitution on the CPU (serialref.h:25-108):
+    returns x (n*rhs values, interleaved); raises CapError on a malformed matrix."""
+    cv = np.ascontiguousarray(csc_val)
+    sfx = _suffix(cv.dtype)
+    b = np.ascontiguousarray(b, dtype=cv.dtype).reshape(-1)
+    if b.size != n * rhs:
+        raise ValueError("b must hold n*rhs values")
+    x = np.zeros(max(n * rhs, 1), cv.dtype)
+    N.check(getattr(N.lib(), f"cap_serial_executor_{sfx}")(_pi(_i32(csc_col_ptr)), _pi(_i32(csc_row_idx)), _pv(cv),
+                                                           _pi(_i32(in_degree)), m, n, substitution, rhs, _pv(b),
+                                                           _pv(x)), "cap_serial_executor")
+    return x[:n * rhs]
+
+
+def sptrsv_syncfree_serialref(csc_col_ptr, csc_row_idx, csc_val, m: int, n: int, nnz: int, substitution: int,
+                              rhs: int, b, x_ref):
+    """The reference's timed serial wrapper (serialref.h:110-177): returns (ret, x)."""
+    cv = np.ascontiguousarray(csc_val)
+    sfx = _suffix(cv.dtype)
+    b = np.ascontiguousarray(b, dtype=cv.dtype).reshape(-1)
+    xr = np.ascontiguousarray(x_ref, dtype=cv.dtype).reshape(-1)
+    if m == n and (b.size != n * rhs or xr.size != n * rhs):  # a non-square call returns -1 untouched
+        raise ValueError("b and x_ref must hold n*rhs values")
+    x = np.zeros(max(n * rhs, 1), cv.dtype)
+    ret = getattr(N.lib(), f"sptrsv_syncfree_serialref_{sfx}")(_pi(_i32(csc_col_ptr)), _pi(_i32(csc_row_idx)),
+                                                                _pv(cv), m, n, nnz, substitution, rhs, _pv(x),
+                                                                _pv(b), _pv(xr))
+    return ret, x[:n * rhs]
+
+
+def serial_csr_solve(row_ptr, col_idx, val, b, rhs: int = 1, arith: str = "reference"):
+    """Row-ordered serial forward substitution on the CPU over the solve
+    contract's CSR, terms in stored order; arith "reference" (rounded product,
+    rounded sum) or "fused" (the GPU kernels' default)."""
+    val = np.ascontiguousarray(val)
+    sfx = _suffix(val.dtype)
+    n = len(row_ptr) - 1
+    b = np.ascontiguousarray(b, dtype=val.dtype).reshape(-1)
+    if b.size != n * rhs:
+        raise ValueError("b must hold n*rhs values")
+    x = np.zeros(max(n * rhs, 1), val.dtype)
+    N.check(getattr(N.lib(), f"cap_serial_csr_{sfx}")(n, len(col_idx), _pi(_i32(row_ptr)), _pi(_i32(col_idx)),
+                                                      _pv(val), rhs, _pv(b), _pv(x), N.ARITH[arith]),
+            "cap_serial_csr")
+    return x[:n * rhs]
+
+
 def datasize(m: int, n: int, nnzL: int, value_bytes: int, len_warp_num: int = 0) -> float:
     return N.lib().cap_datasize(m, n, nnzL, value_bytes, len_warp_num)
 

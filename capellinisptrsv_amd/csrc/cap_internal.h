@@ -11,3 +11,10 @@
             return CAP_ERR_HIP;                                                                   \
         }                                                                                         \
     } while (0)
+
+// returns the CAP_* code of expr when it is not CAP_OK
+#define CAP_TRY(expr)                          \
+    do {                                       \
+        int cap_rc_ = (expr);                  \
+        if (cap_rc_ != CAP_OK) return cap_rc_; \
+    } while (0)

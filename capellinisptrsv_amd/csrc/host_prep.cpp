@@ -1,11 +1,14 @@
-// Host-side preprocessing for the SpTRSV library: Matrix Market input, lower
-// triangle extraction, right-hand side, level analysis, the mix chunk table,
-// CSR->CSC, layout validation and the seeded benchmark generators.
+// Host-side code of the SpTRSV library: Matrix Market input, lower triangle
+// extraction, right-hand side, level analysis, the mix chunk table, CSR->CSC,
+// layout validation, the seeded benchmark generators, and the serial CPU
+// solves (the reference's CPU path, exported as helpers of their own).
 //
-// These run once per matrix on the CPU (as in the reference); everything per
-// solve runs on the GPU.
+// The preprocessing runs once per matrix on the CPU (as in the reference).
+// Every solve entry of the GPU library runs on the GPU: none of them calls
+// the serial solves.
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -396,6 +399,171 @@ int generate_impl(const cap_gen_params *g, int *nnz_out, int *rp, int *ci, T *v)
     return CAP_OK;
 }
 
+// ------------------------------------------------------------ serial solves
+// Structure the column-driven executor relies on
+// (openCL/SynFree_csc/sptrsv_syncfree_serialref.h:25-108): col_ptr runs from
+// 0 to nnz without decreasing; FORWARD takes each column's first entry as its
+// diagonal and needs every other entry below it, BACKWARD takes the last entry
+// and needs every other entry above it.  The reference checks none of this: a
+// column whose producers come later makes its wait loop (:51-55, :80-84) spin
+// forever, and a row index out of range writes outside left_sum.
+int check_csc_factor(int m, int n, int nnz, const int *cp, const int *ri, int substitution)
+{
+    if (m < 0 || n < 0 || nnz < 0 || !cp || (nnz > 0 && !ri)) return CAP_ERR_ARG;
+    if (substitution != CAP_SUBSTITUTION_FORWARD && substitution != CAP_SUBSTITUTION_BACKWARD) return CAP_ERR_ARG;
+    if (cp[0] != 0 || cp[n] != nnz) return CAP_ERR_ROWPTR;
+    const bool fwd = substitution == CAP_SUBSTITUTION_FORWARD;
+    for (int c = 0; c < n; ++c) {
+        const int lo = cp[c], hi = cp[c + 1];
+        if (hi < lo) return CAP_ERR_ROWPTR;
+        if (hi == lo) return CAP_ERR_NO_DIAG;
+        const int dg = fwd ? lo : hi - 1;
+        if (ri[dg] != c) return CAP_ERR_NO_DIAG;
+        for (int j = lo; j < hi; ++j) {
+            if (j == dg) continue;
+            const int r = ri[j];
+            if (r < 0 || r >= m) return CAP_ERR_ARG;
+            if (fwd ? r <= c : r >= c) return CAP_ERR_UPPER;
+        }
+    }
+    return CAP_OK;
+}
+
+// Column-driven substitution: visit the columns in solve order; a column's
+// x is final once all of its row's contributions have arrived (counted
+// against the in-degree table), then its entries are scattered into the
+// left sums of later rows.  Arithmetic: left += x * v (a rounded product,
+// then a rounded sum: this file is compiled without FP contraction) and
+// x = (b - left) / diag, as serialref.h:57-60, :67-68.
+template <typename T>
+int serial_executor_impl(const int *cp, const int *ri, const T *v, const int *in_degree, int m, int n,
+                         int substitution, int rhs, const T *b, T *x)
+{
+    if (m != n) return CAP_ERR_NOT_SQUARE;
+    if (!cp || !v || !in_degree || !b || !x || rhs < 1 || n < 0) return CAP_ERR_ARG;
+    CAP_TRY(check_csc_factor(m, n, cp[n], cp, ri, substitution));
+    const size_t k = (size_t)rhs;
+    std::vector<T> left;
+    std::vector<int> arrived;
+    try {
+        left.assign((size_t)m * k, (T)0);
+        arrived.assign((size_t)m, 0);
+    } catch (const std::bad_alloc &) {
+        return CAP_ERR_ALLOC;
+    }
+    const bool fwd = substitution == CAP_SUBSTITUTION_FORWARD;
+    for (int t = 0; t < n; ++t) {
+        const int c = fwd ? t : n - 1 - t;
+        // a table that disagrees with the matrix would leave the reference waiting
+        if (arrived[c] != in_degree[c] - 1) return CAP_ERR_ARG;
+        const int lo = cp[c], hi = cp[c + 1];
+        const int dg = fwd ? lo : hi - 1;
+        const T d = v[dg];
+        T *xc = x + (size_t)c * k;
+        const T *bc = b + (size_t)c * k;
+        const T *lc = left.data() + (size_t)c * k;
+        for (size_t q = 0; q < k; ++q) xc[q] = (bc[q] - lc[q]) / d;
+        for (int j = lo; j < hi; ++j) {
+            if (j == dg) continue;
+            const int r = ri[j];
+            T *lr = left.data() + (size_t)r * k;
+            const T w = v[j];
+            for (size_t q = 0; q < k; ++q) {
+                const T p = xc[q] * w;
+                lr[q] = lr[q] + p;
+            }
+            ++arrived[r];
+        }
+    }
+    return CAP_OK;
+}
+
+// Row-ordered forward substitution over the solve contract's CSR (lower
+// triangle, diagonal last, entries in stored order): per right-hand side,
+// s accumulates the row's off-diagonal terms in stored order and
+// x = (b - s) / diag.  arith picks fma(v, x, s) or s + v*x per term.
+template <typename T>
+int serial_csr_impl(int n, int nnz, const int *rp, const int *ci, const T *v, int rhs, const T *b, T *x, int arith)
+{
+    if (!b || !x || rhs < 1 || (arith != CAP_ARITH_FUSED && arith != CAP_ARITH_REFERENCE)) return CAP_ERR_ARG;
+    CAP_TRY(validate_impl<T>(n, nnz, rp, ci, v, nullptr));
+    const size_t k = (size_t)rhs;
+    std::vector<T> s;
+    try {
+        s.assign(k, (T)0);
+    } catch (const std::bad_alloc &) {
+        return CAP_ERR_ALLOC;
+    }
+    const bool fused = arith == CAP_ARITH_FUSED;
+    for (int i = 0; i < n; ++i) {
+        std::fill(s.begin(), s.end(), (T)0);
+        const int dg = rp[i + 1] - 1;
+        for (int j = rp[i]; j < dg; ++j) {
+            const T w = v[j];
+            const T *xc = x + (size_t)ci[j] * k;
+            if (fused) {
+                for (size_t q = 0; q < k; ++q) s[q] = std::fma(w, xc[q], s[q]);
+            } else {
+                for (size_t q = 0; q < k; ++q) {
+                    const T p = w * xc[q];
+                    s[q] = s[q] + p;
+                }
+            }
+        }
+        const T d = v[dg];
+        for (size_t q = 0; q < k; ++q) x[(size_t)i * k + q] = (b[(size_t)i * k + q] - s[q]) / d;
+    }
+    return CAP_OK;
+}
+
+double ms_since(const std::chrono::steady_clock::time_point &t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The timed, validated wrapper (serialref.h:110-177): analysis, execution,
+// relative L1 error against x_ref with the reference's 1e-4 bar, and its
+// report lines on stdout.  Returns -1 for a non-square matrix, as the
+// reference does, and the executor's code for a malformed one.
+template <typename T>
+int serialref_impl(const int *cp, const int *ri, const T *v, int m, int n, int nnz, int substitution, int rhs, T *x,
+                   const T *b, const T *x_ref)
+{
+    if (m != n) {
+        std::printf("This is not a square matrix, return.\n");
+        return CAP_ERR_NOT_SQUARE;
+    }
+    if (!cp || !x_ref || rhs < 1 || n < 0 || nnz < 0 || (nnz > 0 && !ri)) return CAP_ERR_ARG;
+    if (cp[n] != nnz) return CAP_ERR_ROWPTR;
+    std::vector<int> indeg;
+    try {
+        indeg.assign((size_t)(m > 0 ? m : 1), 0);
+    } catch (const std::bad_alloc &) {
+        return CAP_ERR_ALLOC;
+    }
+    std::printf(" - SpTRSV Serial analyser start!\n");
+    auto t0 = std::chrono::steady_clock::now();
+    CAP_TRY(cap_serial_analyser(ri, m, n, nnz, indeg.data()));
+    std::printf("SpTRSV Serial analyser on L used %4.2f ms\n", ms_since(t0));
+    std::printf(" - SpTRSV Serial executor start!\n");
+    t0 = std::chrono::steady_clock::now();
+    CAP_TRY(serial_executor_impl<T>(cp, ri, v, indeg.data(), m, n, substitution, rhs, b, x));
+    const double ms = ms_since(t0);
+    std::printf("SpTRSV Serial executor used %4.2f ms, throughput is %4.2f gflops\n", ms,
+                2.0 * rhs * (double)nnz / (1e6 * ms));
+    double ref = 0, res = 0;
+    for (size_t i = 0; i < (size_t)n * rhs; ++i) {
+        ref += std::fabs((double)x_ref[i]);
+        res += std::fabs((double)x[i] - (double)x_ref[i]);
+    }
+    if (ref != 0) res /= ref;
+    if (res < 1e-4)
+        std::printf("SpTRSV Serial executor passed! |x-xref|/|xref| = %8.2e\n", res);
+    else
+        std::printf("SpTRSV Serial executor _NOT_ passed! |x-xref|/|xref| = %8.2e\n", res);
+    return CAP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -508,6 +676,49 @@ void cap_uniform_d(unsigned long long seed, long long count, double lo, double h
         out[i] = lo + (hi - lo) * unit53(splitmix64(&s));
     }
 }
+int cap_serial_analyser(const int *csc_row_idx, int m, int n, int nnz, int *in_degree)
+{
+    (void)n;
+    if (m < 0 || nnz < 0 || !in_degree || (nnz > 0 && !csc_row_idx)) return CAP_ERR_ARG;
+    std::fill(in_degree, in_degree + m, 0);
+    for (int j = 0; j < nnz; ++j) {
+        const int r = csc_row_idx[j];
+        if (r < 0 || r >= m) return CAP_ERR_ARG;
+        ++in_degree[r];
+    }
+    return CAP_OK;
+}
+int cap_serial_executor_d(const int *cp, const int *ri, const double *v, const int *in_degree, int m, int n,
+                          int substitution, int rhs, const double *b, double *x)
+{
+    return serial_executor_impl<double>(cp, ri, v, in_degree, m, n, substitution, rhs, b, x);
+}
+int cap_serial_executor_s(const int *cp, const int *ri, const float *v, const int *in_degree, int m, int n,
+                          int substitution, int rhs, const float *b, float *x)
+{
+    return serial_executor_impl<float>(cp, ri, v, in_degree, m, n, substitution, rhs, b, x);
+}
+int sptrsv_syncfree_serialref_d(const int *cp, const int *ri, const double *v, int m, int n, int nnz,
+                                int substitution, int rhs, double *x, const double *b, const double *x_ref)
+{
+    return serialref_impl<double>(cp, ri, v, m, n, nnz, substitution, rhs, x, b, x_ref);
+}
+int sptrsv_syncfree_serialref_s(const int *cp, const int *ri, const float *v, int m, int n, int nnz,
+                                int substitution, int rhs, float *x, const float *b, const float *x_ref)
+{
+    return serialref_impl<float>(cp, ri, v, m, n, nnz, substitution, rhs, x, b, x_ref);
+}
+int cap_serial_csr_d(int n, int nnz, const int *rp, const int *ci, const double *v, int rhs, const double *b,
+                     double *x, int arith)
+{
+    return serial_csr_impl<double>(n, nnz, rp, ci, v, rhs, b, x, arith);
+}
+int cap_serial_csr_s(int n, int nnz, const int *rp, const int *ci, const float *v, int rhs, const float *b,
+                     float *x, int arith)
+{
+    return serial_csr_impl<float>(n, nnz, rp, ci, v, rhs, b, x, arith);
+}
+
 double cap_datasize(int m, int n, int nnzL, int vb, int len)
 {
     return (double)(n + 1) * 4 + (double)(nnzL + m) * 4 + (double)nnzL * vb + 2.0 * n * vb + (double)len * 4;

@@ -200,12 +200,6 @@ int dev_alloc(P **p, size_t count)
     return CAP_OK;
 }
 
-#define CAP_TRY(expr)                       \
-    do {                                    \
-        int cap_rc_ = (expr);               \
-        if (cap_rc_ != CAP_OK) return cap_rc_; \
-    } while (0)
-
 // ------------------------------------------------- uncached readiness pool
 // Blocks of uncached device memory for staged readiness arrays.  Uncached
 // memory that a kernel has read is never freed before the process ends

@@ -6,15 +6,17 @@ program on every file of a directory, each run appending one row.  The
 matrix list of the paper's collection is matrix/matrix_name.txt (one path
 per line).  This driver does the same with the gfx950 program:
 
-    python -m capellinisptrsv_amd.sweep DIR_OR_LIST [--root DIR] [-a wrtfst,mix,2part,csc,auto,local,partition]
+    python -m capellinisptrsv_amd.sweep DIR_OR_LIST [--root DIR] [-a wrtfst,mix,2part,csc,auto,local,partition,serial]
            [--f32] [--values] [--out-dir DIR] [--keep] [--dry-run]
 
 DIR_OR_LIST is a directory (every *.mtx in it, sorted) or a text file of
 paths; a listed path that does not exist is looked up by file name under
 --root.  Every (matrix, algorithm) pair runs `sptrsv_main` (or
 `sptrsv_main_f32`) in its own child process started in --out-dir (default:
-the working directory), which appends one row to result.csv there; that
-file is removed first unless --keep.  Exit status:
+the working directory), which appends one row to result.csv there and one
+JSON object to result.jsonl (the FP64 serial-solve error and the roofline
+figures of the run); both files are removed first unless --keep.  "serial"
+times the reference's serial CPU executor into the same columns.  Exit status:
 0 if every run succeeded, 1 otherwise (failures are listed).
 """
 from __future__ import annotations
@@ -25,7 +27,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ALGOS = ("wrtfst", "2part", "mix", "csc", "auto", "local", "partition")
+ALGOS = ("wrtfst", "2part", "mix", "csc", "auto", "local", "partition", "serial")
 
 
 def matrices(source: str, root: str | None = None) -> list[str]:
@@ -65,7 +67,7 @@ def main(argv=None) -> int:
     ap.add_argument("-a", "--algos", default="wrtfst", help=f"comma-separated subset of {','.join(ALGOS)}")
     ap.add_argument("--f32", action="store_true", help="VALUE_TYPE float build of the program")
     ap.add_argument("--values", action="store_true", help="keep the matrix values (default: unit values)")
-    ap.add_argument("--out-dir", default=".", help="where result.csv is written")
+    ap.add_argument("--out-dir", default=".", help="where result.csv and result.jsonl are written")
     ap.add_argument("--keep", action="store_true", help="append to an existing CSV")
     ap.add_argument("--dry-run", action="store_true", help="print the commands only")
     ap.add_argument("--timeout", type=float, default=600.0, help="seconds per run")
@@ -81,9 +83,10 @@ def main(argv=None) -> int:
             print(" ".join(c))
         return 0
     os.makedirs(a.out_dir, exist_ok=True)
-    csv = os.path.join(a.out_dir, "result.csv")
-    if not a.keep and os.path.exists(csv):
-        os.remove(csv)
+    for name in ("result.csv", "result.jsonl"):
+        out = os.path.join(a.out_dir, name)
+        if not a.keep and os.path.exists(out):
+            os.remove(out)
     failed = []
     for c in cmds:
         print(" ".join(c[1:]), flush=True)

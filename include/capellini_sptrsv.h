@@ -423,6 +423,58 @@ int cap_generate_s(const cap_gen_params *g, int *nnz, int *rp, int *ci, float *v
 /* fills out[0..count) with U[lo,hi) doubles from splitmix64(seed) */
 void cap_uniform_d(unsigned long long seed, long long count, double lo, double hi, double *out);
 
+/* ------------------------------------------------------- serial solves
+ * The reference's CPU path, for callers that want the CPU result beside the
+ * GPU one (no GPU entry of this library calls these).  Multi-RHS values are
+ * interleaved, x[i*rhs + k] (serialref.h:57-60).  Where the reference would
+ * spin forever or index out of range (a column without its diagonal in the
+ * expected place, an entry on the wrong side of it, a row index outside
+ * [0, m), an in-degree table that does not match the matrix) these return a
+ * CAP_ERR_* code instead. */
+#define CAP_SUBSTITUTION_FORWARD 0  /* SUBSTITUTION_FORWARD, openCL/SynFree_csc/common.h:25 */
+#define CAP_SUBSTITUTION_BACKWARD 1 /* SUBSTITUTION_BACKWARD, common.h:26 */
+
+/* replaces sptrsv_syncfree_analyser, openCL/SynFree_csc/sptrsv_syncfree_serialref.h:6-23:
+ * in_degree[r] = entries of row r in the CSC matrix (m ints). */
+int cap_serial_analyser(const int *csc_row_idx, int m, int n, int nnz, int *in_degree);
+
+/* replaces sptrsv_syncfree_executor, serialref.h:25-108: column-driven
+ * substitution.  FORWARD visits columns 0..n-1 with each column's diagonal
+ * first; BACKWARD visits n-1..0 with the diagonal last.  x = (b - left) / diag
+ * and left += x * v with a rounded product and a rounded sum: the reference's
+ * own bits. */
+int cap_serial_executor_d(const int *cscColPtr, const int *cscRowIdx, const double *cscVal,
+                          const int *in_degree, int m, int n, int substitution, int rhs,
+                          const double *b, double *x);
+int cap_serial_executor_s(const int *cscColPtr, const int *cscRowIdx, const float *cscVal,
+                          const int *in_degree, int m, int n, int substitution, int rhs,
+                          const float *b, float *x);
+
+/* replaces sptrsv_syncfree_serialref, serialref.h:110-177: analyser +
+ * executor, timed, with the relative L1 check against x_ref (< 1e-4) and the
+ * reference's report lines on stdout.  Returns 0 (also when the check fails,
+ * as the reference does), -1 for a non-square matrix, or the executor's code
+ * for a malformed one. */
+int sptrsv_syncfree_serialref_d(const int *cscColPtrTR, const int *cscRowIdxTR,
+                                const double *cscValTR, int m, int n, int nnzTR,
+                                int substitution, int rhs, double *x, const double *b,
+                                const double *x_ref);
+int sptrsv_syncfree_serialref_s(const int *cscColPtrTR, const int *cscRowIdxTR,
+                                const float *cscValTR, int m, int n, int nnzTR,
+                                int substitution, int rhs, float *x, const float *b,
+                                const float *x_ref);
+
+/* No counterpart in the reference (whose serial solve is column-driven only):
+ * row-ordered forward substitution over the solve contract's CSR (host
+ * arrays, checked as cap_validate_csr), off-diagonal terms in stored order,
+ * arith = CAP_ARITH_FUSED or CAP_ARITH_REFERENCE.  It returns the bits of the
+ * GPU thread and wavefront kernels with the same arith; with sorted rows and
+ * CAP_ARITH_REFERENCE, the executor's bits too. */
+int cap_serial_csr_d(int n, int nnz, const int *row_ptr, const int *col_idx, const double *val,
+                     int rhs, const double *b, double *x, int arith);
+int cap_serial_csr_s(int n, int nnz, const int *row_ptr, const int *col_idx, const float *val,
+                     int rhs, const float *b, float *x, int arith);
+
 /* reference metric formulas, spts_syncfree_cuda_csr_wrt.h:206-215 */
 double cap_datasize(int m, int n, int nnzL, int value_bytes, int len_warp_num);
 
@@ -480,6 +532,21 @@ static inline int sptrsv_syncfree_hip_csc(const int *cp, const int *ri, const VA
                : sptrsv_syncfree_hip_csc_s(cp, ri, (const float *)(const void *)v, m, n, nnz,
                                            (float *)(void *)x, (const float *)(const void *)b,
                                            (const float *)(const void *)xr, pre, t, gf, bw);
+}
+static inline int sptrsv_syncfree_serialref(const int *cp, const int *ri, const VALUE_TYPE *v,
+                                            int m, int n, int nnz, int substitution, int rhs,
+                                            VALUE_TYPE *x, const VALUE_TYPE *b,
+                                            const VALUE_TYPE *xr)
+{
+    return sizeof(VALUE_TYPE) == 8
+               ? sptrsv_syncfree_serialref_d(cp, ri, (const double *)(const void *)v, m, n, nnz,
+                                             substitution, rhs, (double *)(void *)x,
+                                             (const double *)(const void *)b,
+                                             (const double *)(const void *)xr)
+               : sptrsv_syncfree_serialref_s(cp, ri, (const float *)(const void *)v, m, n, nnz,
+                                             substitution, rhs, (float *)(void *)x,
+                                             (const float *)(const void *)b,
+                                             (const float *)(const void *)xr);
 }
 
 #ifdef __cplusplus

@@ -57,6 +57,22 @@ void pipeline<double>(int m, int n, int nnz, const int *rp, const int *ci, const
         cap_transpose_d(mm, mm, nnzL, rpL, ciL, vL, cr.data(), cp.data(), cv.data());
         int bad = -1;
         EXPECT(cap_validate_csr_d(mm, nnzL, rpL, ciL, vL, &bad) == CAP_OK);
+        // serial solves: forward on the CSC, backward on the CSR read as CSC, 1 and 3 RHS
+        std::vector<int> deg((size_t)mm + 1);
+        EXPECT(cap_serial_analyser(cr.data(), mm, mm, nnzL, deg.data()) == CAP_OK);
+        for (int rhs : {1, 3}) {
+            std::vector<double> bk((size_t)mm * rhs + 1, 1.0), xk((size_t)mm * rhs + 1), xs((size_t)mm * rhs + 1);
+            EXPECT(cap_serial_executor_d(cp.data(), cr.data(), cv.data(), deg.data(), mm, mm,
+                                         CAP_SUBSTITUTION_FORWARD, rhs, bk.data(), xk.data()) == CAP_OK);
+            EXPECT(cap_serial_csr_d(mm, nnzL, rpL, ciL, vL, rhs, bk.data(), xs.data(), CAP_ARITH_REFERENCE) == CAP_OK);
+            EXPECT(cap_serial_csr_d(mm, nnzL, rpL, ciL, vL, rhs, bk.data(), xs.data(), CAP_ARITH_FUSED) == CAP_OK);
+            std::vector<int> dr((size_t)mm + 1);
+            EXPECT(cap_serial_analyser(ciL, mm, mm, nnzL, dr.data()) == CAP_OK);
+            EXPECT(cap_serial_executor_d(rpL, ciL, vL, dr.data(), mm, mm, CAP_SUBSTITUTION_BACKWARD, rhs, bk.data(),
+                                         xk.data()) == CAP_OK);
+        }
+        EXPECT(sptrsv_syncfree_serialref_d(cp.data(), cr.data(), cv.data(), mm, mm, nnzL, CAP_SUBSTITUTION_FORWARD,
+                                           1, x.data(), b.data(), x.data()) == CAP_OK);
         cap_free(rpL);
         cap_free(ciL);
         cap_free(vL);
@@ -151,6 +167,24 @@ int main(int argc, char **argv)
         const double v2[] = {1, 1, 1, 1, 1};
         EXPECT(cap_validate_csr_d(3, 5, rp2, ci2, v2, &badrow) != CAP_OK);
         EXPECT(cap_validate_csr_d(0, 0, rp, ci, v, &badrow) == CAP_OK);
+        // serial executor on CSC factors that would make the reference spin or
+        // write out of range: a row index past m, a column without its
+        // diagonal first, an entry above the diagonal, a wrong in-degree table
+        const int cp3[] = {0, 2, 3, 4};
+        const int ri_far[] = {0, 9, 1, 2}, ri_nodiag[] = {1, 0, 1, 2}, ri_above[] = {0, 1, 0, 2};
+        const double v3[] = {2, 1, 2, 2};
+        const double b3[] = {1, 1, 1};
+        double x3[3];
+        const int deg3[] = {1, 2, 1};
+        EXPECT(cap_serial_executor_d(cp3, ri_far, v3, deg3, 3, 3, CAP_SUBSTITUTION_FORWARD, 1, b3, x3) != CAP_OK);
+        EXPECT(cap_serial_executor_d(cp3, ri_nodiag, v3, deg3, 3, 3, CAP_SUBSTITUTION_FORWARD, 1, b3, x3) != CAP_OK);
+        EXPECT(cap_serial_executor_d(cp3, ri_above, v3, deg3, 3, 3, CAP_SUBSTITUTION_FORWARD, 1, b3, x3) != CAP_OK);
+        const int ri_ok[] = {0, 1, 1, 2}, deg_bad[] = {1, 1, 1};
+        EXPECT(cap_serial_executor_d(cp3, ri_ok, v3, deg_bad, 3, 3, CAP_SUBSTITUTION_FORWARD, 1, b3, x3) != CAP_OK);
+        EXPECT(cap_serial_executor_d(cp3, ri_ok, v3, deg3, 3, 3, CAP_SUBSTITUTION_FORWARD, 1, b3, x3) == CAP_OK);
+        int deg_out[3];
+        EXPECT(cap_serial_analyser(ri_far, 3, 3, 4, deg_out) != CAP_OK);
+        EXPECT(cap_serial_csr_d(3, 3, rp2, ci2, v2, 1, b3, x3, CAP_ARITH_FUSED) != CAP_OK);
     }
     // generators: every kind, small sizes, both passes, both precisions
     for (int kind = 0; kind < 4; ++kind) {

@@ -84,6 +84,23 @@ def test_driver_appends_result_csv(cuda, tmp_path):
         assert f[0] == str(mtx) and int(f[1]) == m and int(f[2]) == nnzL and int(f[4]) == layer
         assert f[3] == f"{nnzL / m:4.2f}" and f[5] == f"{par:4.2f}"
         assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
+    # result.jsonl: one object per run; x against the FP64 serial solve of the
+    # same L and b (FP64 kernels: order kept, fused terms, so within 1e-12;
+    # the FP32 run within the FP32 bar), roofline figures from the wrapper
+    import json
+    side = [json.loads(line) for line in (tmp_path / "result.jsonl").read_text().splitlines()]
+    assert [s["algo"] for s in side] == ["wrtfst", "mix", "2part", "csc"]
+    for s, row in zip(side, rows[1::2]):
+        f = [c.strip() for c in row.split(",")]
+        assert s["device"] == "gpu" and s["passed"] and (s["m"], s["nnzL"], s["layer"]) == (m, nnzL, layer)
+        assert f"{s['solve_time_ms']:4.2f}" == f[7] and f"{s['bandwidth_gbs']:4.2f}" == f[9]
+        assert s["x_from"] == ("timed solve" if s["algo"] == "csc" else "separate solve, same options")
+        bar = 1e-5 if s["value_type"] == "float" else 1e-12
+        assert s["rel_err_vs_fp64_serial"] <= bar, s
+        rf = s["roofline"]
+        assert rf["peak_gbs"] == 8000 and abs(rf["frac"] - s["bandwidth_gbs"] / 8000) <= 1e-12
+        assert abs(rf["bytes"] - s["bandwidth_gbs"] * 1e6 * s["solve_time_ms"]) <= 1e-6 * rf["bytes"]
+        assert abs(s["us_per_level"] - 1e3 * s["solve_time_ms"] / layer) <= 1e-9 * s["us_per_level"]
 
 
 def test_driver_autotuned_and_deep_matrix_solvers(cuda, tmp_path):
