@@ -37,6 +37,11 @@ namespace cap {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;       // 4 wavefronts per workgroup
 constexpr int kBlockSmall = 128;  // the thread-per-row kernels' other workgroup size
+// Thread-per-row workgroups of several right-hand sides (value protocol, no
+// LDS window): a ticket covers BLOCK / lanes-per-row rows, and each ticket
+// is one atomic on the one ticket counter, which serializes at about 12 ns
+// an atomic; 16 right-hand sides give 16 rows a ticket at 256 threads.
+constexpr int kBlockLarge = 1024;
 constexpr int kWin = 4;      // dependencies polled concurrently per lane (flag protocol)
 
 
@@ -614,6 +619,140 @@ __global__ void __launch_bounds__(BLOCK)
         if ((threadIdx.x & (kWave - 1)) == 0)
             atomic_add_agent(sc.polls + (blockIdx.x * (BLOCK / kWave) + threadIdx.x / kWave) % kPollCounters, t);
     }
+}
+
+// Value protocol row solver for R right-hand sides per lane: the lane owns
+// right-hand sides q, q + L, ..., q + (R-1)L of row i (those below k), L
+// lanes per row, so each poll instruction of the row's L lanes reads L
+// contiguous values.  A dependency is consumed once all of the lane's R
+// values of it are published; each right-hand side keeps its own sum in
+// stored order, so the results are those of R single-RHS lanes.  The window
+// (two dependencies, sliding) and the wave-uniform exit are those of
+// wrtfst_row_value.
+template <typename T, int R>
+__device__ __forceinline__ void wrtfst_row_value_r(const int *__restrict__ ci, const T *__restrict__ val,
+                                                   const T *__restrict__ b, T *__restrict__ x, const SyncCtx &sc,
+                                                   int i, int j, int d, int k, int q, int L)
+{
+    typedef typename VBits<T>::U U;
+    constexpr U kS = VBits<T>::kSentinel;
+    U *xr = ready_array(sc, x);
+    int nr = 0;  // right-hand sides of this lane
+#pragma unroll
+    for (int t = 0; t < R; ++t) nr += q + t * L < k;
+    const long long o = (long long)i * k + q;
+    T bo[R], s[R];
+    const T vd = val[d];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        bo[t] = t < nr ? b[o + t * L] : (T)0;  // before the polling loop, as in wrtfst_row_value
+        s[t] = (T)0;
+    }
+    int c0 = 0, c1 = 0;
+    T v0 = (T)0, v1 = (T)0;
+    U x0[R], x1[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) x0[t] = x1[t] = kS;
+    if (j < d) {
+        c0 = ci[j];
+        v0 = val[j];
+    }
+    if (j + 1 < d) {
+        c1 = ci[j + 1];
+        v1 = val[j + 1];
+    }
+    Idle idle;
+    bool done = false;
+    do {  // wave-uniform exit: see wrtfst_row
+        if (!done) {
+            bool prog = false, r0 = j < d, r1 = j + 1 < d;
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                if (t < nr && j < d && x0[t] == kS) x0[t] = ld_agent(xr + (long long)c0 * k + q + t * L);
+                if (t < nr && j + 1 < d && x1[t] == kS) x1[t] = ld_agent(xr + (long long)c1 * k + q + t * L);
+                r0 = r0 && (t >= nr || x0[t] != kS);
+                r1 = r1 && (t >= nr || x1[t] != kS);
+            }
+            if (r0) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) s[t] = acc(sc, v0, as_val(x0[t]), s[t]);
+                ++j;
+                prog = true;
+                if (r1) {  // both consumed: a fresh window
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        s[t] = acc(sc, v1, as_val(x1[t]), s[t]);
+                        x0[t] = kS;
+                    }
+                    ++j;
+                    if (j < d) {
+                        c0 = ci[j];
+                        v0 = val[j];
+                    }
+                } else {  // slide by one
+                    c0 = c1;
+                    v0 = v1;
+#pragma unroll
+                    for (int t = 0; t < R; ++t) x0[t] = x1[t];
+                }
+#pragma unroll
+                for (int t = 0; t < R; ++t) x1[t] = kS;
+                if (j + 1 < d) {
+                    c1 = ci[j + 1];
+                    v1 = val[j + 1];
+                }
+            }
+            if (prog) {
+                idle.progress();
+            } else if (j < d && give_up(sc, idle)) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) s[t] = as_val(VBits<T>::kQNaN);  // poison; still published below
+                j = d;
+            }
+            if (j >= d) {  // write first: publish inside the polling loop
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    if (t < nr) {
+                        const T xi = (bo[t] - s[t]) / vd;
+                        U u = as_bits(xi);
+                        if (u == kS) u = VBits<T>::kQNaN;
+                        st_agent(xr + o + t * L, u);
+                        if (sc.ready) x[o + t * L] = xi;
+                    }
+                }
+                asm volatile("" ::: "memory");
+                done = true;
+            }
+        }
+    } while (__any(!done));
+}
+
+// Thread-per-row solve with R right-hand sides per lane (value protocol, no
+// LDS window): L = 1 << lpr_log2 lanes per row, BLOCK >> lpr_log2 rows per
+// ticket.  With k right-hand sides, R per lane makes a ticket R times as
+// many rows as k_wrtfst's, every row still polled by its own lanes at once
+// (a ticket's rows are all in flight together).  Tickets, batch schedule
+// and forward progress as in k_wrtfst.
+template <typename T, int BLOCK, int R>
+__global__ void __launch_bounds__(BLOCK)
+    k_wrtfst_r(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
+               const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
+               unsigned *__restrict__ ticket, const int2 *__restrict__ sched, int n_sched)
+{
+    const int L = 1 << lpr_log2;
+    const int q = threadIdx.x & (L - 1);
+    const int r = threadIdx.x >> lpr_log2;
+    const int rows = BLOCK >> lpr_log2;
+    ticket_loop(ticket, sched ? 1 : rows, sched ? n_sched : n, [&](int base) {
+        int lo = base, len = rows;
+        if (sched) {
+            const int2 blk = sched[base];
+            lo = blk.x;
+            len = blk.y;
+        }
+        const int i = lo + r;
+        if (r < len && i < n && q < k) wrtfst_row_value_r<T, R>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, L);
+    });
 }
 
 // ------------------------------------------------------------ refinement

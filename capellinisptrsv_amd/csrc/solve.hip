@@ -79,10 +79,11 @@ struct PartState {
 };
 
 // The RHS-block pick of an autotuned plan for one right-hand-side count:
-// thread-per-row workgroups per CU and threads per workgroup, or the
-// partitioned block solve's right-hand sides per wavefront (0: the plan's).
+// thread-per-row workgroups per CU, threads per workgroup and right-hand
+// sides per lane, or the partitioned block solve's right-hand sides per
+// wavefront (0: the plan's).
 struct RhsPick {
-    int bpc = 0, block = 0, kr = 0;
+    int bpc = 0, block = 0, kr = 0, rpl = 0;
 };
 
 struct cap_plan_s {
@@ -114,6 +115,9 @@ struct cap_plan_s {
     bool tune_rhs = false;
     std::map<int, RhsPick> rhs_pick;
     int part_kr = 0;  // partition block solve: right-hand sides per wavefront (0: the fixed rule)
+    int rpl = 1;      // thread-per-row, several right-hand sides: right-hand sides per lane
+    int fixed_rpl = 0;  // cap_opts.rhs_per_lane: the multi-RHS block the caller fixed (0: none)
+    bool auto_algo = false;  // created with CAP_ALGO_AUTO
     // batch: block schedule (relative level order) for blocks of sched_rows rows
     int2 *sched = nullptr;
     int n_sched = 0, sched_rows = 0;
@@ -592,20 +596,29 @@ int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
 // of one launch (and puts the plan's own settings back).
 struct RhsScope {
     cap_plan p;
-    int bpc, block, kr;
-    RhsScope(cap_plan q, int k) : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr)
+    int bpc, block, kr, rpl;
+    RhsScope(cap_plan q, int k) : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl)
     {
         const auto it = k > 1 ? p->rhs_pick.find(k) : p->rhs_pick.end();
-        if (it == p->rhs_pick.end()) return;
+        if (it == p->rhs_pick.end()) {
+            if (k > 1 && p->fixed_rpl > 0) {  // cap_opts.rhs_per_lane
+                p->bpc = 1;
+                p->wblock = kBlockLarge;
+                p->rpl = p->fixed_rpl;
+            }
+            return;
+        }
         if (it->second.bpc) p->bpc = it->second.bpc;
         if (it->second.block) p->wblock = it->second.block;
         if (it->second.kr) p->part_kr = it->second.kr;
+        if (it->second.rpl) p->rpl = it->second.rpl;
     }
     ~RhsScope()
     {
         p->bpc = bpc;
         p->wblock = block;
         p->part_kr = kr;
+        p->rpl = rpl;
     }
 };
 
@@ -623,7 +636,30 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
     if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
     if (fused) sc.unfused = 0;
-    if (algo == CAP_ALGO_WRTFST) {
+    if (algo == CAP_ALGO_WRTFST && p->rpl > 1 && rhs > 1) {  // several right-hand sides per lane
+        if (rhs > kWave || sync != CAP_SYNC_VALUE || p->lds || p->count_polls) return CAP_ERR_UNSUPPORTED;
+        const int l = lpr_log2_for((rhs + p->rpl - 1) / p->rpl);
+        const int rows_per_block = p->wblock >> l;
+        const bool use_sched = p->sched && p->sched_rows == rows_per_block;
+        const int2 *sched = use_sched ? p->sched : nullptr;
+        const int n_sched = use_sched ? p->n_sched : 0;
+        const unsigned nb =
+            grid_for(p, use_sched ? n_sched : ((long long)n + rows_per_block - 1) / rows_per_block);
+        auto go = [&](auto kernel) {
+            kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, sched, n_sched);
+        };
+        // 8 FP64 right-hand sides per lane would spill at 1024 threads
+        const int key = p->wblock * 16 + p->rpl;
+        if (key == kBlockLarge * 16 + 2) go(k_wrtfst_r<T, kBlockLarge, 2>);
+        else if (key == kBlockLarge * 16 + 4) go(k_wrtfst_r<T, kBlockLarge, 4>);
+        else if (key == kBlock * 16 + 4) go(k_wrtfst_r<T, kBlock, 4>);
+        else if constexpr (sizeof(T) == 4) {
+            if (key == kBlockLarge * 16 + 8) go(k_wrtfst_r<T, kBlockLarge, 8>);
+            else return CAP_ERR_UNSUPPORTED;
+        } else {
+            return CAP_ERR_UNSUPPORTED;
+        }
+    } else if (algo == CAP_ALGO_WRTFST) {
         if (rhs < 1 || rhs > kWave) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for(rhs);
         const int rows_per_block = p->wblock >> l;
@@ -637,7 +673,10 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
             kernel<<<nb, p->wblock, 0, st>>>(p->rp, p->ci, val, b, x, sc, n, rhs, l, p->ticket, 0, sched, n_sched);
         };
         const bool small = p->wblock == kBlockSmall;
-        if (sync == CAP_SYNC_VALUE && p->count_polls) {  // the counting instantiations
+        if (p->wblock == kBlockLarge) {  // the multi-RHS pick only
+            if (sync != CAP_SYNC_VALUE || p->lds || p->count_polls || rhs < 2) return CAP_ERR_UNSUPPORTED;
+            go(k_wrtfst<T, 1, 0, kBlockLarge>);
+        } else if (sync == CAP_SYNC_VALUE && p->count_polls) {  // the counting instantiations
             sc.polls = reinterpret_cast<unsigned long long *>(p->ctrl + 1024);
             if (p->lds)
                 small ? go(k_wrtfst<T, 1, 1, kBlockSmall, true>) : go(k_wrtfst<T, 1, 1, kBlock, true>);
@@ -723,6 +762,19 @@ int reset_t(cap_plan p, int algo, int sync, void *x, int rhs, hipStream_t st)
     if (algo == CAP_ALGO_CSC) return launch_csc_reset<T>(p, st);
     return launch_prepare<T>(p, algo, sync, x, (long long)p->n * rhs, st);
 }
+
+// The algorithm and protocol that solve rhs right-hand sides: the plan's,
+// except that an autotuned plan whose pick has no multi-RHS form (mix,
+// two-part, CSC) solves several with the write-first kernel in the value
+// protocol (its RHS block tuned like any other).
+inline int algo_for(cap_plan p, int rhs)
+{
+    if (rhs > 1 && p->auto_algo &&
+        (p->algo == CAP_ALGO_MIX || p->algo == CAP_ALGO_TWOPART || p->algo == CAP_ALGO_CSC))
+        return CAP_ALGO_WRTFST;
+    return p->algo;
+}
+inline int sync_for(cap_plan p, int rhs) { return algo_for(p, rhs) != p->algo ? CAP_SYNC_VALUE : p->sync; }
 
 int plan_solve_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
 {
@@ -860,12 +912,22 @@ int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, int r
 int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
 {
     std::vector<RhsPick> cands;
-    if (p->algo == CAP_ALGO_WRTFST) {
+    const int algo = algo_for(p, k), sync = sync_for(p, k);
+    if (algo == CAP_ALGO_WRTFST) {
         cands.push_back({p->bpc, p->wblock, 0});
         for (int blk : {kBlock, kBlockSmall})
             for (int bpc : {1, 2, 4})
                 if (!(bpc == p->bpc && blk == p->wblock)) cands.push_back({bpc, blk, 0});
-    } else if (p->algo == CAP_ALGO_PARTITION) {
+        // fewer tickets: 1024 threads (one workgroup per CU: 16 wavefronts,
+        // as many as four of 256), and R right-hand sides per lane (R times
+        // the rows per ticket)
+        if (sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls) {
+            cands.push_back({1, kBlockLarge, 0, 1});
+            for (int r : {2, 4, 8})
+                if (r <= k && (r < 8 || p->vt == CAP_F32)) cands.push_back({1, kBlockLarge, 0, r});
+            if (k >= 4) cands.push_back({1, kBlock, 0, 4});
+        }
+    } else if (algo == CAP_ALGO_PARTITION) {
         const int dflt = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
         cands.push_back({0, 0, dflt});
         for (int kr : {1, 2, 4})
@@ -874,16 +936,17 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->rhs_pick[k] = RhsPick{};
         return CAP_OK;
     }
-    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr;
+    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr, rpl0 = p->rpl;
     float best = 1e30f;
     RhsPick pick = cands[0];
     int rc = CAP_OK;
     for (size_t c = 0; c < cands.size() && rc == CAP_OK; ++c) {
         if (cands[c].bpc) p->bpc = cands[c].bpc;
         if (cands[c].block) p->wblock = cands[c].block;
+        if (cands[c].rpl) p->rpl = cands[c].rpl;
         p->part_kr = cands[c].kr;
         float t = 0;
-        rc = time_candidate(p, p->algo, p->sync, b, x, k, st, best, &t);
+        rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
         if (rc == CAP_OK && t < (c ? 0.98f : 1.0f) * best) {
             best = t;
             pick = cands[c];
@@ -891,6 +954,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->bpc = bpc0;
         p->wblock = blk0;
         p->part_kr = kr0;
+        p->rpl = rpl0;
     }
     if (rc != CAP_OK) return rc;
     p->rhs_pick[k] = pick;
@@ -1355,6 +1419,13 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     if (o.wait_limit_ms > kWaitMsMax) return CAP_ERR_ARG;
     if (o.arith == CAP_ARITH_REFERENCE && (o.algo == CAP_ALGO_CSC || o.algo == CAP_ALGO_PARTITION))
         return CAP_ERR_UNSUPPORTED;
+    if (o.rhs_per_lane != 0 && o.rhs_per_lane != 1 && o.rhs_per_lane != 2 && o.rhs_per_lane != 4 &&
+        o.rhs_per_lane != 8)
+        return CAP_ERR_ARG;
+    if (o.rhs_per_lane > 0 &&
+        (o.algo != CAP_ALGO_WRTFST || (o.sync != CAP_SYNC_AUTO && o.sync != CAP_SYNC_VALUE) || o.lds_window ||
+         o.count_polls || (o.rhs_per_lane == 8 && vt != CAP_F32)))
+        return CAP_ERR_UNSUPPORTED;
     *out = nullptr;
     if (o.upper || o.unit_diag) {
         // convert on the host, then create a plan over the owned lower matrix
@@ -1441,6 +1512,8 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.wait_limit_ms != 0) p->wait_ticks = o.wait_limit_ms > 0 ? o.wait_limit_ms * 100000 : 0;
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         if (o.row_threads > 0) p->wblock = o.row_threads;
+        p->fixed_rpl = o.rhs_per_lane;
+        p->auto_algo = o.algo == CAP_ALGO_AUTO;
         p->lds = o.lds_window > 0;
         p->unfused = o.arith == CAP_ARITH_REFERENCE;
         p->count_polls = o.count_polls != 0;
@@ -1571,10 +1644,10 @@ int reverse_rows(cap_plan p, const void *src, void *dst, int rhs, hipStream_t st
 
 int launch_plain(cap_plan p, const void *b, void *x, int rhs, hipStream_t st)
 {
+    const int algo = algo_for(p, rhs), sync = sync_for(p, rhs);
     if (p->vt == CAP_F64)
-        return launch_kernel<double>(p, p->algo, p->sync, (const double *)p->val, (const double *)b, (double *)x, rhs,
-                                     st);
-    return launch_kernel<float>(p, p->algo, p->sync, (const float *)p->val, (const float *)b, (float *)x, rhs, st);
+        return launch_kernel<double>(p, algo, sync, (const double *)p->val, (const double *)b, (double *)x, rhs, st);
+    return launch_kernel<float>(p, algo, sync, (const float *)p->val, (const float *)b, (float *)x, rhs, st);
 }
 
 }  // namespace
@@ -1586,10 +1659,11 @@ int cap_plan_solve(cap_plan p, const void *b, void *x, int rhs, void *stream)
     if (!p || !b || !x || rhs < 1) return CAP_ERR_ARG;
     if (p->tune_rhs && rhs > 1 && rhs <= kWave && !p->reversed && !p->rhs_pick.count(rhs))
         CAP_TRY(tune_rhs_block(p, b, x, rhs, S(stream)));
-    if (!p->reversed) return plan_solve_with(p, p->algo, p->sync, b, x, rhs, S(stream));
+    const int algo = algo_for(p, rhs), sync = sync_for(p, rhs);
+    if (!p->reversed) return plan_solve_with(p, algo, sync, b, x, rhs, S(stream));
     CAP_TRY(ensure_reversal(p, rhs));
     CAP_TRY(reverse_rows(p, b, p->rev_b, rhs, S(stream)));
-    CAP_TRY(plan_solve_with(p, p->algo, p->sync, p->rev_b, p->rev_x, rhs, S(stream)));
+    CAP_TRY(plan_solve_with(p, algo, sync, p->rev_b, p->rev_x, rhs, S(stream)));
     return reverse_rows(p, p->rev_x, x, rhs, S(stream));
 }
 
@@ -1600,8 +1674,9 @@ int cap_plan_reset(cap_plan p, void *x, int rhs, void *stream)
         CAP_TRY(ensure_reversal(p, rhs));
         x = p->rev_x;
     }
-    return p->vt == CAP_F64 ? reset_t<double>(p, p->algo, p->sync, x, rhs, S(stream))
-                            : reset_t<float>(p, p->algo, p->sync, x, rhs, S(stream));
+    const int algo = algo_for(p, rhs), sync = sync_for(p, rhs);
+    return p->vt == CAP_F64 ? reset_t<double>(p, algo, sync, x, rhs, S(stream))
+                            : reset_t<float>(p, algo, sync, x, rhs, S(stream));
 }
 
 int cap_plan_launch(cap_plan p, const void *b, void *x, int rhs, void *stream)
@@ -1655,7 +1730,8 @@ int cap_plan_poll_count(cap_plan p, unsigned long long *polls, void *stream)
     return CAP_OK;
 }
 
-int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, int *rhs_per_wave)
+int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, int *rhs_per_wave,
+                       int *rhs_per_lane)
 {
     if (!p || k < 1) return CAP_ERR_ARG;
     const auto it = p->rhs_pick.find(k);
@@ -1663,6 +1739,8 @@ int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, 
     if (blocks_per_cu) *blocks_per_cu = it->second.bpc ? it->second.bpc : p->bpc;
     if (row_threads) *row_threads = it->second.block ? it->second.block : p->wblock;
     if (rhs_per_wave) *rhs_per_wave = it->second.kr;
+    if (rhs_per_lane)
+        *rhs_per_lane = algo_for(p, k) == CAP_ALGO_WRTFST ? (it->second.rpl ? it->second.rpl : p->rpl) : 0;
     return CAP_OK;
 }
 
@@ -2101,7 +2179,7 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     cap_opts o;
     cap_opts_default(&o);
     if (opts) o = *opts;
-    if (o.upper || o.unit_diag) return CAP_ERR_UNSUPPORTED;
+    if (o.upper || o.unit_diag || o.rhs_per_lane) return CAP_ERR_UNSUPPORTED;
     hipStream_t st = S(stream);
     int dev = 0;
     CAP_TRY(check_device(&dev));

@@ -49,13 +49,17 @@ class Plan:
     arith: "fused" (one fused multiply-add per term) or "reference" (a
     rounded product then a rounded sum per term: bit-identical to the
     reference's serial CPU executor; not with csc or partition).
+    rhs_per_lane (algo wrtfst, value protocol): solves of several right-hand
+    sides run in 1024-thread workgroups with that many per lane (1, 2, 4;
+    8 for FP32); 0 leaves the choice to the workgroup size above (or to the
+    autotuner with algo auto).
     """
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor, algo="auto",
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
                  upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, arith="fused",
-                 wait_limit_ms: int = 0, count_polls: bool = False, stream=None):
+                 wait_limit_ms: int = 0, count_polls: bool = False, rhs_per_lane: int = 0, stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -69,7 +73,8 @@ class Plan:
                               spin_limit=spin_limit, blocks_per_cu=blocks_per_cu, adjoint=adjoint,
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
                               unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith,
-                              wait_limit_ms=wait_limit_ms, count_polls=int(count_polls))
+                              wait_limit_ms=wait_limit_ms, count_polls=int(count_polls),
+                              rhs_per_lane=rhs_per_lane)
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -99,13 +104,14 @@ class Plan:
 
     def rhs_block(self, k: int):
         """The RHS block the autotuner picked for k right-hand sides (after a
-        solve with k): dict(blocks_per_cu, row_threads, rhs_per_wave), or None."""
-        bpc, thr, kr = c_int(), c_int(), c_int()
-        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr))
+        solve with k): dict(blocks_per_cu, row_threads, rhs_per_wave,
+        rhs_per_lane), or None."""
+        bpc, thr, kr, rpl = c_int(), c_int(), c_int(), c_int()
+        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr), byref(rpl))
         if rc == N.CAP_ERR_UNSUPPORTED:
             return None
         N.check(rc, "cap_plan_rhs_block")
-        return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value)
+        return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value, rhs_per_lane=rpl.value)
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

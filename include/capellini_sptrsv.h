@@ -144,6 +144,14 @@ typedef struct cap_opts {
                           (0 = default 10000 ms, < 0 = none, at most 40000):
                           every kernel drains in bounded time, whatever a
                           poll costs under memory contention */
+    int rhs_per_lane;  /* algo WRTFST, value protocol, no LDS window, no poll
+                          counting: solves of k > 1 right-hand sides run in
+                          1024-thread workgroups (one per compute unit) with
+                          this many right-hand sides per lane: 1, 2, 4, or 8
+                          for FP32 (0 = the 128/256-thread solve above, or
+                          the autotuned RHS block with algo AUTO).  Fewer
+                          lanes per row put more rows in each workgroup
+                          ticket, whose atomics serialize on one counter. */
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -181,15 +189,21 @@ int cap_plan_poll_count(cap_plan plan, unsigned long long *polls, void *stream);
 /* The RHS block an autotuned plan (algo AUTO, blocks_per_cu and row_threads
  * left 0) picked for k > 1 right-hand sides.  The first cap_plan_solve with
  * a new k times the candidates on its own b and x and waits for the device:
- * thread-per-row plans 1/2/4 workgroups per CU at 128/256 threads,
- * partitioned plans 1/2/4 right-hand sides per wavefront of the block solve
- * (*rhs_per_wave; 0 for the others).  Every candidate gives the same bits.
+ * thread-per-row plans 1/2/4 workgroups per CU at 128/256 threads, and
+ * 1024-thread workgroups with 1, 2, 4 or (FP32) 8 right-hand sides per
+ * lane (*rhs_per_lane; 0 for the others), partitioned plans 1/2/4 right-hand
+ * sides per wavefront of the block solve (*rhs_per_wave; 0 for the others).
+ * Every candidate gives the same bits.
  * CAP_ERR_UNSUPPORTED until a solve with that k has run.  (The reference's
  * OPT_WARP_RHS / OPT_WARP_AUTO switches, openCL/WrtFst/common.h:28-30, set
  * this by hand at compile time.) */
 int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
-                       int *row_threads, int *rhs_per_wave);
-/* Which algorithm / sync protocol the plan resolved to, and its mix table. */
+                       int *row_threads, int *rhs_per_wave, int *rhs_per_lane);
+/* Which algorithm / sync protocol the plan resolved to, and its mix table.
+ * (An autotuned plan that resolved to mix, two-part or CSC, which solve one
+ * right-hand side, solves several with the write-first kernel in the value
+ * protocol; a plan created with one of those algorithms refuses k > 1 with
+ * CAP_ERR_UNSUPPORTED.) */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
 /* Grid, window and readiness choices the plan uses (after autotuning);

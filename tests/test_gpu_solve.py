@@ -121,6 +121,35 @@ def test_multi_rhs_bit_exact(cuda, k, algo, sync, lds, threads):
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("name,mat", [MATS[0], MATS[3], MATS[5]], ids=["band", "cfg1_small", "chain"])
+def test_rhs_per_lane_bit_exact(cuda, name, mat, dtype):
+    """1024-thread workgroups with 1, 2, 4 (8 for FP32) right-hand sides per
+    lane, at right-hand-side counts that do and do not fill the lanes (a lane
+    owning fewer than R of them): bit-exact against the fused oracle, and in
+    the reference arithmetic against the unfused one."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = mat
+    v = v.astype(dtype)
+    n = len(rp) - 1
+    d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+    rpls = (1, 2, 4, 8) if dtype == np.float32 else (1, 2, 4)
+    for k in (2, 3, 5, 16, 33, 64):
+        b = api.uniform(200 + k, n * k).astype(dtype)
+        bt = torch.from_numpy(b.reshape(n, k)).to(cuda)
+        for arith, fused in (("fused", True), ("reference", False)):
+            if arith == "reference" and k not in (3, 16):
+                continue
+            ref = oracle.csr_forward(rp, ci, v, b, rhs=k, fused=fused)
+            for r in rpls:
+                plan = Plan(*d, algo="wrtfst", sync="value", rhs_per_lane=r, arith=arith)
+                x = plan.solve(bt).cpu().numpy().reshape(-1)
+                assert plan.status() == 0
+                plan.close()
+                assert np.array_equal(x, ref), (name, k, r, arith)
+
+
 def test_local_deep_band_bit_exact(cuda):
     """One workgroup walks a deep banded matrix (n = 20,000, bandwidth 128,
     ~38 nnz/row, thousands of levels) and matches the oracle bit for bit."""
@@ -354,18 +383,22 @@ def test_poll_counter_build(cuda, lds):
     plain.close()
 
 
-@pytest.mark.parametrize("cfg,scale", [(1, 0.05), (2, 0.02)])
+@pytest.mark.parametrize("cfg,scale", [(1, 0.05), (2, 0.02), (0, 1.0)])
 def test_rhs_block_tuned_per_count(cuda, cfg, scale):
     """An autotuned plan times its RHS block on the first solve with each new
-    right-hand-side count (thread-per-row: workgroups per CU and threads per
-    workgroup; partitioned: right-hand sides per wavefront); every solve,
-    tuning or not, gives the oracle's results."""
+    right-hand-side count (thread-per-row: workgroups per CU, threads per
+    workgroup and right-hand sides per lane; partitioned: right-hand sides per
+    wavefront); every solve, tuning or not, gives the oracle's results.  A
+    plan whose single-RHS pick is mix (the 100k band) solves several with the
+    write-first kernel."""
     import torch
     from capellinisptrsv_amd import Plan
     rp, ci, v = api.make_config(cfg, seed=4, scale=scale)
     n = len(rp) - 1
     plan = Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, v)))
     algo = plan.info()["algo"]
+    if algo in ("mix", "twopart", "csc"):
+        algo = "wrtfst"
     for k in (8, 16, 8):
         assert plan.rhs_block(k) is None or k == 8
         b = api.uniform(80 + k, n * k)
@@ -381,5 +414,6 @@ def test_rhs_block_tuned_per_count(cuda, cfg, scale):
         if algo == "partition":
             assert pick["rhs_per_wave"] in (1, 2, 4) and k % pick["rhs_per_wave"] == 0
         elif algo == "wrtfst":
-            assert pick["blocks_per_cu"] in (1, 2, 4) and pick["row_threads"] in (128, 256)
+            assert pick["blocks_per_cu"] in (1, 2, 4) and pick["row_threads"] in (128, 256, 1024)
+            assert pick["rhs_per_lane"] in (1, 2, 4, 8) and pick["rhs_per_lane"] <= k
     plan.close()

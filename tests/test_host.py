@@ -254,6 +254,31 @@ def test_plan_create_refuses_bad_grid_options():
         assert not h.value
 
 
+def test_plan_create_checks_rhs_per_lane():
+    """rhs_per_lane takes 0, 1, 2, 4 or 8 (8 in FP32 only), with the
+    write-first algorithm in the value protocol and neither the LDS window
+    nor the poll counter; batches refuse it -- all before any device call."""
+    from ctypes import byref, c_void_p
+    rp, ci, v = random_lower(50, seed=3)
+    h = c_void_p()
+    cases = ((3, {}, _native.CAP_ERR_ARG), (-1, {}, _native.CAP_ERR_ARG), (16, {}, _native.CAP_ERR_ARG),
+             (2, dict(algo="auto"), _native.CAP_ERR_UNSUPPORTED), (2, dict(algo="mix"), _native.CAP_ERR_UNSUPPORTED),
+             (2, dict(sync="flag"), _native.CAP_ERR_UNSUPPORTED), (2, dict(lds_window=1), _native.CAP_ERR_UNSUPPORTED),
+             (2, dict(count_polls=1), _native.CAP_ERR_UNSUPPORTED), (8, {}, _native.CAP_ERR_UNSUPPORTED))
+    for r, kw, want in cases:
+        kw = dict(dict(algo="wrtfst", sync="value"), **kw)
+        o = _native.default_opts(host_inputs=1, rhs_per_lane=r, **kw)
+        rc = _native.lib().cap_plan_create(byref(h), 50, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                           _native.CAP_F64, byref(o), None)
+        assert rc == want, (r, kw, rc)
+        assert not h.value
+    ro = np.array([0, 50], np.int32)
+    o = _native.default_opts(host_inputs=1, rhs_per_lane=2)
+    rc = _native.lib().cap_batch_create(byref(h), 1, ro.ctypes.data, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                        _native.CAP_F64, 4, byref(o), None)
+    assert rc == _native.CAP_ERR_UNSUPPORTED and not h.value
+
+
 def test_plan_create_checks_the_arithmetic_option():
     """arith takes CAP_ARITH_FUSED or CAP_ARITH_REFERENCE; the reference
     arithmetic is refused by the algorithms whose order is not the sequential
