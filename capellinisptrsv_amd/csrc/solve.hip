@@ -118,9 +118,9 @@ struct cap_plan_s {
     int rpl = 1;      // thread-per-row, several right-hand sides: right-hand sides per lane
     int fixed_rpl = 0;  // cap_opts.rhs_per_lane: the multi-RHS block the caller fixed (0: none)
     bool auto_algo = false;  // created with CAP_ALGO_AUTO
-    // batch: block schedule (relative level order) for blocks of sched_rows rows
-    int2 *sched = nullptr;
-    int n_sched = 0, sched_rows = 0;
+    // batch: block schedules (relative level order), by rows per block: the
+    // thread-per-row solve's and the refinement sweep's (device array, count)
+    std::map<int, std::pair<int2 *, int>> scheds;
     // CAP_ALGO_LOCAL: row segments [x, y) with bandwidth z, one per matrix,
     // largest first; LDS available to one workgroup
     int4 *seg = nullptr;
@@ -293,7 +293,8 @@ void plan_free(cap_plan p)
         (void)hipFree(p->ci);
         (void)hipFree(p->val);
     }
-    void *bufs[] = {p->rev_b, p->rev_x, p->sched, p->seg, p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
+    for (auto &e : p->scheds) (void)hipFree(e.second.first);
+    void *bufs[] = {p->rev_b, p->rev_x, p->seg, p->flags, p->ctrl, p->unit, p->cp, p->cri, p->deg0, p->deg, p->cv, p->left_sum,
                     p->cnt0, p->cnt, p->adj_ls, p->rpT, p->ciT, p->vT, p->val32, p->r32, p->d32, p->pair};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -592,6 +593,16 @@ int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
     return CAP_OK;
 }
 
+// A batch's block schedule for blocks of `rows` rows (nullptr if none was
+// built for that size: the solve then hands out rows in plain order, which
+// is correct, only without the relative-level pacing).
+inline const int2 *sched_for(cap_plan p, int rows, int *n_sched)
+{
+    const auto it = p->scheds.find(rows);
+    *n_sched = it == p->scheds.end() ? 0 : it->second.second;
+    return it == p->scheds.end() ? nullptr : it->second.first;
+}
+
 // Applies the plan's RHS-block pick for k right-hand sides for the length
 // of one launch (and puts the plan's own settings back).
 struct RhsScope {
@@ -640,9 +651,9 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         if (rhs > kWave || sync != CAP_SYNC_VALUE || p->lds || p->count_polls) return CAP_ERR_UNSUPPORTED;
         const int l = lpr_log2_for((rhs + p->rpl - 1) / p->rpl);
         const int rows_per_block = p->wblock >> l;
-        const bool use_sched = p->sched && p->sched_rows == rows_per_block;
-        const int2 *sched = use_sched ? p->sched : nullptr;
-        const int n_sched = use_sched ? p->n_sched : 0;
+        int n_sched = 0;
+        const int2 *sched = sched_for(p, rows_per_block, &n_sched);
+        const bool use_sched = sched != nullptr;
         const unsigned nb =
             grid_for(p, use_sched ? n_sched : ((long long)n + rows_per_block - 1) / rows_per_block);
         auto go = [&](auto kernel) {
@@ -664,9 +675,9 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const int l = lpr_log2_for(rhs);
         const int rows_per_block = p->wblock >> l;
         // a batch's block schedule, when built for this block size
-        const bool use_sched = p->sched && p->sched_rows == rows_per_block;
-        const int2 *sched = use_sched ? p->sched : nullptr;
-        const int n_sched = use_sched ? p->n_sched : 0;
+        int n_sched = 0;
+        const int2 *sched = sched_for(p, rows_per_block, &n_sched);
+        const bool use_sched = sched != nullptr;
         const unsigned nb =
             grid_for(p, use_sched ? n_sched : ((long long)n + rows_per_block - 1) / rows_per_block);
         auto go = [&](auto kernel) {
@@ -1861,11 +1872,12 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     CAP_HIP_TRY(hipGetLastError());
     if (p->n == 0) return CAP_OK;
     const int l = lpr_log2_for(rhs);
-    const int block = p->rblock ? p->rblock : p->wblock;
+    // the sweep's own sizes (a 1024-thread multi-RHS pick is the solve's only)
+    const int block = (p->rblock ? p->rblock : p->wblock) == kBlockSmall ? kBlockSmall : kBlock;
     const int rows_per_block = block >> l;
-    const bool use_sched = p->sched && p->sched_rows == rows_per_block;
-    const int2 *sched = use_sched ? p->sched : nullptr;
-    const int n_sched = use_sched ? p->n_sched : 0;
+    int n_sched = 0;
+    const int2 *sched = sched_for(p, rows_per_block, &n_sched);
+    const bool use_sched = sched != nullptr;
     const unsigned nb = grid_for(p, use_sched ? n_sched : ((long long)p->n + rows_per_block - 1) / rows_per_block);
     const SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, 0, p->wait_ticks};
     const double *val = (const double *)p->val;
@@ -2076,13 +2088,24 @@ namespace {
 // Block schedule of a batch: every matrix is cut into blocks of `rows`
 // consecutive rows, and a stable sort by key interleaves the matrices so
 // that all of them advance in one launch instead of one after another.
-int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipStream_t st)
+// Row levels of a batch's block-diagonal matrix (host copy).
+int batch_levels(cap_plan p, std::vector<int> *lv)
 {
-    const int n = p->n, nnz = p->nnz, n_mat = (int)ro.size() - 1;
-    std::vector<int> rp((size_t)n + 1), ci((size_t)nnz), lv((size_t)std::max(n, 1));
+    const int n = p->n, nnz = p->nnz;
+    std::vector<int> rp((size_t)n + 1), ci((size_t)nnz);
+    lv->assign((size_t)std::max(n, 1), 0);
     CAP_HIP_TRY(hipMemcpy(rp.data(), p->rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
     CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
-    cap_matrix_layer(n, rp.data(), ci.data(), nullptr, nullptr, lv.data());
+    cap_matrix_layer(n, rp.data(), ci.data(), nullptr, nullptr, lv->data());
+    return CAP_OK;
+}
+
+// Builds (once per size) the block schedule for blocks of `rows` rows.
+int build_batch_schedule(cap_plan p, const std::vector<int> &ro, const std::vector<int> &lv, int rows,
+                         hipStream_t st)
+{
+    if (p->scheds.count(rows)) return CAP_OK;
+    const int n = p->n, n_mat = (int)ro.size() - 1;
     struct Blk {
         double key;
         int lo, len;
@@ -2109,11 +2132,11 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipSt
     std::stable_sort(blks.begin(), blks.end(), [](const Blk &a, const Blk &b) { return a.key < b.key; });
     std::vector<int2> h(blks.size());
     for (size_t t = 0; t < blks.size(); ++t) h[t] = make_int2(blks[t].lo, blks[t].len);
-    CAP_TRY(dev_alloc(&p->sched, h.size()));
-    CAP_HIP_TRY(hipMemcpyAsync(p->sched, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice, st));
+    int2 *d = nullptr;
+    CAP_TRY(dev_alloc(&d, std::max<size_t>(h.size(), 1)));
+    p->scheds[rows] = {d, (int)h.size()};
+    CAP_HIP_TRY(hipMemcpyAsync(d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice, st));
     CAP_HIP_TRY(hipStreamSynchronize(st));
-    p->n_sched = (int)h.size();
-    p->sched_rows = rows;
     return CAP_OK;
 }
 
@@ -2123,7 +2146,8 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, int rows, hipSt
 // blocks_per_cu, the thread-per-row grid is timed at 1, 2 and 4 workgroups
 // per compute unit (the schedule's pace depends on how many blocks are in
 // flight); a size must be 2% faster than a smaller one to win.
-int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, hipStream_t st)
+int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std::vector<int> &ro,
+                      const std::vector<int> &lv, hipStream_t st)
 {
     if (requested == CAP_ALGO_LOCAL || (requested == CAP_ALGO_WRTFST && bpc_fixed)) {
         p->algo = requested;
@@ -2155,6 +2179,34 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, hipStream
         }
     }
     p->bpc = best_bpc;
+    // several right-hand sides: 1024-thread workgroups with R per lane put
+    // more rows in each ticket (one atomic on one counter, ~12 ns each),
+    // each with a schedule of its own block size
+    if (rc == CAP_OK && k > 1 && !bpc_fixed && p->sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls) {
+        int best_r = 0;
+        for (int r : {1, 2, 4}) {
+            if (rc != CAP_OK || r > k) break;
+            const int keep_bpc = p->bpc, keep_blk = p->wblock, keep_rpl = p->rpl;
+            p->bpc = 1;
+            p->wblock = kBlockLarge;
+            p->rpl = r;
+            rc = build_batch_schedule(p, ro, lv, kBlockLarge >> lpr_log2_for((k + r - 1) / r), st);
+            float t = 1e30f;
+            if (rc == CAP_OK) rc = time_candidate(p, CAP_ALGO_WRTFST, p->sync, b, x, k, st, t_w, &t);
+            p->bpc = keep_bpc;
+            p->wblock = keep_blk;
+            p->rpl = keep_rpl;
+            if (t < 0.98f * t_w) {
+                t_w = t;
+                best_r = r;
+            }
+        }
+        if (best_r) {
+            p->bpc = 1;
+            p->wblock = kBlockLarge;
+            p->rpl = best_r;
+        }
+    }
     if (rc == CAP_OK && requested == CAP_ALGO_AUTO)
         rc = time_candidate(p, CAP_ALGO_LOCAL, CAP_SYNC_VALUE, b, x, k, st, t_w, &t_l);
     (void)hipStreamSynchronize(st);
@@ -2238,8 +2290,11 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
         return rc;
     }
     p->owns_matrix = true;
-    if ((rc = build_batch_schedule(p, ro, p->wblock >> lpr_log2_for(k), st)) != CAP_OK ||
-        (rc = ensure_segments(p, &ro)) != CAP_OK || (rc = choose_batch_algo(p, o.algo, o.blocks_per_cu > 0, k, st)) != CAP_OK) {
+    std::vector<int> lv;
+    if ((rc = batch_levels(p, &lv)) != CAP_OK ||
+        (rc = build_batch_schedule(p, ro, lv, p->wblock >> lpr_log2_for(k), st)) != CAP_OK ||
+        (rc = ensure_segments(p, &ro)) != CAP_OK ||
+        (rc = choose_batch_algo(p, o.algo, o.blocks_per_cu > 0, k, ro, lv, st)) != CAP_OK) {
         cap_plan_destroy(p);
         return rc;
     }
@@ -2256,6 +2311,14 @@ int cap_batch_info(cap_batch bt, int *algo, int *blocks_per_cu)
     if (!bt) return CAP_ERR_ARG;
     if (algo) *algo = bt->plan->algo;
     if (blocks_per_cu) *blocks_per_cu = bt->plan->bpc;
+    return CAP_OK;
+}
+
+int cap_batch_tuning(cap_batch bt, int *row_threads, int *rhs_per_lane)
+{
+    if (!bt) return CAP_ERR_ARG;
+    if (row_threads) *row_threads = bt->plan->wblock;
+    if (rhs_per_lane) *rhs_per_lane = bt->plan->wblock == kBlockLarge ? bt->plan->rpl : 1;
     return CAP_OK;
 }
 

@@ -202,6 +202,9 @@ class Batch:
         N.check(N.lib().cap_batch_info(h, byref(a), byref(bpc)), "cap_batch_info")
         self.algo = N.ALGO_NAME[a.value]
         self.blocks_per_cu = bpc.value
+        thr, rpl = c_int(), c_int()
+        N.check(N.lib().cap_batch_tuning(h, byref(thr), byref(rpl)), "cap_batch_tuning")
+        self.row_threads, self.rhs_per_lane = thr.value, rpl.value
 
     def solve(self, b: torch.Tensor, out=None, stream=None) -> torch.Tensor:
         b = b.contiguous()

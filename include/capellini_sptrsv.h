@@ -245,7 +245,9 @@ int cap_plan_solve_refined(cap_plan plan64, const double *b, double *x, int rhs,
  * every matrix advances at the same pace), CAP_ALGO_LOCAL (one workgroup per
  * matrix, largest first) or CAP_ALGO_AUTO (both timed at creation, the
  * faster kept).  opts->blocks_per_cu = 0 lets creation time 1, 2 and 4
- * workgroups per compute unit for the thread-per-row schedule. */
+ * workgroups per compute unit for the thread-per-row schedule, and, with
+ * k > 1 (value protocol, no LDS window), 1024-thread workgroups with 1, 2
+ * or 4 right-hand sides per lane, each with a schedule of its block size. */
 typedef struct cap_batch_s *cap_batch;
 int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
                      const int *row_ptr, const int *col_idx, const void *val,
@@ -253,6 +255,9 @@ int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
 /* The algorithm the batch resolved to (CAP_ALGO_WRTFST or CAP_ALGO_LOCAL)
  * and its persistent grid size in workgroups per compute unit. */
 int cap_batch_info(cap_batch batch, int *algo, int *blocks_per_cu);
+/* No counterpart in the reference: the thread-per-row solve's workgroup size
+ * (128, 256 or 1024) and right-hand sides per lane the batch resolved to. */
+int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane);
 int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
 int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
                             double tol, int max_iter, int *iters,
