@@ -1574,14 +1574,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ------------------------------------------------ partitioned banded solve
-// For deep matrices whose lower bandwidth w = max(i - col) is at most kPartW
-// (the dependency chain is ~n/levels rows long and no row order can shorten
-// it).  Rows are cut into P blocks of m >= kPartW rows; block p's rows only
-// reach back into the last kPartW rows of block p-1.  With D the
+// For deep matrices whose lower bandwidth w = max(i - col) is at most the
+// tail width W (the dependency chain is ~n/levels rows long and no row order
+// can shorten it).  Rows are cut into P blocks of m >= W rows; block p's
+// rows only reach back into the last W rows of block p-1.  With D the
 // block-diagonal part of L, y = D^{-1} b (independent blocks) and V_p =
-// D_pp^{-1} C_p (C_p: block p's entries in the coupling columns [pm - kPartW,
-// pm)), the exact solution is x_p = y_p - V_p t_{p-1} with t_p the last
-// kPartW values of x_p:
+// D_pp^{-1} C_p (C_p: block p's entries in the coupling columns [pm - W,
+// pm)), the exact solution is x_p = y_p - V_p t_{p-1} with t_p the last W
+// values of x_p:
 //   t_p = a_p + B_p t_{p-1},   a_p = tail(y_p),   B_p = -tail(V_p).
 // The recurrence over the Q = P - 1 tails is scanned in two levels: in
 // groups of gs consecutive tails sequentially (u_p from the group's first
@@ -1590,58 +1590,61 @@ __global__ void __launch_bounds__(kBlock)
 // t_p = u_p + W_p T_{g-1}.  The matrix products are formed once per plan
 // (k_part_gemm); a solve only touches vectors.  The sums are reassociated,
 // so results match the sequential solve to rounding, not bit for bit.
-// Matrices are column-major kPartW x kPartW (element (c, k) at k * kPartW +
-// c); V is two halves of 64 interleaved columns (n x 64 each).  The tables
-// and the solve's work vectors are FP64 for FP32 plans too: b is widened
-// when the block kernel reads it and x rounded once when the finish writes
-// it.
-constexpr int kPartW = 128;
+// The tail width W (the number of coupling columns, 128 or 256) is the
+// smallest of those at least the matrix's lower bandwidth; blocks hold at
+// least W rows.  Matrices are column-major W x W (element (c, k) at k * W +
+// c); V comes from W / 64 batch solves of 64 interleaved columns (n x 64
+// each, one after another).  The tables and the solve's work vectors are
+// FP64 for FP32 plans too: b is widened when the block kernel reads it and x
+// rounded once when the finish writes it.
+constexpr int kPartW = 128;     // the narrower tail width
+constexpr int kPartWMax = 256;  // the wider one
 
-// Coupling right-hand sides (two zeroed halves of 64 interleaved columns):
-// an entry of row i in block p >= 1 whose column c lies before the block
-// goes to column q = c - (pm - kPartW); duplicated entries add up.
+// Coupling right-hand sides (W / 64 zeroed slabs of 64 interleaved columns,
+// slab h at r + h * n * 64): an entry of row i in block p >= 1 whose column c
+// lies before the block goes to column q = c - (pm - W); duplicated entries
+// add up.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_part_coupling(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val, int n, int m,
-                    double *__restrict__ r0, double *__restrict__ r1)
+                    int W, double *__restrict__ r)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int start = (i / m) * m;
     if (start == 0) return;
-    const int base = start - kPartW;
+    const int base = start - W;
     for (int j = rp[i]; j < rp[i + 1]; ++j) {
         const int c = ci[j];
         const int q = c - base;
         if (c >= start || q < 0) continue;  // inside the block; q < 0 cannot occur (bandwidth checked on the host)
-        double *dst = q < 64 ? r0 + (long long)i * 64 + q : r1 + (long long)i * 64 + (q - 64);
-        *dst += (double)val[j];
+        r[(long long)(q >> 6) * n * 64 + (long long)i * 64 + (q & 63)] += (double)val[j];
     }
 }
 
-// V from the two 64-column halves the batch solve writes (row-major, 64
-// per row) to column-major vt (column k at k * n): the finish then reads
-// consecutive rows per column.
+// V from the 64-column slabs the batch solves write (row-major, 64 per row)
+// to column-major vt (column k at k * n): the finish then reads consecutive
+// rows per column.
 __global__ void __launch_bounds__(kBlock)
-    k_part_transpose(const double *__restrict__ v0, const double *__restrict__ v1, int n, double *__restrict__ vt)
+    k_part_transpose(const double *__restrict__ v, int n, int W, double *__restrict__ vt)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)n * kPartW) return;
+    if (g >= (long long)n * W) return;
     const int k = (int)(g / n), i = (int)(g % n);
-    vt[g] = k < 64 ? v0[(long long)i * 64 + k] : v1[(long long)i * 64 + (k - 64)];
+    vt[g] = v[(long long)(k >> 6) * n * 64 + (long long)i * 64 + (k & 63)];
 }
 
-// B_p = -(the last kPartW rows of V_p) for p in [1, q_count), stored at
-// index p - 1 (the first round's matrices).
+// B_p = -(the last W rows of V_p) for p in [1, q_count), stored at index
+// p - 1 (the first round's matrices).
 __global__ void __launch_bounds__(kBlock)
-    k_part_tail_matrix(const double *__restrict__ vt, int n, int m, int q_count, double *__restrict__ b0)
+    k_part_tail_matrix(const double *__restrict__ vt, int n, int m, int q_count, int W, double *__restrict__ b0)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)(q_count - 1) * kPartW * kPartW) return;
-    const int c = (int)(g % kPartW);
-    const int k = (int)((g / kPartW) % kPartW);
-    const int p = 1 + (int)(g / ((long long)kPartW * kPartW));
-    const long long row = (long long)(p + 1) * m - kPartW + c;
+    if (g >= (long long)(q_count - 1) * W * W) return;
+    const int c = (int)(g % W);
+    const int k = (int)((g / W) % W);
+    const int p = 1 + (int)(g / ((long long)W * W));
+    const long long row = (long long)(p + 1) * m - W + c;
     b0[g] = -vt[(long long)k * n + row];
 }
 
@@ -1649,8 +1652,8 @@ __global__ void __launch_bounds__(kBlock)
 // entries in columns i-1 and i-2 are kept apart (near: their values, summed
 // per column); the others (columns <= i-3) are stored ELL-style, ew <=
 // kPartEll per row (value, and the byte offset of the column's slot in the
-// LDS ring of the block's values: ((col - block start) mod kPartRing) * 8;
-// pads point at slot kPartRing, which holds 0), the rest of a longer row in
+// LDS ring of the block's values: ((col - block start) mod 2W) * 8; pads
+// point at slot 2W, which holds 0), the rest of a longer row in
 // an overflow CSR (ov_rp per row, absolute columns).  Entry l of a row sits
 // in lane l; its products with the block's earlier values are summed through
 // LDS (8 lanes add 8 products each, then an 8-lane butterfly), and
@@ -1662,7 +1665,8 @@ __global__ void __launch_bounds__(kBlock)
 // bit-exact), like the whole partitioned solve.
 constexpr int kPartEll = 64;
 constexpr int kPartPre = 16;
-constexpr int kPartRing = 2 * kPartW;  // ring slots (rows' values by row mod kPartRing)
+// ring slots of a tail width W (rows' values by row mod 2W)
+constexpr int part_ring(int W) { return 2 * W; }
 
 // A double moved between lanes by DPP (the GFX9 DPP set).  Lanes whose
 // source lies outside their row read 0 (bound_ctrl); lanes in rows outside
@@ -1700,14 +1704,14 @@ __device__ __forceinline__ double lane_read(double v, int src)
 }
 
 
-template <int KR, typename T>
+template <int W, int KR, typename T>
 __global__ void __launch_bounds__(kWave)
     k_part_block(const int *__restrict__ ell_c, const double *__restrict__ ell_v, const int *__restrict__ ov_rp,
                  const int *__restrict__ ov_c, const double *__restrict__ ov_v, const double *__restrict__ inv_diag,
                  const double2 *__restrict__ near, const T *__restrict__ b, double *__restrict__ y,
                  double *__restrict__ tails, int n, int m, int ew, int k, int q_count)
 {
-    constexpr int kRing = kPartRing;
+    constexpr int kRing = part_ring(W);
     constexpr int kRow = KR + 3;  // a row's scalars in LDS: b per RHS, 1 / diagonal, near (i-1), near (i-2)
     __shared__ double ring[KR][kRing + 1];
     __shared__ double rowd[kPartPre][kRow];
@@ -1895,48 +1899,55 @@ __global__ void __launch_bounds__(kWave)
             group(std::false_type{});
     }
     store_group(start + (end - 1 - start) / kPartPre * kPartPre);  // the last group
-    // a_p = the block's last kPartW values, the tail scan's input (blocks
-    // p < q_count, which are whole: m >= kPartW rows), at (p * k + q) *
-    // kPartW + c, from the ring
+    // a_p = the block's last W values, the tail scan's input (blocks p <
+    // q_count, which are whole: m >= W rows), at (p * k + q) * W + c, from
+    // the ring
     if ((int)blockIdx.x < q_count) {
 #pragma unroll
-        for (int c = lane; c < kPartW; c += kWave) {
-            const int slot = (end - kPartW + c - start) & (kRing - 1);
+        for (int c = lane; c < W; c += kWave) {
+            const int slot = (end - W + c - start) & (kRing - 1);
 #pragma unroll
-            for (int j = 0; j < KR; ++j) tails[((long long)blockIdx.x * k + q0 + j) * kPartW + c] = ring[j][slot];
+            for (int j = 0; j < KR; ++j) tails[((long long)blockIdx.x * k + q0 + j) * W + c] = ring[j][slot];
         }
     }
 }
 
-// Plan-time matrix products C = A * B of the partition's tables, one per
-// workgroup: ops[i] = {C, A, B} as matrix indices into bm (B < 0: C = A, a
-// copy).  Each of the 256 threads owns an 8 x 8 tile of C; the inner
-// dimension is staged through LDS 16 at a time.
+// Plan-time matrix products C = A * B of the partition's W x W tables:
+// ops[i] = {C, A, B} as matrix indices into bm (B < 0: C = A, a copy), one
+// per blockIdx.x, each split into (W / 128)^2 quadrants of 128 x 128 by
+// blockIdx.y.  Each of the 256 threads owns an 8 x 8 tile of its quadrant;
+// the inner dimension is staged through LDS 16 at a time.
+template <int W>
 __global__ void __launch_bounds__(256) k_part_gemm(const int3 *__restrict__ ops, double *__restrict__ bm)
 {
-    constexpr long long kMat = (long long)kPartW * kPartW;
+    constexpr long long kMat = (long long)W * W;
+    constexpr int kQ = 128, kQn = W / kQ;
     const int3 op = ops[blockIdx.x];
     const double *A = bm + (long long)op.y * kMat;
     double *C = bm + (long long)op.x * kMat;
     const int t = threadIdx.x;
-    if (op.z < 0) {  // copy (uniform)
-        for (int e = t; e < kPartW * kPartW; e += 256) C[e] = A[e];
+    const int qr = (int)blockIdx.y % kQn * kQ, qc = (int)blockIdx.y / kQn * kQ;  // quadrant: rows, columns
+    if (op.z < 0) {  // copy (uniform) of the quadrant
+        for (int e = t; e < kQ * kQ; e += 256) {
+            const long long o = (long long)(qc + e / kQ) * W + qr + e % kQ;
+            C[o] = A[o];
+        }
         return;
     }
     const double *B = bm + (long long)op.z * kMat;
-    __shared__ double sa[16][kPartW];  // A(c, j0 + jj)
-    __shared__ double sb[16][kPartW];  // B(j0 + jj, k)
+    __shared__ double sa[16][kQ];  // A(qr + c, j0 + jj)
+    __shared__ double sb[16][kQ];  // B(j0 + jj, qc + k)
     const int c0 = (t & 15) * 8, k0 = (t >> 4) * 8;
     double acc[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
-    for (int j0 = 0; j0 < kPartW; j0 += 16) {
+    for (int j0 = 0; j0 < W; j0 += 16) {
         __syncthreads();
-        for (int e = t; e < 16 * kPartW; e += 256) {
-            sa[e / kPartW][e % kPartW] = A[(long long)(j0 + e / kPartW) * kPartW + e % kPartW];
-            sb[e % 16][e / 16] = B[(long long)(e / 16) * kPartW + j0 + e % 16];
+        for (int e = t; e < 16 * kQ; e += 256) {
+            sa[e / kQ][e % kQ] = A[(long long)(j0 + e / kQ) * W + qr + e % kQ];
+            sb[e % 16][e / 16] = B[(long long)(qc + e / 16) * W + j0 + e % 16];
         }
         __syncthreads();
 #pragma unroll 4
@@ -1955,15 +1966,15 @@ __global__ void __launch_bounds__(256) k_part_gemm(const int3 *__restrict__ ops,
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) C[(long long)(k0 + b) * kPartW + c0 + a] = acc[a][b];
+        for (int b = 0; b < 8; ++b) C[(long long)(qc + k0 + b) * W + qr + c0 + a] = acc[a][b];
 }
 
-// y_j = M v_j for one kPartW x kPartW column-major M and KQ vectors v_j in
-// LDS: thread c returns row c of each (four partial sums, by index mod 4, in
-// the same order for every KQ, so a right-hand side gets the same bits
-// whichever chunk it is solved in).  M is read once for all KQ vectors.
-template <int KQ, bool NT>
-__device__ __forceinline__ void part_matvec(const double *__restrict__ M, const double (*sv)[kPartW], int c,
+// y_j = M v_j for one W x W column-major M and KQ vectors v_j in LDS:
+// thread c returns row c of each (four partial sums, by index mod 4, in the
+// same order for every KQ, so a right-hand side gets the same bits whichever
+// chunk it is solved in).  M is read once for all KQ vectors.
+template <int W, int KQ, bool NT>
+__device__ __forceinline__ void part_matvec(const double *__restrict__ M, const double (*sv)[W], int c,
                                             double (&out)[KQ])
 {
     double d[KQ][4];
@@ -1973,11 +1984,11 @@ __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const 
         for (int u = 0; u < 4; ++u) d[j][u] = 0.0;
     constexpr int kUnroll = KQ >= 8 ? 2 : 8;
 #pragma unroll kUnroll
-    for (int kk = 0; kk < kPartW; kk += 4) {
+    for (int kk = 0; kk < W; kk += 4) {
         double mv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * kPartW + c) : M[(long long)(kk + u) * kPartW + c];
+            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * W + c) : M[(long long)(kk + u) * W + c];
 #pragma unroll
         for (int j = 0; j < KQ; ++j)
 #pragma unroll
@@ -1989,8 +2000,8 @@ __device__ __forceinline__ void part_matvec(const double *__restrict__ M, const 
 
 // The part of y_j = M v_j that columns k0 .. k0 + KLEN - 1 of M contribute,
 // for row c (four partial sums by index mod 4, joined as in part_matvec).
-template <int KQ, int KLEN, bool NT>
-__device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, const double (*sv)[kPartW], int c, int k0,
+template <int W, int KQ, int KLEN, bool NT>
+__device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, const double (*sv)[W], int c, int k0,
                                                  double (&out)[KQ])
 {
     double d[KQ][4];
@@ -2003,7 +2014,7 @@ __device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, c
         double mv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * kPartW + c) : M[(long long)(kk + u) * kPartW + c];
+            mv[u] = NT ? __builtin_nontemporal_load(M + (long long)(kk + u) * W + c) : M[(long long)(kk + u) * W + c];
 #pragma unroll
         for (int j = 0; j < KQ; ++j)
 #pragma unroll
@@ -2018,124 +2029,124 @@ __device__ __forceinline__ void part_matvec_cols(const double *__restrict__ M, c
 constexpr int kPartSplit = 4;
 
 // The scan's vectors hold k right-hand sides: tail p of RHS q at (p * k + q)
-// * kPartW.  A workgroup takes the KQ right-hand sides from blockIdx.y * KQ;
+// * W.  A workgroup takes the KQ right-hand sides from blockIdx.y * KQ;
 // part_vec gives the j-th of them.
-template <int KQ>
+template <int W, int KQ>
 __device__ __forceinline__ long long part_vec(int p, int k, int j)
 {
-    return ((long long)p * k + (long long)blockIdx.y * KQ + j) * kPartW;
+    return ((long long)p * k + (long long)blockIdx.y * KQ + j) * W;
 }
 
-// Level 1 of the tail scan, one workgroup of kPartSplit x kPartW threads per
+// Level 1 of the tail scan, one workgroup of kPartSplit x W threads per
 // group of `gs` consecutive tails and chunk of KQ right-hand sides: u_first
 // = a_first, u_p = a_p + B_p u_{p-1} (B_p at index p - 1 of bm).  The group's
 // last u also goes to carry[g].  A step is sequential, so each product is
 // split over the columns (thread: row c, quarter kq) to have four times the
-// loads in flight; the first kPartW threads own the vectors.
-template <int KQ, bool NT>
-__global__ void __launch_bounds__(kPartSplit * kPartW)
+// loads in flight; the first W threads own the vectors.
+template <int W, int KQ, bool NT>
+__global__ void __launch_bounds__(kPartSplit * W)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
                       const double *__restrict__ bm, int gs, int q_count, int k)
 {
-    constexpr int kLen = kPartW / kPartSplit;
-    const int g = (int)blockIdx.x, t = (int)threadIdx.x, c = t % kPartW, kq = t / kPartW;
+    constexpr int kLen = W / kPartSplit;
+    const int g = (int)blockIdx.x, t = (int)threadIdx.x, c = t % W, kq = t / W;
     const int p0 = g * gs, p1 = min(q_count, p0 + gs);
-    __shared__ double sv[KQ][kPartW];
-    __shared__ double red[kPartSplit][KQ][kPartW];
+    __shared__ double sv[KQ][W];
+    __shared__ double red[kPartSplit][KQ][W];
     double v[KQ];
-    if (t < kPartW) {
+    if (t < W) {
 #pragma unroll
         for (int j = 0; j < KQ; ++j) {
-            v[j] = a[part_vec<KQ>(p0, k, j) + c];
-            u[part_vec<KQ>(p0, k, j) + c] = v[j];
+            v[j] = a[part_vec<W, KQ>(p0, k, j) + c];
+            u[part_vec<W, KQ>(p0, k, j) + c] = v[j];
         }
     }
     for (int p = p0 + 1; p < p1; ++p) {
         __syncthreads();  // the previous product is done with sv and red
-        if (t < kPartW) {
+        if (t < W) {
 #pragma unroll
             for (int j = 0; j < KQ; ++j) sv[j][c] = v[j];
         }
         __syncthreads();
         double mv[KQ];
-        part_matvec_cols<KQ, kLen, NT>(bm + (long long)(p - 1) * kPartW * kPartW, sv, c, kq * kLen, mv);
+        part_matvec_cols<W, KQ, kLen, NT>(bm + (long long)(p - 1) * W * W, sv, c, kq * kLen, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) red[kq][j][c] = mv[j];
         __syncthreads();
-        if (t < kPartW) {
+        if (t < W) {
 #pragma unroll
             for (int j = 0; j < KQ; ++j) {
-                v[j] = a[part_vec<KQ>(p, k, j) + c] + ((red[0][j][c] + red[1][j][c]) + (red[2][j][c] + red[3][j][c]));
-                u[part_vec<KQ>(p, k, j) + c] = v[j];
+                v[j] = a[part_vec<W, KQ>(p, k, j) + c] + ((red[0][j][c] + red[1][j][c]) + (red[2][j][c] + red[3][j][c]));
+                u[part_vec<W, KQ>(p, k, j) + c] = v[j];
             }
         }
     }
-    if (t < kPartW) {
+    if (t < W) {
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) carry[part_vec<KQ>(g, k, j) + c] = v[j];
+        for (int j = 0; j < KQ; ++j) carry[part_vec<W, KQ>(g, k, j) + c] = v[j];
     }
 }
 
 // Level 3: t_p = u_p + W_p T_{g-1} for tails of groups g >= 1 (W_p at index
 // p - gs of wm, T the scanned group carries), t_p = u_p in group 0.
-template <int KQ, bool NT>
-__global__ void __launch_bounds__(kPartW)
+template <int W, int KQ, bool NT>
+__global__ void __launch_bounds__(W)
     k_part_fixup(const double *__restrict__ u, const double *__restrict__ tg, const double *__restrict__ wm, int gs,
                  double *__restrict__ t, int k)
 {
     const int p = (int)blockIdx.x, c = (int)threadIdx.x;
     const int g = p / gs;
-    __shared__ double sv[KQ][kPartW];
+    __shared__ double sv[KQ][W];
     double v[KQ];
 #pragma unroll
-    for (int j = 0; j < KQ; ++j) v[j] = u[part_vec<KQ>(p, k, j) + c];
+    for (int j = 0; j < KQ; ++j) v[j] = u[part_vec<W, KQ>(p, k, j) + c];
     if (g > 0) {  // uniform in the workgroup
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) sv[j][c] = tg[part_vec<KQ>(g - 1, k, j) + c];
+        for (int j = 0; j < KQ; ++j) sv[j][c] = tg[part_vec<W, KQ>(g - 1, k, j) + c];
         __syncthreads();
         double mv[KQ];
-        part_matvec<KQ, NT>(wm + (long long)(p - gs) * kPartW * kPartW, sv, c, mv);
+        part_matvec<W, KQ, NT>(wm + (long long)(p - gs) * W * W, sv, c, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) v[j] += mv[j];
     }
 #pragma unroll
-    for (int j = 0; j < KQ; ++j) t[part_vec<KQ>(p, k, j) + c] = v[j];
+    for (int j = 0; j < KQ; ++j) t[part_vec<W, KQ>(p, k, j) + c] = v[j];
 }
 
 
 // One round of the log-depth scan over the group carries (kPartSplit
-// workgroups of kPartW threads per entry and chunk of KQ right-hand sides,
+// workgroups of W threads per entry and chunk of KQ right-hand sides,
 // each for a quarter of the rows: the round has only G entries, so each
 // product is spread over four workgroups; thread: row, quarter of the
 // columns, partial sums joined in LDS): dst_g = src_g + M_g src_{g-s} for g >= s (M_g at
 // index g - s of the round's matrices), dst_g = src_g below s.
-template <int KQ, bool NT>
-__global__ void __launch_bounds__(kPartW)
+template <int W, int KQ, bool NT>
+__global__ void __launch_bounds__(W)
     k_part_scan(const double *__restrict__ src, double *__restrict__ dst, const double *__restrict__ bm, int s, int k)
 {
-    constexpr int kRows = kPartW / kPartSplit, kLen = kPartW / kPartSplit;
+    constexpr int kRows = W / kPartSplit, kLen = W / kPartSplit;
     const int p = (int)blockIdx.x / kPartSplit, part = (int)blockIdx.x % kPartSplit;
     const int t = (int)threadIdx.x, r = t % kRows, kq = t / kRows, c = part * kRows + r;
-    __shared__ double sv[KQ][kPartW];
+    __shared__ double sv[KQ][W];
     __shared__ double red[kPartSplit][KQ][kRows];
     if (p >= s) {  // uniform in the workgroup
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) sv[j][t] = src[part_vec<KQ>(p - s, k, j) + t];
+        for (int j = 0; j < KQ; ++j) sv[j][t] = src[part_vec<W, KQ>(p - s, k, j) + t];
         __syncthreads();
         double mv[KQ];
-        part_matvec_cols<KQ, kLen, NT>(bm + (long long)(p - s) * kPartW * kPartW, sv, c, kq * kLen, mv);
+        part_matvec_cols<W, KQ, kLen, NT>(bm + (long long)(p - s) * W * W, sv, c, kq * kLen, mv);
 #pragma unroll
         for (int j = 0; j < KQ; ++j) red[kq][j][r] = mv[j];
         __syncthreads();
         if (t < kRows) {
 #pragma unroll
             for (int j = 0; j < KQ; ++j)
-                dst[part_vec<KQ>(p, k, j) + c] = src[part_vec<KQ>(p, k, j) + c] +
-                                                 ((red[0][j][r] + red[1][j][r]) + (red[2][j][r] + red[3][j][r]));
+                dst[part_vec<W, KQ>(p, k, j) + c] = src[part_vec<W, KQ>(p, k, j) + c] +
+                                                    ((red[0][j][r] + red[1][j][r]) + (red[2][j][r] + red[3][j][r]));
         }
     } else if (t < kRows) {
 #pragma unroll
-        for (int j = 0; j < KQ; ++j) dst[part_vec<KQ>(p, k, j) + c] = src[part_vec<KQ>(p, k, j) + c];
+        for (int j = 0; j < KQ; ++j) dst[part_vec<W, KQ>(p, k, j) + c] = src[part_vec<W, KQ>(p, k, j) + c];
     }
 }
 
@@ -2143,7 +2154,7 @@ __global__ void __launch_bounds__(kPartW)
 // block 0.  One thread per (row, RHS), g = row * k + q: V column-major, so
 // for each column the lanes read consecutive rows (the k lanes of one row
 // share an element).
-template <typename T>
+template <int W, typename T>
 __global__ void __launch_bounds__(kBlock)
     k_part_finish(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t, int n,
                   int m, int k, T *__restrict__ x)
@@ -2154,10 +2165,10 @@ __global__ void __launch_bounds__(kBlock)
     const int p = i / m;
     double xi = y[gi];
     if (p > 0) {
-        const double *tp = t + ((long long)(p - 1) * k + q) * kPartW;
+        const double *tp = t + ((long long)(p - 1) * k + q) * W;
         double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
-        for (int c = 0; c < kPartW; c += 4)
+        for (int c = 0; c < W; c += 4)
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 s[u] = __builtin_fma(__builtin_nontemporal_load(vt + (long long)(c + u) * n + i), tp[c + u], s[u]);
@@ -2167,26 +2178,26 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // The finish for chunks of KQ >= 2 right-hand sides: a workgroup takes
-// kPartW consecutive rows (at most two blocks, as m >= kPartW) and RHS
+// W consecutive rows (at most two blocks, as m >= W) and RHS
 // blockIdx.y * KQ + j.  It stages both blocks' t_{p-1} in LDS (RHS fastest,
 // read as broadcasts), and each thread reads its row of V once for all KQ.
 // The sums run in the single-RHS kernel's order (four partial sums by
 // column mod 4), so each RHS gets that kernel's bits.  NT: V is read with
 // non-temporal loads (one chunk covers every RHS, so V is read once; with
 // several chunks, cached reads let later chunks reuse some of it).
-template <int KQ, typename T, bool NT>
-__global__ void __launch_bounds__(kPartW)
+template <int W, int KQ, typename T, bool NT>
+__global__ void __launch_bounds__(W)
     k_part_finish_multi(const double *__restrict__ y, const double *__restrict__ vt, const double *__restrict__ t,
                         int n, int m, int k, T *__restrict__ x)
 {
-    __shared__ double st[2][kPartW][KQ];
-    const int lo = (int)blockIdx.x * kPartW;
+    __shared__ double st[2][W][KQ];
+    const int lo = (int)blockIdx.x * W;
     const long long q0 = (long long)blockIdx.y * KQ;
-    const int p_lo = lo / m, p_hi = (min(n, lo + kPartW) - 1) / m;
-    for (int e = (int)threadIdx.x; e < 2 * KQ * kPartW; e += kPartW) {
-        const int sblk = e / (KQ * kPartW), j = (e / kPartW) % KQ, c = e % kPartW;
+    const int p_lo = lo / m, p_hi = (min(n, lo + W) - 1) / m;
+    for (int e = (int)threadIdx.x; e < 2 * KQ * W; e += W) {
+        const int sblk = e / (KQ * W), j = (e / W) % KQ, c = e % W;
         const int p = sblk == 0 ? p_lo : p_hi;
-        st[sblk][c][j] = p > 0 ? t[((long long)(p - 1) * k + q0 + j) * kPartW + c] : 0.0;
+        st[sblk][c][j] = p > 0 ? t[((long long)(p - 1) * k + q0 + j) * W + c] : 0.0;
     }
     __syncthreads();
     const int i = lo + (int)threadIdx.x;
@@ -2200,7 +2211,7 @@ __global__ void __launch_bounds__(kPartW)
         for (int u = 0; u < 4; ++u) acc[j][u] = 0.0;
     if (p > 0) {
 #pragma unroll 2
-        for (int c = 0; c < kPartW; c += 4) {
+        for (int c = 0; c < W; c += 4) {
             double mv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)

@@ -57,12 +57,14 @@ inline int lpr_log2_for(int k)
 }  // namespace
 
 // State of a partitioned banded solve: blocks of m rows (P of them, Q = P - 1
-// coupled tails); the block-diagonal part's off-diagonals in ELL slabs of
-// ew per row plus an overflow CSR, and its reciprocal diagonal; V = D^{-1} C
-// column-major (n x kPartW); the tail scan's matrices in bm: B_p at p - 1,
+// coupled tails of W values, W = 128 or 256); the block-diagonal part's
+// off-diagonals in ELL slabs of ew per row plus an overflow CSR, and its
+// reciprocal diagonal; V = D^{-1} C column-major (n x W); the tail scan's
+// W x W matrices in bm: B_p at p - 1,
 // the in-group products W_p at w_off + p - gs, and for each round r of the
 // group-carry scan G - 2^r matrices from m_off[r]; the solve's work vectors.
 struct PartState {
+    int W = kPartW;                   // tail width: the coupling columns
     int m = 0, P = 0, Q = 0, ew = 0;  // ew: ELL slab width (<= kPartEll)
     int gs = 0, G = 0, rounds = 0;    // tails per group, groups, carry-scan rounds
     int *ell_c = nullptr, *ov_rp = nullptr, *ov_c = nullptr;
@@ -167,6 +169,10 @@ struct cap_batch_s {
     cap_plan plan = nullptr;
     int k = 1;
     int n_mat = 0;
+    // hybrid: the deepest matrices solved by partitioned plans of their own
+    // (first rows part_lo), the rest by `plan` over a schedule without them
+    std::vector<cap_plan> parts;
+    std::vector<int> part_lo;
 };
 
 namespace {
@@ -457,7 +463,7 @@ int ensure_part_rhs(cap_plan p, int k)
         *v = nullptr;  // res / cor come back at the new size when a refined solve needs them
     }
     ps->kcap = 0;
-    const size_t tails = (size_t)ps->Q * k * kPartW, carries = (size_t)ps->G * k * kPartW;
+    const size_t tails = (size_t)ps->Q * k * ps->W, carries = (size_t)ps->G * k * ps->W;
     CAP_TRY(dev_alloc(&ps->y, (size_t)p->n * k));
     CAP_TRY(dev_alloc(&ps->a, tails));
     CAP_TRY(dev_alloc(&ps->u, tails));
@@ -485,11 +491,11 @@ inline int part_chunk(int k, int cap)
     return c;
 }
 
-template <int KR, typename T>
+template <int W, int KR, typename T>
 void launch_part_block(cap_plan p, const T *b, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
-    k_part_block<KR, T><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
+    k_part_block<W, KR, T><<<dim3((unsigned)ps->P, (unsigned)(k / KR)), kWave, 0, st>>>(
         ps->ell_c, ps->ell_v, ps->ov_rp, ps->ov_c, ps->ov_v, ps->diag, (const double2 *)ps->near, b, ps->y, ps->a,
         p->n, ps->m, ps->ew, k, ps->Q);
 }
@@ -497,7 +503,8 @@ void launch_part_block(cap_plan p, const T *b, int k, hipStream_t st)
 // Right-hand sides per workgroup of the group scan and the carry rounds, at
 // most: those grids have only G (= P / gs) workgroups per chunk, and each step
 // of a workgroup is a chain, so they keep more, smaller chunks than the
-// fix-up and the finish (Q and n / kPartW workgroups), which take 16.
+// fix-up and the finish (Q and n / W workgroups), which take 16 (8 at W =
+// 256: the finish stages 2 x W x KQ values in LDS).
 constexpr int kPartScanRhs = 4;
 
 // The tail scan over the group carries, KQ right-hand sides per workgroup
@@ -505,52 +512,53 @@ constexpr int kPartScanRhs = 4;
 // matrices are read with non-temporal loads, when one chunk covers every
 // RHS and each matrix is read once (with several chunks, cached reads let
 // later chunks reuse them).
-template <int KQ, bool NT>
+template <int W, int KQ, bool NT>
 double *part_carries(cap_plan p, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     const int Q = ps->Q;
     const unsigned chunks = (unsigned)(k / KQ);
-    const long long mat = (long long)kPartW * kPartW;
-    k_part_group_scan<KQ, NT><<<dim3(ps->G, chunks), kPartSplit * kPartW, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm, ps->gs,
-                                                                              Q, k);
+    const long long mat = (long long)W * W;
+    k_part_group_scan<W, KQ, NT><<<dim3(ps->G, chunks), kPartSplit * W, 0, st>>>(ps->a, ps->u, ps->c0, ps->bm,
+                                                                                 ps->gs, Q, k);
     double *cur = ps->c0, *nxt = ps->c1;
     for (int r = 0; r < ps->rounds; ++r) {
-        k_part_scan<KQ, NT><<<dim3(ps->G * kPartSplit, chunks), kPartW, 0, st>>>(
+        k_part_scan<W, KQ, NT><<<dim3(ps->G * kPartSplit, chunks), W, 0, st>>>(
             cur, nxt, ps->bm + ps->m_off[(size_t)r] * mat, 1 << r, k);
         std::swap(cur, nxt);
     }
     return cur;
 }
 
-template <int KQ>
+template <int W, int KQ>
 double *launch_part_carries(cap_plan p, int k, hipStream_t st)
 {
-    return k == KQ ? part_carries<KQ, true>(p, k, st) : part_carries<KQ, false>(p, k, st);
+    return k == KQ ? part_carries<W, KQ, true>(p, k, st) : part_carries<W, KQ, false>(p, k, st);
 }
 
 // The fix-up of every tail and the finish, KQ right-hand sides per
 // workgroup (KQ | k); `tg`: the scanned group carries.
-template <int KQ, typename T>
+template <int W, int KQ, typename T>
 void launch_part_fix(cap_plan p, const double *tg, T *x, int k, hipStream_t st)
 {
     const PartState *ps = p->part;
     const unsigned chunks = (unsigned)(k / KQ);
-    const long long mat = (long long)kPartW * kPartW;
-    if (chunks == 1)  // non-temporal matrix reads when each W is read once
-        k_part_fixup<KQ, true><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
-                                                                      ps->t, k);
+    const long long mat = (long long)W * W;
+    if (chunks == 1)  // non-temporal matrix reads when each W_p is read once
+        k_part_fixup<W, KQ, true><<<dim3(ps->Q, chunks), W, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
+                                                                     ps->t, k);
     else
-        k_part_fixup<KQ, false><<<dim3(ps->Q, chunks), kPartW, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
-                                                                       ps->t, k);
+        k_part_fixup<W, KQ, false><<<dim3(ps->Q, chunks), W, 0, st>>>(ps->u, tg, ps->bm + ps->w_off * mat, ps->gs,
+                                                                      ps->t, k);
     if constexpr (KQ == 1)
-        k_part_finish<T><<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
-    else {  // m >= kPartW: a workgroup's kPartW rows span at most two blocks
-        const dim3 grid((unsigned)((p->n + kPartW - 1) / kPartW), chunks);
+        k_part_finish<W, T><<<blocks_for((long long)p->n * k), kBlock, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k,
+                                                                               x);
+    else {  // m >= W: a workgroup's W rows span at most two blocks
+        const dim3 grid((unsigned)((p->n + W - 1) / W), chunks);
         if (chunks == 1)
-            k_part_finish_multi<KQ, T, true><<<grid, kPartW, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+            k_part_finish_multi<W, KQ, T, true><<<grid, W, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
         else
-            k_part_finish_multi<KQ, T, false><<<grid, kPartW, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
+            k_part_finish_multi<W, KQ, T, false><<<grid, W, 0, st>>>(ps->y, ps->vt, ps->t, p->n, ps->m, k, x);
     }
 }
 
@@ -560,37 +568,56 @@ void launch_part_fix(cap_plan p, const double *tg, T *x, int k, hipStream_t st)
 // over the group carries, then the fix-up; grids over tails x chunks), then
 // x = y - V t.  Every step is on `st`; none of them waits on another
 // workgroup.  Each right-hand side gets the bits of its own one-RHS solve.
-template <typename T>
-int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
+template <int W, typename T>
+int launch_partition_w(cap_plan p, const T *b, T *x, int k, hipStream_t st)
 {
-    PartState *ps = p->part;
-    if (!ps || k < 1 || k > kWave || ps->m < kPartW) return CAP_ERR_ARG;
-    CAP_TRY(ensure_part_rhs(p, k));
     // right-hand sides per wavefront of the block solve: the plan's tuned
     // pick for k when it has one, else the largest power of two up to
     // min(4, k / 2) that divides k
     int kr = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
     if (p->part_kr > 0 && p->part_kr <= kPartBlockRhs && k % p->part_kr == 0) kr = p->part_kr;
     switch (kr) {
-    case 4: launch_part_block<4, T>(p, b, k, st); break;
-    case 2: launch_part_block<2, T>(p, b, k, st); break;
-    default: launch_part_block<1, T>(p, b, k, st); break;
+    case 4: launch_part_block<W, 4, T>(p, b, k, st); break;
+    case 2: launch_part_block<W, 2, T>(p, b, k, st); break;
+    default: launch_part_block<W, 1, T>(p, b, k, st); break;
     }
     const double *tg = nullptr;
     switch (part_chunk(k, kPartScanRhs)) {
-    case 4: tg = launch_part_carries<4>(p, k, st); break;
-    case 2: tg = launch_part_carries<2>(p, k, st); break;
-    default: tg = launch_part_carries<1>(p, k, st); break;
+    case 4: tg = launch_part_carries<W, 4>(p, k, st); break;
+    case 2: tg = launch_part_carries<W, 2>(p, k, st); break;
+    default: tg = launch_part_carries<W, 1>(p, k, st); break;
     }
-    switch (part_chunk(k, 16)) {
-    case 16: launch_part_fix<16, T>(p, tg, x, k, st); break;
-    case 8: launch_part_fix<8, T>(p, tg, x, k, st); break;
-    case 4: launch_part_fix<4, T>(p, tg, x, k, st); break;
-    case 2: launch_part_fix<2, T>(p, tg, x, k, st); break;
-    default: launch_part_fix<1, T>(p, tg, x, k, st); break;
+    switch (part_chunk(k, W == kPartW ? 16 : 8)) {
+    case 16:
+        if constexpr (W == kPartW) launch_part_fix<W, 16, T>(p, tg, x, k, st);
+        break;
+    case 8: launch_part_fix<W, 8, T>(p, tg, x, k, st); break;
+    case 4: launch_part_fix<W, 4, T>(p, tg, x, k, st); break;
+    case 2: launch_part_fix<W, 2, T>(p, tg, x, k, st); break;
+    default: launch_part_fix<W, 1, T>(p, tg, x, k, st); break;
     }
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
+}
+
+template <typename T>
+int launch_partition(cap_plan p, const T *b, T *x, int k, hipStream_t st)
+{
+    PartState *ps = p->part;
+    if (!ps || k < 1 || k > kWave || ps->m < ps->W) return CAP_ERR_ARG;
+    CAP_TRY(ensure_part_rhs(p, k));
+    if (ps->W == kPartWMax) return launch_partition_w<kPartWMax, T>(p, b, x, k, st);
+    return launch_partition_w<kPartW, T>(p, b, x, k, st);
+}
+
+// Threads per workgroup of the fused refinement sweep for rhs right-hand
+// sides: the plan's sweep size (128 or 256), or 1024 when the plan's solve
+// uses 1024-thread workgroups for several right-hand sides (a batch's pick;
+// tickets then hold 4 x the rows).
+inline int refine_block(cap_plan p, int rhs)
+{
+    if (rhs > 1 && p->rblock == kBlockLarge && !p->lds) return kBlockLarge;
+    return (p->rblock ? p->rblock : p->wblock) == kBlockSmall ? kBlockSmall : kBlock;
 }
 
 // A batch's block schedule for blocks of `rows` rows (nullptr if none was
@@ -983,13 +1010,13 @@ void apply_choice(cap_plan p, const TuneChoice &c)
 }
 
 // Tables of the partitioned banded solve (CAP_ALGO_PARTITION), built once:
-// FP64 plans whose lower bandwidth is at most kPartW and that split into at
-// least two blocks; CAP_ERR_UNSUPPORTED otherwise.  Blocks hold
-// m = max(kPartW, ceil(n / kPartBlocks)) rows.  The host lays out the
-// block-diagonal part (ELL slabs + overflow CSR + reciprocal diagonal); V = D^{-1} C
-// comes from two 64-RHS solves of a batch over the blocks (dropped
-// afterwards) and is stored column-major; the scan's matrices are formed
-// round by round.
+// plans whose lower bandwidth is at most kPartWMax and that split into at
+// least two blocks; CAP_ERR_UNSUPPORTED otherwise.  The tail width W is 128
+// when the bandwidth allows, else 256; blocks hold m = max(W, ceil(n /
+// kPartBlocks)) rows.  The host lays out the block-diagonal part (ELL slabs
+// + overflow CSR + reciprocal diagonal); V = D^{-1} C comes from W / 64
+// 64-RHS solves of a batch over the blocks (dropped afterwards) and is
+// stored column-major; the scan's matrices are formed round by round.
 constexpr int kPartBlocks = 1024;
 // largest relative max-norm difference from the sequential solve accepted
 // when the tables are checked at plan creation (the FP64 bar is 1e-12)
@@ -1016,15 +1043,16 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
 {
     if (p->part) return CAP_OK;
     const int n = p->n, nnz = p->nnz;
-    const int m = std::max(kPartW, (n + kPartBlocks - 1) / kPartBlocks);
-    const int P = n > 0 ? (n + m - 1) / m : 0;
-    if (P < 2) return CAP_ERR_UNSUPPORTED;
     std::vector<int> ci((size_t)nnz);
     CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
     int w = 0;
     for (int i = 0; i < n; ++i)
         for (int j = rp_host[i]; j < rp_host[i + 1]; ++j) w = std::max(w, i - ci[j]);
-    if (w > kPartW) return CAP_ERR_UNSUPPORTED;
+    if (w > kPartWMax) return CAP_ERR_UNSUPPORTED;
+    const int W = w <= kPartW ? kPartW : kPartWMax, ring = 2 * W;
+    const int m = std::max(W, (n + kPartBlocks - 1) / kPartBlocks);
+    const int P = n > 0 ? (n + m - 1) / m : 0;
+    if (P < 2) return CAP_ERR_UNSUPPORTED;
     // ELL slab width: the longest row's in-block off-diagonals in columns
     // <= i - 3 (columns i-1 and i-2 go to the near table), rounded up to 8,
     // at most kPartEll (the rest of a longer row goes to the overflow)
@@ -1046,8 +1074,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     for (int r = 0; r < rounds; ++r) m_off[(size_t)r + 1] = m_off[(size_t)r] + (G - (1 << r));
     const long long n_mat = m_off[(size_t)rounds];
     {
-        const double bytes = (double)n * (12.0 * ew + 8.0 * kPartW + 40.0) + (double)n_mat * kPartW * kPartW * 8.0 +
-                             12.0 * nnz;
+        const double bytes = (double)n * (12.0 * ew + 8.0 * W + 40.0) + (double)n_mat * W * W * 8.0 + 12.0 * nnz;
         if (bytes > kPartMaxBytes) return CAP_ERR_UNSUPPORTED;
     }
     std::vector<double> v((size_t)nnz);  // FP32 values widen exactly
@@ -1064,7 +1091,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     // column's slot in the block kernel's LDS ring; pads point at the ring's
     // zero slot), the rest in the overflow CSR (absolute columns); and, for
     // the coupling solves, all of them as a batch with local column indices
-    std::vector<int> ell_c((size_t)n * ew, kPartRing * 8), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1),
+    std::vector<int> ell_c((size_t)n * ew, ring * 8), ov_rp((size_t)n + 1), ov_c, ro((size_t)P + 1),
         rpd((size_t)n + 1), cid;
     std::vector<double> ell_v((size_t)n * ew, 0.0), ov_v, dg((size_t)n), vd, nr((size_t)n * 2, 0.0);
     cid.reserve((size_t)nnz);
@@ -1084,7 +1111,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
                 continue;
             }
             if (used < ew) {
-                ell_c[(size_t)i * ew + used] = ((ci[j] - start) & (kPartRing - 1)) * 8;  // ring slot byte offset
+                ell_c[(size_t)i * ew + used] = ((ci[j] - start) & (ring - 1)) * 8;  // ring slot byte offset
                 ell_v[(size_t)i * ew + used] = v[j];
             } else {
                 ov_c.push_back(ci[j]);
@@ -1100,6 +1127,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     rpd[(size_t)n] = (int)cid.size();
     PartState *ps = new PartState();
     p->part = ps;  // freed by free_partition on every failure from here on
+    ps->W = W;
     ps->m = m;
     ps->ew = ew;
     ps->P = P;
@@ -1154,9 +1182,9 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     o.wait_limit_ms = p->wait_ticks > 0 ? p->wait_ticks / 100000 : -1;
     o.readiness = CAP_READY_IN_PLACE;  // plan-time only: keep the uncached pool at the solves' size
     cap_batch b64 = nullptr;
-    double *r0 = nullptr, *r1 = nullptr, *v0 = nullptr, *v1 = nullptr;
-    const size_t half = (size_t)n * 64;
-    const long long mat = (long long)kPartW * kPartW;
+    double *rr = nullptr, *vv = nullptr;  // W / 64 slabs of n x 64 coupling columns, and their solves
+    const size_t slab = (size_t)n * 64, slabs = (size_t)W / 64;
+    const long long mat = (long long)W * W;
     const size_t n_ov = std::max<size_t>(1, ov_c.size());
     int rc = CAP_OK;
     do {
@@ -1164,7 +1192,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
             (rc = dev_alloc(&ps->ov_rp, ov_rp.size())) || (rc = dev_alloc(&ps->ov_c, n_ov)) ||
             (rc = dev_alloc(&ps->ov_v, n_ov)) || (rc = dev_alloc(&ps->diag, (size_t)n)) ||
             (rc = dev_alloc(&ps->near, (size_t)n * 2)) ||
-            (rc = dev_alloc(&ps->vt, (size_t)n * kPartW)) || (rc = ensure_part_rhs(p, 1)))
+            (rc = dev_alloc(&ps->vt, (size_t)n * W)) || (rc = ensure_part_rhs(p, 1)))
             break;
         if (n_mat > 0 && (rc = dev_alloc(&ps->bm, (size_t)(n_mat * mat)))) break;
         if (!all_ops.empty() && (rc = dev_alloc(&d_ops, all_ops.size()))) break;
@@ -1179,44 +1207,47 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
             rc = CAP_ERR_HIP;
             break;
         }
-        // V = D^{-1} C: the coupling columns as two 64-RHS solves of the blocks
+        // V = D^{-1} C: the coupling columns as W / 64 64-RHS solves of the blocks
         if ((rc = cap_batch_create(&b64, P, ro.data(), rpd.data(), cid.data(), vd.data(), CAP_F64, 64, &o, st)))
             break;
-        if ((rc = dev_alloc(&r0, half)) || (rc = dev_alloc(&r1, half)) || (rc = dev_alloc(&v0, half)) ||
-            (rc = dev_alloc(&v1, half)))
-            break;
-        if (hipMemsetAsync(r0, 0, half * sizeof(double), st) != hipSuccess ||
-            hipMemsetAsync(r1, 0, half * sizeof(double), st) != hipSuccess) {
+        if ((rc = dev_alloc(&rr, slab * slabs)) || (rc = dev_alloc(&vv, slab * slabs))) break;
+        if (hipMemsetAsync(rr, 0, slab * slabs * sizeof(double), st) != hipSuccess) {
             rc = CAP_ERR_HIP;
             break;
         }
         if (p->vt == CAP_F64)
-            k_part_coupling<double><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, n, m, r0, r1);
+            k_part_coupling<double><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const double *)p->val, n, m, W, rr);
         else
-            k_part_coupling<float><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const float *)p->val, n, m, r0, r1);
+            k_part_coupling<float><<<blocks_for(n), kBlock, 0, st>>>(p->rp, p->ci, (const float *)p->val, n, m, W, rr);
         if (hipGetLastError() != hipSuccess) {
             rc = CAP_ERR_HIP;
             break;
         }
-        if ((rc = cap_batch_solve(b64, r0, v0, st)) || (rc = cap_batch_solve(b64, r1, v1, st))) break;
-        if ((rc = cap_plan_status(b64->plan, st)) != CAP_OK) break;
-        k_part_transpose<<<blocks_for((long long)n * kPartW), kBlock, 0, st>>>(v0, v1, n, ps->vt);
+        for (size_t h = 0; h < slabs && rc == CAP_OK; ++h) rc = cap_batch_solve(b64, rr + h * slab, vv + h * slab, st);
+        if (rc != CAP_OK || (rc = cap_plan_status(b64->plan, st)) != CAP_OK) break;
+        k_part_transpose<<<blocks_for((long long)n * W), kBlock, 0, st>>>(vv, n, W, ps->vt);
         if (Q > 1)
-            k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->vt, n, m, Q, ps->bm);
+            k_part_tail_matrix<<<blocks_for((long long)(Q - 1) * mat), kBlock, 0, st>>>(ps->vt, n, m, Q, W, ps->bm);
         if (!all_ops.empty()) {
             if (hipMemcpy(d_ops, all_ops.data(), sizeof(int3) * all_ops.size(), hipMemcpyHostToDevice) !=
                 hipSuccess) {
                 rc = CAP_ERR_HIP;
                 break;
             }
-            for (size_t l = 0; l < launches.size(); ++l)
-                k_part_gemm<<<(unsigned)launches[l].size(), 256, 0, st>>>(d_ops + op_start[l], ps->bm);
+            const unsigned quads = (unsigned)((W / 128) * (W / 128));
+            for (size_t l = 0; l < launches.size(); ++l) {
+                const dim3 grid((unsigned)launches[l].size(), quads);
+                if (W == kPartWMax)
+                    k_part_gemm<kPartWMax><<<grid, 256, 0, st>>>(d_ops + op_start[l], ps->bm);
+                else
+                    k_part_gemm<kPartW><<<grid, 256, 0, st>>>(d_ops + op_start[l], ps->bm);
+            }
         }
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = CAP_ERR_HIP;
     } while (0);
     (void)hipStreamSynchronize(st);
     if (b64) cap_batch_destroy(b64);
-    for (double *t : {r0, r1, v0, v1})
+    for (double *t : {rr, vv})
         if (t) (void)hipFree(t);
     if (d_ops) (void)hipFree(d_ops);
     // Accuracy check against the sequential solve.  The block solves and V
@@ -1872,8 +1903,9 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
     CAP_HIP_TRY(hipGetLastError());
     if (p->n == 0) return CAP_OK;
     const int l = lpr_log2_for(rhs);
-    // the sweep's own sizes (a 1024-thread multi-RHS pick is the solve's only)
-    const int block = (p->rblock ? p->rblock : p->wblock) == kBlockSmall ? kBlockSmall : kBlock;
+    // the sweep's own sizes: 128 or 256 threads, or 1024 for a batch that
+    // picked it for several right-hand sides (no LDS window)
+    const int block = refine_block(p, rhs);
     const int rows_per_block = block >> l;
     int n_sched = 0;
     const int2 *sched = sched_for(p, rows_per_block, &n_sched);
@@ -1886,7 +1918,9 @@ int refine_fused(cap_plan p, const double *b, double *x, int rhs, hipStream_t st
                                      p->norms);
     };
     const bool small = block == kBlockSmall;
-    if (p->lds)
+    if (block == kBlockLarge)
+        go(k_refine2<0, kBlockLarge>);
+    else if (p->lds)
         small ? go(k_refine2<1, kBlockSmall>) : go(k_refine2<1, kBlock>);
     else
         small ? go(k_refine2<0, kBlockSmall>) : go(k_refine2<0, kBlock>);
@@ -2085,9 +2119,6 @@ extern "C" int cap_plan_solve_refined(cap_plan p, const double *b, double *x, in
 // ---------------------------------------------------------------- batches
 namespace {
 
-// Block schedule of a batch: every matrix is cut into blocks of `rows`
-// consecutive rows, and a stable sort by key interleaves the matrices so
-// that all of them advance in one launch instead of one after another.
 // Row levels of a batch's block-diagonal matrix (host copy).
 int batch_levels(cap_plan p, std::vector<int> *lv)
 {
@@ -2100,11 +2131,17 @@ int batch_levels(cap_plan p, std::vector<int> *lv)
     return CAP_OK;
 }
 
-// Builds (once per size) the block schedule for blocks of `rows` rows.
+// Block schedule of a batch: every matrix is cut into blocks of `rows`
+// consecutive rows, and a stable sort by key interleaves the matrices so
+// that all of them advance in one launch instead of one after another.
+// Built once per size into `into` (default: the plan's schedules); matrices
+// with skip[m] set get no blocks (a hybrid batch solves them apart).
 int build_batch_schedule(cap_plan p, const std::vector<int> &ro, const std::vector<int> &lv, int rows,
-                         hipStream_t st)
+                         hipStream_t st, const std::vector<char> *skip = nullptr,
+                         std::map<int, std::pair<int2 *, int>> *into = nullptr)
 {
-    if (p->scheds.count(rows)) return CAP_OK;
+    if (!into) into = &p->scheds;
+    if (into->count(rows)) return CAP_OK;
     const int n = p->n, n_mat = (int)ro.size() - 1;
     struct Blk {
         double key;
@@ -2119,6 +2156,7 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, const std::vect
     std::vector<Blk> blks;
     blks.reserve((size_t)n / rows + n_mat + 1);
     for (int m = 0; m < n_mat; ++m) {
+        if (skip && (*skip)[(size_t)m]) continue;
         int depth = 1;
         for (int r = ro[m]; r < ro[m + 1]; ++r) depth = std::max(depth, lv[r] + 1);
         int run = 0;
@@ -2134,11 +2172,20 @@ int build_batch_schedule(cap_plan p, const std::vector<int> &ro, const std::vect
     for (size_t t = 0; t < blks.size(); ++t) h[t] = make_int2(blks[t].lo, blks[t].len);
     int2 *d = nullptr;
     CAP_TRY(dev_alloc(&d, std::max<size_t>(h.size(), 1)));
-    p->scheds[rows] = {d, (int)h.size()};
+    (*into)[rows] = {d, (int)h.size()};
     CAP_HIP_TRY(hipMemcpyAsync(d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice, st));
     CAP_HIP_TRY(hipStreamSynchronize(st));
     return CAP_OK;
 }
+
+// Rows per block of the thread-per-row solve and of the fused refinement
+// sweep of a batch plan with k right-hand sides (their schedules' sizes).
+int batch_solve_rows(cap_plan p, int k)
+{
+    if (p->wblock == kBlockLarge && p->rpl > 1) return kBlockLarge >> lpr_log2_for((k + p->rpl - 1) / p->rpl);
+    return p->wblock >> lpr_log2_for(k);
+}
+int batch_refine_rows(cap_plan p, int k) { return refine_block(p, k) >> lpr_log2_for(k); }
 
 // A batch runs either the scheduled thread-per-row kernel over all matrices
 // or one workgroup per matrix; unless the caller named one, both are timed
@@ -2205,6 +2252,8 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std
             p->bpc = 1;
             p->wblock = kBlockLarge;
             p->rpl = best_r;
+            p->rblock = kBlockLarge;  // the refinement sweep's tickets too
+            rc = build_batch_schedule(p, ro, lv, batch_refine_rows(p, k), st);
         }
     }
     if (rc == CAP_OK && requested == CAP_ALGO_AUTO)
@@ -2215,6 +2264,186 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std
     if (rc != CAP_OK) return rc;
     p->algo = t_l < t_w ? CAP_ALGO_LOCAL : CAP_ALGO_WRTFST;
     return CAP_OK;
+}
+
+// One solve of a batch: the partitioned matrices first (each writes only its
+// own rows; a refinement's residual over all rows then sees their final
+// values), then the thread-per-row or one-workgroup solve of the rest, all
+// on st.  (On a second stream, overlapping the rest's persistent kernel,
+// the partitioned solves slowed its chain more than they saved: 256 x 16
+// batch 11.5 against 10.9 ms.)
+int batch_solve_parts(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k,
+                      const void *b, void *x, hipStream_t st)
+{
+    const size_t row_bytes = (size_t)k * p->vbytes;
+    for (size_t h = 0; h < parts.size(); ++h)
+        CAP_TRY(plan_solve_with(parts[h], parts[h]->algo, parts[h]->sync, (const char *)b + lo[h] * row_bytes,
+                                (char *)x + lo[h] * row_bytes, k, st));
+    return plan_solve_with(p, p->algo, p->sync, b, x, k, st);
+}
+
+float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k, const void *b,
+                 void *x, hipStream_t st, int *rc)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best = 1e30f;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        *rc = CAP_ERR_HIP;
+    } else {
+        for (int r = 0; r < 3 && *rc == CAP_OK; ++r) {
+            (void)hipEventRecord(e0, st);
+            *rc = batch_solve_parts(p, parts, lo, k, b, x, st);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            if (r > 0) best = std::min(best, t);  // the first run is a warm-up
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return best;
+}
+
+// Hybrid batch: a deep matrix (few rows per level) sets the whole batch's
+// time with its chain of levels, and a partitioned solve shortens exactly
+// that.  Matrices are taken deepest first while each is deep and its
+// partitioned plan builds (lower bandwidth <= 256, blocks stable); the first
+// matrix that is not stops the list, since the rest's time is the deepest
+// remaining chain.  Without its deep chains the rest is bound by how fast
+// rows are handed out (one ticket atomic per block, ~12 ns each), so its
+// thread-per-row shape is timed again: 1, 2 and 4 workgroups of 256 threads
+// per CU, and 1024-thread workgroups with 1, 2 or 4 right-hand sides per
+// lane, each over a schedule without the partitioned matrices (b = 0).  The
+// fastest hybrid is kept when 2 % faster than the plain batch as it
+// resolved (thread-per-row or one workgroup per matrix).  The kept plans go
+// to *parts.  Plans in the reference arithmetic or with readiness in place
+// (the rest's reset fills x) are left as they are.
+constexpr int kHybridMax = 8;           // partitioned matrices per batch, at most
+constexpr int kHybridMinLevels = 1024;  // deep enough for a chain to matter
+
+int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int> &ro, const std::vector<int> &lv,
+                        hipStream_t st, std::vector<cap_plan> *parts, std::vector<int> *lo)
+{
+    if (p->unfused || !p->staged) return CAP_OK;
+    const int n_mat = (int)ro.size() - 1;
+    std::vector<int> depth((size_t)n_mat, 1), order((size_t)n_mat);
+    for (int m = 0; m < n_mat; ++m) {
+        for (int r = ro[m]; r < ro[m + 1]; ++r) depth[(size_t)m] = std::max(depth[(size_t)m], lv[r] + 1);
+        order[(size_t)m] = m;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return depth[(size_t)a] > depth[(size_t)b]; });
+    // the deep matrices' partitioned plans, from host copies of their rows
+    std::vector<cap_plan> cand;
+    std::vector<int> cand_lo;
+    std::vector<char> skip((size_t)n_mat, 0);
+    int rc = CAP_OK;
+    for (int m : order) {
+        const int nm = ro[m + 1] - ro[m];
+        if ((int)cand.size() >= kHybridMax || depth[(size_t)m] < kHybridMinLevels ||
+            (double)nm / depth[(size_t)m] > kLocalMaxWidth)
+            break;
+        std::vector<int> rp((size_t)nm + 1);
+        CAP_HIP_TRY(hipMemcpy(rp.data(), p->rp + ro[m], sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+        const int e0 = rp[0], nnz = rp[(size_t)nm] - e0;
+        std::vector<int> ci((size_t)nnz);
+        std::vector<char> val((size_t)nnz * p->vbytes);
+        CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci + e0, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+        CAP_HIP_TRY(hipMemcpy(val.data(), (const char *)p->val + (size_t)e0 * p->vbytes, val.size(),
+                              hipMemcpyDeviceToHost));
+        for (int &r : rp) r -= e0;
+        for (int &c : ci) c -= ro[m];  // global columns to the matrix's own
+        cap_opts o;
+        cap_opts_default(&o);
+        o.algo = CAP_ALGO_PARTITION;
+        o.host_inputs = 1;
+        o.validate = 0;  // checked with the batch
+        o.spin_limit = p->spin_limit;
+        o.wait_limit_ms = p->wait_ticks > 0 ? p->wait_ticks / 100000 : -1;
+        cap_plan q = nullptr;
+        rc = plan_create_impl(&q, nm, nnz, rp.data(), ci.data(), val.data(), p->vt, &o, st);
+        if (rc == CAP_ERR_UNSUPPORTED) {  // band or stability: the list stops here
+            rc = CAP_OK;
+            break;
+        }
+        if (rc != CAP_OK) break;
+        cand.push_back(q);
+        cand_lo.push_back(ro[m]);
+        skip[(size_t)m] = 1;
+    }
+    if (rc != CAP_OK || cand.empty()) {
+        for (cap_plan q : cand) cap_plan_destroy(q);
+        return rc;
+    }
+    const size_t bytes = (size_t)p->n * k * p->vbytes + 8;
+    void *b = nullptr, *x = nullptr;
+    if (hipMalloc(&b, bytes) != hipSuccess || hipMalloc(&x, bytes) != hipSuccess ||
+        hipMemsetAsync(b, 0, bytes, st) != hipSuccess)
+        rc = CAP_ERR_ALLOC;
+    struct Shape {
+        int bpc, block, rpl;
+    };
+    std::vector<Shape> shapes = {{1, kBlock, 1}, {2, kBlock, 1}, {4, kBlock, 1}};
+    if (k > 1 && p->sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls)  // the multi-RHS shapes
+        for (int r : {1, 2, 4})
+            if (r <= k) shapes.push_back({1, kBlockLarge, r});
+    if (bpc_fixed) shapes = {{p->bpc, p->wblock, p->rpl}};  // the caller's grid stays
+    const Shape plain{p->bpc, p->wblock, p->rpl};
+    const int plain_algo = p->algo;
+    Shape pick{0, 0, 0};
+    float best = 1e30f;
+    std::map<int, std::pair<int2 *, int>> trial;  // the rest's schedules, by rows per block
+    if (rc == CAP_OK) best = time_batch(p, {}, {}, k, b, x, st, &rc);
+    if (rc == CAP_OK) {
+        float best_h = 1e30f;
+        std::swap(p->scheds, trial);  // the hybrid's rest runs over the trial schedules
+        p->algo = CAP_ALGO_WRTFST;
+        for (const Shape &sh : shapes) {
+            p->bpc = sh.bpc;
+            p->wblock = sh.block;
+            p->rpl = sh.rpl;
+            if ((rc = build_batch_schedule(p, ro, lv, batch_solve_rows(p, k), st, &skip)) != CAP_OK) break;
+            const float t = time_batch(p, cand, cand_lo, k, b, x, st, &rc);
+            if (rc != CAP_OK) break;
+            if (t < best_h) {
+                best_h = t;
+                pick = sh;
+            }
+        }
+        std::swap(p->scheds, trial);
+        if (rc == CAP_OK && best_h < 0.98f * best) {
+            p->algo = CAP_ALGO_WRTFST;
+            p->bpc = pick.bpc;
+            p->wblock = pick.block;
+            p->rpl = pick.rpl;
+            // the refinement sweep follows the solve's ticket size
+            p->rblock = pick.block == kBlockLarge ? kBlockLarge : (p->rblock == kBlockLarge ? kBlock : p->rblock);
+        } else {
+            p->algo = plain_algo;
+            p->bpc = plain.bpc;
+            p->wblock = plain.block;
+            p->rpl = plain.rpl;
+            pick.block = 0;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(b);
+    (void)hipFree(x);
+    for (auto &e : trial) (void)hipFree(e.second.first);
+    if (rc == CAP_OK && pick.block) {
+        // the kept matrices leave every schedule the plan solves with
+        for (auto &e : p->scheds) (void)hipFree(e.second.first);
+        p->scheds.clear();
+        if ((rc = build_batch_schedule(p, ro, lv, batch_solve_rows(p, k), st, &skip)) == CAP_OK)
+            rc = build_batch_schedule(p, ro, lv, batch_refine_rows(p, k), st, &skip);
+    }
+    if (rc == CAP_OK && pick.block) {
+        *parts = cand;
+        *lo = cand_lo;
+    } else {
+        for (cap_plan q : cand) cap_plan_destroy(q);
+    }
+    return rc;
 }
 
 }  // namespace
@@ -2298,10 +2527,19 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
         cap_plan_destroy(p);
         return rc;
     }
+    std::vector<cap_plan> parts;
+    std::vector<int> part_lo;
+    if (o.algo == CAP_ALGO_AUTO &&
+        (rc = choose_batch_hybrid(p, k, o.blocks_per_cu > 0, ro, lv, st, &parts, &part_lo)) != CAP_OK) {
+        cap_plan_destroy(p);
+        return rc;
+    }
     cap_batch bt = new cap_batch_s();
     bt->plan = p;
     bt->k = k;
     bt->n_mat = n_mat;
+    bt->parts = parts;
+    bt->part_lo = part_lo;
     *out = bt;
     return CAP_OK;
 }
@@ -2314,30 +2552,64 @@ int cap_batch_info(cap_batch bt, int *algo, int *blocks_per_cu)
     return CAP_OK;
 }
 
-int cap_batch_tuning(cap_batch bt, int *row_threads, int *rhs_per_lane)
+int cap_batch_tuning(cap_batch bt, int *row_threads, int *rhs_per_lane, int *partitioned)
 {
     if (!bt) return CAP_ERR_ARG;
     if (row_threads) *row_threads = bt->plan->wblock;
     if (rhs_per_lane) *rhs_per_lane = bt->plan->wblock == kBlockLarge ? bt->plan->rpl : 1;
+    if (partitioned) *partitioned = (int)bt->parts.size();
     return CAP_OK;
+}
+
+int cap_batch_partitioned(cap_batch bt, int *first_rows, int capacity)
+{
+    if (!bt || capacity < 0 || (capacity > 0 && !first_rows)) return CAP_ERR_ARG;
+    const int cnt = (int)bt->part_lo.size();
+    for (int h = 0; h < std::min(cnt, capacity); ++h) first_rows[h] = bt->part_lo[(size_t)h];
+    return cnt;
 }
 
 int cap_batch_solve(cap_batch bt, const void *b, void *x, void *stream)
 {
     if (!bt || !b || !x) return CAP_ERR_ARG;
-    return plan_solve_with(bt->plan, bt->plan->algo, bt->plan->sync, b, x, bt->k, S(stream));
+    return batch_solve_parts(bt->plan, bt->parts, bt->part_lo, bt->k, b, x, S(stream));
 }
 
 int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol, int max_iter, int *iters,
                             double *last_rel, void *stream)
 {
-    if (!bt) return CAP_ERR_ARG;
-    return refined_impl(bt->plan, b, x, bt->k, tol, max_iter, iters, last_rel, S(stream));
+    if (!bt || !b || !x) return CAP_ERR_ARG;
+    // the partitioned matrices first (FP64 partitioned solves and
+    // corrections), then the rest: iterations and correction are the largest
+    int it_max = 0;
+    double rel_max = 0.0;
+    int rc = CAP_OK;
+    const size_t row = (size_t)bt->k;
+    for (size_t h = 0; h < bt->parts.size() && rc == CAP_OK; ++h) {
+        int it = 0;
+        double rel = 0.0;
+        rc = refined_impl(bt->parts[h], b + bt->part_lo[h] * row, x + bt->part_lo[h] * row, bt->k, tol, max_iter,
+                          &it, &rel, S(stream));
+        it_max = std::max(it_max, it);
+        rel_max = std::max(rel_max, rel);
+    }
+    if (rc == CAP_OK || rc == CAP_ERR_NOT_CONVERGED) {
+        int it = 0;
+        double rel = 0.0;
+        const int rc2 = refined_impl(bt->plan, b, x, bt->k, tol, max_iter, &it, &rel, S(stream));
+        it_max = std::max(it_max, it);
+        rel_max = std::max(rel_max, rel);
+        if (rc == CAP_OK) rc = rc2;
+    }
+    if (iters) *iters = it_max;
+    if (last_rel) *last_rel = rel_max;
+    return rc;
 }
 
 int cap_batch_destroy(cap_batch bt)
 {
     if (!bt) return CAP_ERR_ARG;
+    for (cap_plan q : bt->parts) cap_plan_destroy(q);
     cap_plan_destroy(bt->plan);
     delete bt;
     return CAP_OK;

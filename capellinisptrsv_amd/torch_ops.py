@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes
 from ctypes import byref, c_double, c_int, c_void_p
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -202,9 +203,16 @@ class Batch:
         N.check(N.lib().cap_batch_info(h, byref(a), byref(bpc)), "cap_batch_info")
         self.algo = N.ALGO_NAME[a.value]
         self.blocks_per_cu = bpc.value
-        thr, rpl = c_int(), c_int()
-        N.check(N.lib().cap_batch_tuning(h, byref(thr), byref(rpl)), "cap_batch_tuning")
-        self.row_threads, self.rhs_per_lane = thr.value, rpl.value
+        thr, rpl, npart = c_int(), c_int(), c_int()
+        N.check(N.lib().cap_batch_tuning(h, byref(thr), byref(rpl), byref(npart)), "cap_batch_tuning")
+        self.row_threads, self.rhs_per_lane, self.partitioned = thr.value, rpl.value, npart.value
+        rows = (c_int * max(1, npart.value))()
+        got = N.lib().cap_batch_partitioned(h, rows, npart.value)
+        if got < 0:
+            N.check(got, "cap_batch_partitioned")
+        off = row_off.cpu().numpy() if row_off.is_cuda else row_off.numpy()
+        # the matrices solved by partitioned plans (hybrid batch), by index
+        self.partitioned_matrices = sorted(int(np.searchsorted(off, r, side="right") - 1) for r in rows[:got])
 
     def solve(self, b: torch.Tensor, out=None, stream=None) -> torch.Tensor:
         b = b.contiguous()

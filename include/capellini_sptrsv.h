@@ -59,9 +59,10 @@ extern "C" {
 #define CAP_ALGO_LOCAL 5   /* one workgroup per matrix, wavefront per row, dependencies
                               resolved inside one compute unit: deep matrices
                               (few rows per level); value protocol on x itself */
-#define CAP_ALGO_PARTITION 6 /* lower bandwidth <= 128: row blocks solved
-                              independently, joined by a scan over the 128-value
-                              block tails (FP64 tables built at plan creation;
+#define CAP_ALGO_PARTITION 6 /* lower bandwidth <= 256: row blocks solved
+                              independently, joined by a scan over the block
+                              tails of W = 128 values (bandwidth <= 128) or
+                              256 (FP64 tables built at plan creation;
                               FP32 plans widen b and round x once); sums are
                               reassociated (within rounding of the sequential
                               solve, not bit-exact).  Up to 64
@@ -247,7 +248,16 @@ int cap_plan_solve_refined(cap_plan plan64, const double *b, double *x, int rhs,
  * faster kept).  opts->blocks_per_cu = 0 lets creation time 1, 2 and 4
  * workgroups per compute unit for the thread-per-row schedule, and, with
  * k > 1 (value protocol, no LDS window), 1024-thread workgroups with 1, 2
- * or 4 right-hand sides per lane, each with a schedule of its block size. */
+ * or 4 right-hand sides per lane, each with a schedule of its block size.
+ * With CAP_ALGO_AUTO a batch may also go hybrid: its deepest matrices
+ * (at least 1024 levels, at most 256 rows per level, taken deepest first
+ * while their partitioned plans build: lower bandwidth <= 256, stable
+ * blocks) are solved by partitioned plans of their own, and the rest by the
+ * thread-per-row schedule without them; each prefix of that list is timed
+ * at creation against the plain batch and kept only when faster.  Those
+ * matrices' results are then within rounding of the sequential solve
+ * (reassociated sums, <= 1e-12), not bit-exact.  Not with the reference
+ * arithmetic or readiness in place. */
 typedef struct cap_batch_s *cap_batch;
 int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
                      const int *row_ptr, const int *col_idx, const void *val,
@@ -256,8 +266,14 @@ int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
  * and its persistent grid size in workgroups per compute unit. */
 int cap_batch_info(cap_batch batch, int *algo, int *blocks_per_cu);
 /* No counterpart in the reference: the thread-per-row solve's workgroup size
- * (128, 256 or 1024) and right-hand sides per lane the batch resolved to. */
-int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane);
+ * (128, 256 or 1024) and right-hand sides per lane the batch resolved to,
+ * and how many of its matrices it solves with partitioned plans of their own
+ * (the hybrid batch of CAP_ALGO_AUTO: see cap_batch_create). */
+int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane, int *partitioned);
+/* No counterpart in the reference: the first rows of the matrices a hybrid
+ * batch solves with partitioned plans (up to `capacity` of them); returns
+ * how many there are, or a negative CAP_ERR_* code. */
+int cap_batch_partitioned(cap_batch batch, int *first_rows, int capacity);
 int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
 int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
                             double tol, int max_iter, int *iters,

@@ -131,9 +131,13 @@ def test_batch_256x16_fp64_and_refined(cuda, batch256):
     x = bt.solve(bd).cpu().numpy().reshape(-1)
     xr, iters, rel = bt.solve_refined(bd, tol=1e-12)
     xr = xr.cpu().numpy().reshape(-1)
+    part = set(bt.partitioned_matrices)  # the hybrid batch's partitioned matrices: within 1e-12
     bt.close()
     for m, (mrp, mci, mv) in enumerate(mats):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
         ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
-        assert np.array_equal(x[lo:hi], ref), m
+        if m in part:
+            assert rel_maxnorm(x[lo:hi], ref) <= FP64_TOL, m
+        else:
+            assert np.array_equal(x[lo:hi], ref), m
         assert rel_maxnorm(xr[lo:hi], ref) <= FP64_TOL, (m, iters, rel)

@@ -1,9 +1,11 @@
 """GPU tests of the partitioned banded solve (CAP_ALGO_PARTITION).
 
 The partition reassociates the sums (block solves joined by a scan over the
-128-value block tails), so its bar is the FP64 tolerance: relative max-norm
-error <= 1e-12 against the oracle's sequential forward substitution.  Cases:
-bandwidths 1 (one row per level), 16, 100 and 128 (the limit), block counts
+block tails of W = 128 values, or 256 for bandwidths 129 to 256), so its bar
+is the FP64 tolerance: relative max-norm error <= 1e-12 against the oracle's
+sequential forward substitution.  Cases: bandwidths 1 (one row per level),
+16, 100 and 128 (the narrow tails' limit), 129, 200 and 256 (the wide
+tails, the batch's deepest matrix among them), block counts
 from 2 up, a ragged last block, duplicated entries, repeated solves (same
 bits each time), rows longer than the 64 entries of an ELL slab, several
 right-hand sides (1 to 64, interleaved; each column against its own
@@ -12,7 +14,7 @@ solves), refinement (FP64 partitioned solves and one correction, one and
 several right-hand sides), FP32 plans (FP64 tables, b widened and x rounded
 once: within the FP32 tolerance, 1e-5, of the oracle's FP32 sequential
 solve, and within FP32 rounding of the FP64 solve of the same FP32 system),
-and the refusals (bandwidth above 128, too few rows for two blocks, blocks
+and the refusals (bandwidth above 256, too few rows for two blocks, blocks
 whose inverses grow: the plan-creation check that compares the tables'
 solve of two right-hand sides with the sequential solve).
 """
@@ -57,9 +59,24 @@ def _cases():
     yield "band128_two_blocks", api.generate("band_rowcount", n=200, offdiag=10, band=128, seed=6)
     yield "dup", random_lower(4000, max_off=6, band=90, seed=7, dup=True)
     yield "dense_band128", _dense_band(3000, 128, seed=8)
+    yield "band129", api.generate("band_rowcount", n=40000, offdiag=9, band=129, seed=10)
+    yield "band200_ragged", api.generate("band_rowcount", n=90001, offdiag=12, band=200, seed=11)
+    yield "band256_two_blocks", api.generate("band_rowcount", n=600, offdiag=10, band=256, seed=12)
+    yield "dense_band256", _dense_band(3000, 256, seed=13)
+    yield "batch_deepest", _batch_deepest()
+
+
+def _batch_deepest():
+    """The deepest matrix of the 256-matrix batch workload: n = 158,094,
+    16,132 levels, lower bandwidth 129."""
+    mats = api.make_batch(256, seed=0)[0]
+    depth = [api.matrix_layer(len(rp) - 1, len(rp) - 1, len(ci), rp, ci, v)[0] for rp, ci, v in mats]
+    best = int(np.argmax(depth))
+    return mats[best]
 
 
 CASES = list(_cases())
+IDX = {c[0]: i for i, c in enumerate(CASES)}
 
 
 def _plan(cuda, rp, ci, v, **kw):
@@ -92,7 +109,10 @@ def _columns_within_tol(x, ref, k):
     assert max(errs) <= FP64_TOL, errs
 
 
-@pytest.mark.parametrize("name,mat", [CASES[0], CASES[3], CASES[6]], ids=[CASES[i][0] for i in (0, 3, 6)])
+SEVERAL = ("deep_band_20k", "band100_ragged", "dense_band128", "band200_ragged", "band256_two_blocks")
+
+
+@pytest.mark.parametrize("name,mat", [CASES[IDX[c]] for c in SEVERAL], ids=SEVERAL)
 def test_partition_several_rhs(cuda, name, mat):
     """k = 16, 64, then smaller counts (the work vectors grow, then serve a
     smaller k as they are), with chunks of 16, 8, 4, 2 and 1 right-hand sides
@@ -137,7 +157,10 @@ def test_partition_refinement(cuda, k):
     plan.close()
 
 
-@pytest.mark.parametrize("name,mat", [CASES[0], CASES[3], CASES[6]], ids=[CASES[i][0] for i in (0, 3, 6)])
+FP32_CASES = ("deep_band_20k", "band100_ragged", "dense_band128", "band129")
+
+
+@pytest.mark.parametrize("name,mat", [CASES[IDX[c]] for c in FP32_CASES], ids=FP32_CASES)
 @pytest.mark.parametrize("k", [1, 4])
 def test_partition_fp32(cuda, name, mat, k):
     import torch
@@ -162,7 +185,13 @@ def test_partition_refusals(cuda):
     from capellinisptrsv_amd import CapError
     rp, ci, v = api.generate("band_rowcount", n=5000, offdiag=8, band=300, seed=9)
     with pytest.raises(CapError):
-        _plan(cuda, rp, ci, v, algo="partition")  # bandwidth 300 > 128
+        _plan(cuda, rp, ci, v, algo="partition")  # bandwidth 300 > 256
+    rp, ci, v = _dense_band(1500, 257, seed=14)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v, algo="partition")  # bandwidth 257 > 256
+    rp, ci, v = api.generate("band_rowcount", n=250, offdiag=8, band=200, seed=9)
+    with pytest.raises(CapError):
+        _plan(cuda, rp, ci, v, algo="partition")  # one block at the wide tails' 256 rows
     rp, ci, v = api.generate("band_rowcount", n=128, offdiag=4, band=64, seed=9)
     with pytest.raises(CapError):
         _plan(cuda, rp, ci, v, algo="partition")  # one block

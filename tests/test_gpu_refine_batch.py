@@ -94,10 +94,60 @@ def test_batch_bit_exact_per_matrix(cuda, k, algo):
     if algo != "auto":
         assert bt.algo == algo
     x = bt.solve(torch.from_numpy(b.reshape(N, k)).to(cuda)).cpu().numpy().reshape(-1)
+    part = set(bt.partitioned_matrices)  # a hybrid batch's partitioned matrices: within 1e-12
     for m, (mrp, mci, mv) in enumerate(mats):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
         ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
-        assert np.array_equal(x[lo:hi], ref), m
+        if m in part:
+            assert rel_maxnorm(x[lo:hi], ref) <= 1e-12, m
+        else:
+            assert np.array_equal(x[lo:hi], ref), m
+
+
+def _hybrid_batch():
+    """Deep banded matrices (bandwidths 3, 129 and 200, thousands of levels)
+    among 20 shallow wide ones: the deep chains dominate the batch."""
+    mats = [api.generate("band", n=60000, offdiag=2, band=3, seed=81),
+            api.generate("band_rowcount", n=40000, offdiag=9, band=129, seed=82),
+            api.generate("band_rowcount", n=30000, offdiag=9, band=200, seed=83)]
+    mats += [api.generate("level", n=8000, levels=4, width=2000, offdiag=6, depth=2, seed=90 + i) for i in range(20)]
+    row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
+    nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
+    rp = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
+    return mats, row_off, rp, np.concatenate([m[1] for m in mats]).astype(np.int32), np.concatenate(
+        [m[2] for m in mats])
+
+
+@pytest.mark.parametrize("k", [1, 16])
+def test_hybrid_batch(cuda, k):
+    """An autotuned batch whose deep matrices are solved by partitioned plans
+    of their own: those within 1e-12 of their sequential solves, the others
+    bit-exact; the refined solve within 1e-12 everywhere; repeated solves give
+    the same bits."""
+    import torch
+    from capellinisptrsv_amd import Batch
+    mats, row_off, rp, ci, v = _hybrid_batch()
+    N = int(row_off[-1])
+    b = api.uniform(13, N * k)
+    t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
+    bt = Batch(*t, k=k, algo="auto")
+    assert bt.algo == "wrtfst" and bt.partitioned >= 1 and set(bt.partitioned_matrices) <= {0, 1, 2}
+    bd = torch.from_numpy(b.reshape(N, k)).to(cuda)
+    x = bt.solve(bd).cpu().numpy().reshape(-1)
+    x2 = bt.solve(bd).cpu().numpy().reshape(-1)
+    assert np.array_equal(x, x2)
+    xr, iters, rel = bt.solve_refined(bd, tol=1e-12)
+    xr = xr.cpu().numpy().reshape(-1)
+    part = set(bt.partitioned_matrices)
+    for m, (mrp, mci, mv) in enumerate(mats):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
+        if m in part:
+            assert rel_maxnorm(x[lo:hi], ref) <= 1e-12, m
+        else:
+            assert np.array_equal(x[lo:hi], ref), m
+        assert rel_maxnorm(xr[lo:hi], ref) <= 1e-12, (m, iters, rel)
+    bt.close()
 
 
 def _chain(n):

@@ -47,8 +47,8 @@ def test_triangular_kinds_match_the_oracle(cuda, kind, algo):
     assert plan.status() == 0
     x = x.cpu().numpy()
     ref = oracle.csr_solve(rp, ci, v, b, upper=upper, unit=unit, fused=True)
-    if algo == "csc" or (algo == "auto" and plan.info()["algo"] == "csc"):
-        assert rel_maxnorm(x, ref) <= 1e-12
+    if algo == "csc" or (algo == "auto" and plan.info()["algo"] in ("csc", "partition")):
+        assert rel_maxnorm(x, ref) <= 1e-12  # reassociated sums (band 200: the partition is a candidate)
     else:
         assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
     plan.close()
