@@ -54,6 +54,8 @@ def main(argv=None):
         _, row_off, rp, ci, v = api.make_batch(256, seed=0)
         k, n = 16, int(row_off[-1])
         bt = Batch(*(torch.from_numpy(t).to(dev) for t in (row_off, rp, ci, v)), k=k)
+        print(f"batched: {bt.algo}, {bt.blocks_per_cu} workgroup(s) per CU, {bt.row_threads} threads, "
+              f"{bt.rhs_per_lane} RHS per lane, partitioned matrices {bt.partitioned_matrices}", flush=True)
         b = torch.from_numpy(api.uniform(3, n * k).reshape(n, k)).to(dev)
         x = torch.empty_like(b)
         for _ in range(a.steps):
