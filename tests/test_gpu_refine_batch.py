@@ -133,6 +133,37 @@ def test_hybrid_batch(cuda, k):
     bt = Batch(*t, k=k, algo="auto")
     assert bt.algo == "wrtfst" and bt.partitioned >= 1 and set(bt.partitioned_matrices) <= {0, 1, 2}
     bd = torch.from_numpy(b.reshape(N, k)).to(cuda)
+    _check_hybrid(bt, bd, mats, row_off, b, k)
+    bt.close()
+
+
+def test_hybrid_batch_fp32(cuda):
+    """The FP32 hybrid batch: FP32 partitioned plans (FP64 tables, b widened
+    and x rounded once) within 1e-5 of the FP64 solve of the same FP32
+    system, the thread-per-row matrices bit-exact against the FP32 oracle."""
+    import torch
+    from capellinisptrsv_amd import Batch
+    mats, row_off, rp, ci, v = _hybrid_batch()
+    k = 4
+    N = int(row_off[-1])
+    b = api.uniform(14, N * k).astype(np.float32)
+    t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v.astype(np.float32))]
+    bt = Batch(*t, k=k, algo="auto")
+    assert bt.partitioned >= 1
+    x = bt.solve(torch.from_numpy(b.reshape(N, k)).to(cuda)).cpu().numpy().reshape(-1)
+    part = set(bt.partitioned_matrices)
+    for m, (mrp, mci, mv) in enumerate(mats):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        v32 = mv.astype(np.float32)
+        if m in part:
+            ref64 = oracle.csr_forward(mrp, mci, v32.astype(np.float64), b[lo:hi].astype(np.float64), rhs=k)
+            assert rel_maxnorm(x[lo:hi], ref64) <= 1e-5, m
+        else:
+            assert np.array_equal(x[lo:hi], oracle.csr_forward(mrp, mci, v32, b[lo:hi], rhs=k, fused=True)), m
+    bt.close()
+
+
+def _check_hybrid(bt, bd, mats, row_off, b, k):
     x = bt.solve(bd).cpu().numpy().reshape(-1)
     x2 = bt.solve(bd).cpu().numpy().reshape(-1)
     assert np.array_equal(x, x2)
@@ -147,7 +178,6 @@ def test_hybrid_batch(cuda, k):
         else:
             assert np.array_equal(x[lo:hi], ref), m
         assert rel_maxnorm(xr[lo:hi], ref) <= 1e-12, (m, iters, rel)
-    bt.close()
 
 
 def _chain(n):
