@@ -71,7 +71,7 @@ def main(argv=None):
     n = len(rp) - 1
     d = [torch.from_numpy(x).to(dev) for x in (rp, ci, v)]
     b = torch.from_numpy(api.uniform(1, n)).to(dev)
-    if a.target == "deep" and a.rhs > 1:
+    if a.target in ("deep", "solve") and a.rhs > 1:
         b = torch.from_numpy(api.uniform(1, n * a.rhs).reshape(n, a.rhs)).to(dev)
     x = torch.empty_like(b)
     plan = Plan(*d, algo=a.algo) if a.target in ("solve", "deep", "refine", "band") else \
@@ -87,7 +87,9 @@ def main(argv=None):
         print(f"refined: iterations={it} last correction={rel:.3e}", flush=True)
     torch.cuda.synchronize()
     assert plan.status() == 0
-    print(f"{a.target}: n={n} nnz={len(ci)} plan={plan.info()} steps={a.steps}", flush=True)
+    pick = plan.rhs_block(a.rhs) if a.rhs > 1 else None
+    print(f"{a.target}: n={n} nnz={len(ci)} plan={plan.info()} rhs={a.rhs} rhs block={pick} steps={a.steps}",
+          flush=True)
     return 0
 
 
