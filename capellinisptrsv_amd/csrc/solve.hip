@@ -86,6 +86,7 @@ struct PartState {
 // wavefront (0: the plan's).
 struct RhsPick {
     int bpc = 0, block = 0, kr = 0, rpl = 0;
+    int staged = -1;  // readiness: 1 staged, 0 in x itself, -1 the plan's
 };
 
 struct cap_plan_s {
@@ -635,7 +636,9 @@ inline const int2 *sched_for(cap_plan p, int rows, int *n_sched)
 struct RhsScope {
     cap_plan p;
     int bpc, block, kr, rpl;
-    RhsScope(cap_plan q, int k) : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl)
+    bool staged;
+    RhsScope(cap_plan q, int k)
+        : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl), staged(q->staged)
     {
         const auto it = k > 1 ? p->rhs_pick.find(k) : p->rhs_pick.end();
         if (it == p->rhs_pick.end()) {
@@ -650,6 +653,7 @@ struct RhsScope {
         if (it->second.block) p->wblock = it->second.block;
         if (it->second.kr) p->part_kr = it->second.kr;
         if (it->second.rpl) p->rpl = it->second.rpl;
+        if (it->second.staged >= 0) p->staged = it->second.staged != 0;
     }
     ~RhsScope()
     {
@@ -657,6 +661,7 @@ struct RhsScope {
         p->wblock = block;
         p->part_kr = kr;
         p->rpl = rpl;
+        p->staged = staged;
     }
 };
 
@@ -786,6 +791,7 @@ int launch_csc_reset(cap_plan p, hipStream_t st)
 template <typename T>
 int solve_t(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
 {
+    const RhsScope scope(p, rhs);  // the reset and the launch see the same RHS-block pick
     if (algo == CAP_ALGO_CSC) {
         CAP_TRY(launch_csc_reset<T>(p, st));
     } else {
@@ -797,6 +803,7 @@ int solve_t(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hip
 template <typename T>
 int reset_t(cap_plan p, int algo, int sync, void *x, int rhs, hipStream_t st)
 {
+    const RhsScope scope(p, rhs);
     if (algo == CAP_ALGO_CSC) return launch_csc_reset<T>(p, st);
     return launch_prepare<T>(p, algo, sync, x, (long long)p->n * rhs, st);
 }
@@ -993,6 +1000,21 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->wblock = blk0;
         p->part_kr = kr0;
         p->rpl = rpl0;
+    }
+    // readiness in x itself for the pick: one store per value instead of
+    // two (x and the staged copy), at k > 1 the larger share of the bytes
+    if (rc == CAP_OK && algo == CAP_ALGO_WRTFST && sync == CAP_SYNC_VALUE && p->staged && !p->count_polls) {
+        if (pick.bpc) p->bpc = pick.bpc;
+        if (pick.block) p->wblock = pick.block;
+        if (pick.rpl) p->rpl = pick.rpl;
+        p->staged = false;
+        float t = 0;
+        rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
+        p->staged = true;
+        p->bpc = bpc0;
+        p->wblock = blk0;
+        p->rpl = rpl0;
+        if (rc == CAP_OK && t < 0.98f * best) pick.staged = 0;
     }
     if (rc != CAP_OK) return rc;
     p->rhs_pick[k] = pick;
@@ -1773,7 +1795,7 @@ int cap_plan_poll_count(cap_plan p, unsigned long long *polls, void *stream)
 }
 
 int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, int *rhs_per_wave,
-                       int *rhs_per_lane)
+                       int *rhs_per_lane, int *readiness)
 {
     if (!p || k < 1) return CAP_ERR_ARG;
     const auto it = p->rhs_pick.find(k);
@@ -1783,6 +1805,10 @@ int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, 
     if (rhs_per_wave) *rhs_per_wave = it->second.kr;
     if (rhs_per_lane)
         *rhs_per_lane = algo_for(p, k) == CAP_ALGO_WRTFST ? (it->second.rpl ? it->second.rpl : p->rpl) : 0;
+    if (readiness) {
+        const bool staged = it->second.staged >= 0 ? it->second.staged != 0 : p->staged;
+        *readiness = staged ? CAP_READY_STAGED : CAP_READY_IN_PLACE;
+    }
     return CAP_OK;
 }
 

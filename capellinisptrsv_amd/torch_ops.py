@@ -106,13 +106,14 @@ class Plan:
     def rhs_block(self, k: int):
         """The RHS block the autotuner picked for k right-hand sides (after a
         solve with k): dict(blocks_per_cu, row_threads, rhs_per_wave,
-        rhs_per_lane), or None."""
-        bpc, thr, kr, rpl = c_int(), c_int(), c_int(), c_int()
-        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr), byref(rpl))
+        rhs_per_lane, readiness), or None."""
+        bpc, thr, kr, rpl, rdy = c_int(), c_int(), c_int(), c_int(), c_int()
+        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr), byref(rpl), byref(rdy))
         if rc == N.CAP_ERR_UNSUPPORTED:
             return None
         N.check(rc, "cap_plan_rhs_block")
-        return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value, rhs_per_lane=rpl.value)
+        return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value, rhs_per_lane=rpl.value,
+                    readiness=N.READINESS_NAME.get(rdy.value, "?"))
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""

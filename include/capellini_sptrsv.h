@@ -194,12 +194,15 @@ int cap_plan_poll_count(cap_plan plan, unsigned long long *polls, void *stream);
  * 1024-thread workgroups with 1, 2, 4 or (FP32) 8 right-hand sides per
  * lane (*rhs_per_lane; 0 for the others), partitioned plans 1/2/4 right-hand
  * sides per wavefront of the block solve (*rhs_per_wave; 0 for the others).
- * Every candidate gives the same bits.
+ * The thread-per-row pick is then also timed with readiness in x itself
+ * (*readiness: CAP_READY_STAGED or CAP_READY_IN_PLACE), which saves the
+ * staged copy's stores.  Every candidate gives the same bits.
  * CAP_ERR_UNSUPPORTED until a solve with that k has run.  (The reference's
  * OPT_WARP_RHS / OPT_WARP_AUTO switches, openCL/WrtFst/common.h:28-30, set
  * this by hand at compile time.) */
 int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
-                       int *row_threads, int *rhs_per_wave, int *rhs_per_lane);
+                       int *row_threads, int *rhs_per_wave, int *rhs_per_lane,
+                       int *readiness);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table.
  * (An autotuned plan that resolved to mix, two-part or CSC, which solve one
  * right-hand side, solves several with the write-first kernel in the value

@@ -416,4 +416,5 @@ def test_rhs_block_tuned_per_count(cuda, cfg, scale):
         elif algo == "wrtfst":
             assert pick["blocks_per_cu"] in (1, 2, 4) and pick["row_threads"] in (128, 256, 1024)
             assert pick["rhs_per_lane"] in (1, 2, 4, 8) and pick["rhs_per_lane"] <= k
+            assert pick["readiness"] in ("staged", "in_place")
     plan.close()
