@@ -87,6 +87,7 @@ struct PartState {
 struct RhsPick {
     int bpc = 0, block = 0, kr = 0, rpl = 0;
     int staged = -1;  // readiness: 1 staged, 0 in x itself, -1 the plan's
+    int lds = -1;     // LDS window: 0 off (the 1024-thread shapes), -1 the plan's
 };
 
 struct cap_plan_s {
@@ -635,10 +636,10 @@ inline const int2 *sched_for(cap_plan p, int rows, int *n_sched)
 // of one launch (and puts the plan's own settings back).
 struct RhsScope {
     cap_plan p;
-    int bpc, block, kr, rpl;
+    int bpc, block, kr, rpl, lds;
     bool staged;
     RhsScope(cap_plan q, int k)
-        : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl), staged(q->staged)
+        : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl), lds(q->lds), staged(q->staged)
     {
         const auto it = k > 1 ? p->rhs_pick.find(k) : p->rhs_pick.end();
         if (it == p->rhs_pick.end()) {
@@ -654,6 +655,7 @@ struct RhsScope {
         if (it->second.kr) p->part_kr = it->second.kr;
         if (it->second.rpl) p->rpl = it->second.rpl;
         if (it->second.staged >= 0) p->staged = it->second.staged != 0;
+        if (it->second.lds >= 0) p->lds = it->second.lds;
     }
     ~RhsScope()
     {
@@ -661,6 +663,7 @@ struct RhsScope {
         p->wblock = block;
         p->part_kr = kr;
         p->rpl = rpl;
+        p->lds = lds;
         p->staged = staged;
     }
 };
@@ -950,10 +953,12 @@ int time_candidate(cap_plan p, int algo, int sync, const void *b, void *x, int r
 // the caller's b and x (x is overwritten by the solve that follows) the first
 // time cap_plan_solve sees that k: thread-per-row plans time 1, 2 and 4
 // workgroups per CU at 256 and 128 threads (how many rows, each with k
-// lanes, poll at once); partitioned plans time 1, 2 and 4 right-hand sides
-// per wavefront of the block solve (the ones that divide k).  Every
-// candidate gives the same bits.  The plan's own settings are timed first and
-// another must be 2 % faster to win.  Waits for the device (events).
+// lanes, poll at once), 1024-thread workgroups with 1, 2, 4 (8 in FP32)
+// right-hand sides per lane, and the pick again with readiness in x;
+// partitioned plans time 1, 2 and 4 right-hand sides per wavefront of the
+// block solve (the ones that divide k).  Every candidate gives the same
+// bits.  The plan's own settings are timed first and another must be 2 %
+// faster to win.  Waits for the device (events).
 int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
 {
     std::vector<RhsPick> cands;
@@ -965,12 +970,13 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
                 if (!(bpc == p->bpc && blk == p->wblock)) cands.push_back({bpc, blk, 0});
         // fewer tickets: 1024 threads (one workgroup per CU: 16 wavefronts,
         // as many as four of 256), and R right-hand sides per lane (R times
-        // the rows per ticket)
-        if (sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls) {
-            cands.push_back({1, kBlockLarge, 0, 1});
+        // the rows per ticket); these run without the LDS window whatever the
+        // plan's single-RHS pick
+        if (sync == CAP_SYNC_VALUE && !p->count_polls) {
+            cands.push_back({1, kBlockLarge, 0, 1, -1, 0});
             for (int r : {2, 4, 8})
-                if (r <= k && (r < 8 || p->vt == CAP_F32)) cands.push_back({1, kBlockLarge, 0, r});
-            if (k >= 4) cands.push_back({1, kBlock, 0, 4});
+                if (r <= k && (r < 8 || p->vt == CAP_F32)) cands.push_back({1, kBlockLarge, 0, r, -1, 0});
+            if (k >= 4) cands.push_back({1, kBlock, 0, 4, -1, 0});
         }
     } else if (algo == CAP_ALGO_PARTITION) {
         const int dflt = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
@@ -981,7 +987,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->rhs_pick[k] = RhsPick{};
         return CAP_OK;
     }
-    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr, rpl0 = p->rpl;
+    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr, rpl0 = p->rpl, lds0 = p->lds;
     float best = 1e30f;
     RhsPick pick = cands[0];
     int rc = CAP_OK;
@@ -989,6 +995,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         if (cands[c].bpc) p->bpc = cands[c].bpc;
         if (cands[c].block) p->wblock = cands[c].block;
         if (cands[c].rpl) p->rpl = cands[c].rpl;
+        if (cands[c].lds >= 0) p->lds = cands[c].lds;
         p->part_kr = cands[c].kr;
         float t = 0;
         rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
@@ -1000,6 +1007,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->wblock = blk0;
         p->part_kr = kr0;
         p->rpl = rpl0;
+        p->lds = lds0;
     }
     // readiness in x itself for the pick: one store per value instead of
     // two (x and the staged copy), at k > 1 the larger share of the bytes
@@ -1007,6 +1015,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         if (pick.bpc) p->bpc = pick.bpc;
         if (pick.block) p->wblock = pick.block;
         if (pick.rpl) p->rpl = pick.rpl;
+        if (pick.lds >= 0) p->lds = pick.lds;
         p->staged = false;
         float t = 0;
         rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
@@ -1014,6 +1023,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->bpc = bpc0;
         p->wblock = blk0;
         p->rpl = rpl0;
+        p->lds = lds0;
         if (rc == CAP_OK && t < 0.98f * best) pick.staged = 0;
     }
     if (rc != CAP_OK) return rc;
