@@ -2302,20 +2302,23 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std
     return CAP_OK;
 }
 
-// One solve of a batch: the partitioned matrices first (each writes only its
-// own rows; a refinement's residual over all rows then sees their final
-// values), then the thread-per-row or one-workgroup solve of the rest, all
-// on st.  (On a second stream, overlapping the rest's persistent kernel,
-// the partitioned solves slowed its chain more than they saved: 256 x 16
-// batch 11.5 against 10.9 ms.)
+// One solve of a batch, all on st: the partitioned matrices (each writes
+// only its own rows) and the thread-per-row or one-workgroup solve of the
+// rest.  With staged readiness the partitioned ones go first; with
+// readiness in x the rest goes first, since its reset fills all of x with
+// the sentinel.  (On a second stream, overlapping the rest's persistent
+// kernel, the partitioned solves slowed its chain more than they saved:
+// 256 x 16 batch 11.5 against 10.9 ms.)
 int batch_solve_parts(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k,
                       const void *b, void *x, hipStream_t st)
 {
     const size_t row_bytes = (size_t)k * p->vbytes;
+    const bool rest_first = !p->staged && p->algo != CAP_ALGO_LOCAL;
+    if (rest_first) CAP_TRY(plan_solve_with(p, p->algo, p->sync, b, x, k, st));
     for (size_t h = 0; h < parts.size(); ++h)
         CAP_TRY(plan_solve_with(parts[h], parts[h]->algo, parts[h]->sync, (const char *)b + lo[h] * row_bytes,
                                 (char *)x + lo[h] * row_bytes, k, st));
-    return plan_solve_with(p, p->algo, p->sync, b, x, k, st);
+    return rest_first ? CAP_OK : plan_solve_with(p, p->algo, p->sync, b, x, k, st);
 }
 
 float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k, const void *b,
@@ -2353,8 +2356,10 @@ float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vect
 // lane, each over a schedule without the partitioned matrices (b = 0).  The
 // fastest hybrid is kept when 2 % faster than the plain batch as it
 // resolved (thread-per-row or one workgroup per matrix).  The kept plans go
-// to *parts.  Plans in the reference arithmetic or with readiness in place
-// (the rest's reset fills x) are left as they are.
+// to *parts.  With several right-hand sides the rest is also timed with
+// readiness in x itself (one store per value instead of two).  Plans in the
+// reference arithmetic or created with readiness in place are left as they
+// are.
 constexpr int kHybridMax = 8;           // partitioned matrices per batch, at most
 constexpr int kHybridMinLevels = 1024;  // deep enough for a chain to matter
 
@@ -2418,15 +2423,17 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
         rc = CAP_ERR_ALLOC;
     struct Shape {
         int bpc, block, rpl;
+        bool staged;
     };
-    std::vector<Shape> shapes = {{1, kBlock, 1}, {2, kBlock, 1}, {4, kBlock, 1}};
+    std::vector<Shape> shapes = {{1, kBlock, 1, true}, {2, kBlock, 1, true}, {4, kBlock, 1, true}};
     if (k > 1 && p->sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls)  // the multi-RHS shapes
         for (int r : {1, 2, 4})
-            if (r <= k) shapes.push_back({1, kBlockLarge, r});
-    if (bpc_fixed) shapes = {{p->bpc, p->wblock, p->rpl}};  // the caller's grid stays
-    const Shape plain{p->bpc, p->wblock, p->rpl};
+            if (r <= k)
+                for (bool sg : {true, false}) shapes.push_back({1, kBlockLarge, r, sg});
+    if (bpc_fixed) shapes = {{p->bpc, p->wblock, p->rpl, true}};  // the caller's grid stays
+    const Shape plain{p->bpc, p->wblock, p->rpl, p->staged != 0};
     const int plain_algo = p->algo;
-    Shape pick{0, 0, 0};
+    Shape pick{0, 0, 0, true};
     float best = 1e30f;
     std::map<int, std::pair<int2 *, int>> trial;  // the rest's schedules, by rows per block
     if (rc == CAP_OK) best = time_batch(p, {}, {}, k, b, x, st, &rc);
@@ -2438,6 +2445,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->bpc = sh.bpc;
             p->wblock = sh.block;
             p->rpl = sh.rpl;
+            p->staged = sh.staged;
             if ((rc = build_batch_schedule(p, ro, lv, batch_solve_rows(p, k), st, &skip)) != CAP_OK) break;
             const float t = time_batch(p, cand, cand_lo, k, b, x, st, &rc);
             if (rc != CAP_OK) break;
@@ -2452,6 +2460,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->bpc = pick.bpc;
             p->wblock = pick.block;
             p->rpl = pick.rpl;
+            p->staged = pick.staged;
             // the refinement sweep follows the solve's ticket size
             p->rblock = pick.block == kBlockLarge ? kBlockLarge : (p->rblock == kBlockLarge ? kBlock : p->rblock);
         } else {
@@ -2459,6 +2468,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->bpc = plain.bpc;
             p->wblock = plain.block;
             p->rpl = plain.rpl;
+            p->staged = plain.staged;
             pick.block = 0;
         }
     }
