@@ -54,7 +54,8 @@ class cap_opts(Structure):
                 ("host_inputs", c_int), ("spin_limit", c_int), ("blocks_per_cu", c_int),
                 ("adjoint", c_int), ("lds_window", c_int), ("readiness", c_int),
                 ("upper", c_int), ("unit_diag", c_int), ("row_threads", c_int), ("arith", c_int),
-                ("count_polls", c_int), ("wait_limit_ms", c_int), ("rhs_per_lane", c_int)]
+                ("count_polls", c_int), ("wait_limit_ms", c_int), ("rhs_per_lane", c_int),
+                ("row_tiles", c_int)]
 
 
 class cap_gen_params(Structure):
@@ -81,7 +82,7 @@ SIGNATURES = {
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
     "cap_plan_poll_count": (c_int, [c_void_p, POINTER(c_ulonglong), c_void_p]),
     "cap_plan_rhs_block": (c_int, [c_void_p, c_int, PI, PI, PI, PI, PI]),
-    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI, PI]),
+    "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI, PI, PI]),
     "cap_batch_info": (c_int, [c_void_p, PI, PI]),
     "cap_batch_tuning": (c_int, [c_void_p, PI, PI, PI]),
     "cap_batch_partitioned": (c_int, [c_void_p, PI, c_int]),

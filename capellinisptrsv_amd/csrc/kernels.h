@@ -581,7 +581,13 @@ __device__ __forceinline__ void wrtfst_row(const int *__restrict__ ci, const T *
 // COUNT (value protocol only): every lane counts the poll loads it issues;
 // each wavefront adds its total to one of sc.polls' kPollCounters counters
 // when the grid drains (one atomic per wavefront).
-template <typename T, int SYNC, int LDS, int BLOCK, bool COUNT = false>
+// TILE: a ticket's rows are consecutive, so their entries are one range
+// [rp[lo], rp[lo + len]) of col_idx and val; the workgroup first copies it
+// into LDS with coalesced loads (consecutive lanes, consecutive entries),
+// and the rows read their entries there.  A range longer than kRowTile
+// entries (uniform in the workgroup) is read from global memory as usual.
+constexpr int kRowTile = 2048;
+template <typename T, int SYNC, int LDS, int BLOCK, bool COUNT = false, bool TILE = false>
 __global__ void __launch_bounds__(BLOCK)
     k_wrtfst(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
              const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, int n, int k, int lpr_log2,
@@ -590,6 +596,8 @@ __global__ void __launch_bounds__(BLOCK)
     typedef typename VBits<T>::U U;
     static_assert(!COUNT || SYNC == 1, "the poll counter is for the value protocol");
     __shared__ U s_x[LDS ? BLOCK : 1];
+    __shared__ int s_ci[TILE ? kRowTile : 1];
+    __shared__ T s_v[TILE ? kRowTile : 1];
     const int q = threadIdx.x & ((1 << lpr_log2) - 1);
     const int r = threadIdx.x >> lpr_log2;
     const int rows = BLOCK >> lpr_log2;
@@ -608,9 +616,25 @@ __global__ void __launch_bounds__(BLOCK)
             }
             const int i = lo + r;
             LdsWin<U> win{s_x, lo, (LDS && !n_rev) ? len : 0, k};
+            const int *cix = ci;
+            const T *vx = val;
+            int e0 = 0;  // entry j of a row is cix[j - e0]
+            if constexpr (TILE) {
+                const int first = rp[lo], last = rp[min(lo + len, n)];
+                if (last - first <= kRowTile) {  // uniform in the workgroup
+                    for (int e = threadIdx.x; e < last - first; e += BLOCK) {
+                        s_ci[e] = ci[first + e];
+                        s_v[e] = val[first + e];
+                    }
+                    __syncthreads();
+                    cix = s_ci;
+                    vx = s_v;
+                    e0 = first;
+                }
+            }
             if (r < len && i < n && q < k)
-                wrtfst_row<T, SYNC, COUNT>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, k, q, (T)0, n_rev, win,
-                                           &np);
+                wrtfst_row<T, SYNC, COUNT>(cix, vx, b, x, sc, i, rp[i] - e0, rp[i + 1] - 1 - e0, k, q, (T)0, n_rev,
+                                           win, &np);
         });
     if constexpr (COUNT) {
         unsigned long long t = np;
@@ -951,7 +975,12 @@ __global__ void __launch_bounds__(kBlock)
 // (wave_row: parallel polls, sequential fused sum); a longer unit is solved
 // thread-per-row with the write-first loop.  The 16 units of a ticket share
 // an LDS window.  Single right-hand side.
-template <typename T, int SYNC, int BS>
+// TILE: the ticket's units are consecutive rows [s_lo, s_lo + s_len), whose
+// entries are first copied into LDS with coalesced loads (up to kMixTile of
+// them; a longer range, uniform in the workgroup, is read from global memory)
+// for the thread-per-row units.  Single-row units read global memory.
+constexpr int kMixTile = 4096;
+template <typename T, int SYNC, int BS, bool TILE = false>
 __global__ void __launch_bounds__(BS)
     k_mix(const int *__restrict__ rp, const int *__restrict__ ci, const T *__restrict__ val,
           const T *__restrict__ b, T *__restrict__ x, SyncCtx sc, const int *__restrict__ unit, int n_units,
@@ -961,6 +990,8 @@ __global__ void __launch_bounds__(BS)
     constexpr int kUnits = BS / kWave;     // one unit per wavefront per ticket
     __shared__ U s_x[BS / kWave * kWave];  // rows of the ticket: <= 64 per unit
     __shared__ int s_lo, s_len;
+    __shared__ int s_ci[TILE ? kMixTile : 1];
+    __shared__ T s_v[TILE ? kMixTile : 1];
     const int lane = threadIdx.x & (kWave - 1);
     ticket_loop(
         ticket, kUnits, n_units,
@@ -972,13 +1003,30 @@ __global__ void __launch_bounds__(BS)
             }
         },
         [&](int base) {
+            const int *cix = ci;
+            const T *vx = val;
+            int e0 = 0;  // entry j of a thread-per-row unit's row is cix[j - e0]
+            if constexpr (TILE) {
+                const int first = rp[s_lo], last = rp[s_lo + s_len];
+                if (last - first <= kMixTile) {  // uniform in the workgroup
+                    for (int e = threadIdx.x; e < last - first; e += BS) {
+                        s_ci[e] = ci[first + e];
+                        s_v[e] = val[first + e];
+                    }
+                    __syncthreads();
+                    cix = s_ci;
+                    vx = s_v;
+                    e0 = first;
+                }
+            }
             const int u = base + (int)(threadIdx.x / kWave);
             if (u >= n_units) return;
             const LdsWin<U> win{s_x, s_lo, SYNC == 1 ? s_len : 0, 1};
             const int r0 = unit[u], r1 = unit[u + 1];
             if (r1 - r0 > 1) {
                 const int i = r0 + lane;
-                if (i < r1) wrtfst_row<T, SYNC>(ci, val, b, x, sc, i, rp[i], rp[i + 1] - 1, 1, 0, (T)0, 0, win);
+                if (i < r1)
+                    wrtfst_row<T, SYNC>(cix, vx, b, x, sc, i, rp[i] - e0, rp[i + 1] - 1 - e0, 1, 0, (T)0, 0, win);
                 return;
             }
             wave_row<T, SYNC>(rp, ci, val, b, x, sc, win, r0);

@@ -112,6 +112,7 @@ struct cap_plan_s {
     int wblock = kBlock;     // threads per workgroup of k_wrtfst (128 or 256)
     int rblock = 0;          // of k_refine2; 0 = not chosen yet (autotuned plans time it)
     int lds = 0;             // thread-per-row: workgroup-local LDS readiness window
+    int tile = 0;            // thread-per-row, value protocol: tickets' col_idx/val tiles staged in LDS
     int unfused = 0;         // CAP_ARITH_REFERENCE: rounded products, then rounded sums
     int count_polls = 0;     // the write-first solve counts its poll loads (ctrl + 1024)
     // multi-RHS solves of an autotuned plan: the RHS block is timed per
@@ -728,7 +729,11 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
                 small ? go(k_wrtfst<T, 1, 1, kBlockSmall, true>) : go(k_wrtfst<T, 1, 1, kBlock, true>);
             else
                 small ? go(k_wrtfst<T, 1, 0, kBlockSmall, true>) : go(k_wrtfst<T, 1, 0, kBlock, true>);
-        } else if (sync == CAP_SYNC_VALUE && p->lds)
+        } else if (sync == CAP_SYNC_VALUE && p->tile && p->lds)
+            small ? go(k_wrtfst<T, 1, 1, kBlockSmall, false, true>) : go(k_wrtfst<T, 1, 1, kBlock, false, true>);
+        else if (sync == CAP_SYNC_VALUE && p->tile)
+            small ? go(k_wrtfst<T, 1, 0, kBlockSmall, false, true>) : go(k_wrtfst<T, 1, 0, kBlock, false, true>);
+        else if (sync == CAP_SYNC_VALUE && p->lds)
             small ? go(k_wrtfst<T, 1, 1, kBlockSmall>) : go(k_wrtfst<T, 1, 1, kBlock>);
         else if (sync == CAP_SYNC_VALUE)
             small ? go(k_wrtfst<T, 1, 0, kBlockSmall>) : go(k_wrtfst<T, 1, 0, kBlock>);
@@ -745,7 +750,10 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         constexpr int kMixBlock = 1024;  // 16 units per ticket share an LDS window
         const unsigned nb = grid_for(p, ((long long)p->n_units * kWave + kMixBlock - 1) / kMixBlock);
-        if (sync == CAP_SYNC_VALUE)
+        if (sync == CAP_SYNC_VALUE && p->tile)
+            k_mix<T, 1, kMixBlock, true><<<nb, kMixBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit,
+                                                                   p->n_units, p->ticket);
+        else if (sync == CAP_SYNC_VALUE)
             k_mix<T, 1, kMixBlock><<<nb, kMixBlock, 0, st>>>(p->rp, p->ci, val, b, x, sc, p->unit, p->n_units,
                                                              p->ticket);
         else
@@ -900,7 +908,7 @@ struct TuneKey {
 };
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int algo, sync, border, bpc, lds, staged, block;
+    int algo, sync, border, bpc, lds, staged, block, tile = 0;
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 
@@ -1039,6 +1047,7 @@ void apply_choice(cap_plan p, const TuneChoice &c)
     p->lds = c.lds;
     p->staged = c.staged;
     p->wblock = c.block;
+    p->tile = c.tile;
 }
 
 // Tables of the partitioned banded solve (CAP_ALGO_PARTITION), built once:
@@ -1376,6 +1385,9 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
                                      {CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 1, stg, S},
                                      {CAP_ALGO_WRTFST, CAP_SYNC_FLAG, 0, 1, 0, 0, B}};
     if (stg) cands.push_back({CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
+    // the same value-protocol shapes with each ticket's col_idx/val tile staged in LDS
+    for (int lds : {0, 1})
+        for (int blk : {B, S}) cands.push_back({CAP_ALGO_WRTFST, CAP_SYNC_VALUE, 0, 1, lds, stg, blk, 1});
     const double avg = (double)p->nnz / std::max(1, p->n);
     // one workgroup per matrix pays off only with few rows per level
     {
@@ -1393,6 +1405,8 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
         if (avg * 4 >= bd) {
             cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg, B});
             cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg, B});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg, B, 1});
+            cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg, B, 1});
         }
     float best = 1e30f;
     TuneChoice pick = cands[0];
@@ -1496,6 +1510,11 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
     if (o.rhs_per_lane != 0 && o.rhs_per_lane != 1 && o.rhs_per_lane != 2 && o.rhs_per_lane != 4 &&
         o.rhs_per_lane != 8)
         return CAP_ERR_ARG;
+    if (o.row_tiles != 0 && o.row_tiles != 1) return CAP_ERR_ARG;
+    if (o.row_tiles &&
+        ((o.algo != CAP_ALGO_WRTFST && o.algo != CAP_ALGO_MIX) || (o.sync != CAP_SYNC_AUTO && o.sync != CAP_SYNC_VALUE) ||
+         o.count_polls || o.rhs_per_lane > 1))
+        return CAP_ERR_UNSUPPORTED;
     if (o.rhs_per_lane > 0 &&
         (o.algo != CAP_ALGO_WRTFST || (o.sync != CAP_SYNC_AUTO && o.sync != CAP_SYNC_VALUE) || o.lds_window ||
          o.count_polls || (o.rhs_per_lane == 8 && vt != CAP_F32)))
@@ -1587,6 +1606,7 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.blocks_per_cu > 0) p->bpc = o.blocks_per_cu;
         if (o.row_threads > 0) p->wblock = o.row_threads;
         p->fixed_rpl = o.rhs_per_lane;
+        p->tile = o.row_tiles != 0;
         p->auto_algo = o.algo == CAP_ALGO_AUTO;
         p->lds = o.lds_window > 0;
         p->unfused = o.arith == CAP_ARITH_REFERENCE;
@@ -1823,9 +1843,10 @@ int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, 
 }
 
 int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoint, int *readiness,
-                    int *row_threads)
+                    int *row_threads, int *row_tiles)
 {
     if (!p) return CAP_ERR_ARG;
+    if (row_tiles) *row_tiles = p->tile;
     if (blocks_per_cu) *blocks_per_cu = p->bpc;
     if (row_threads) *row_threads = p->wblock;
     if (lds_window) *lds_window = p->lds;
@@ -2506,7 +2527,7 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     cap_opts o;
     cap_opts_default(&o);
     if (opts) o = *opts;
-    if (o.upper || o.unit_diag || o.rhs_per_lane) return CAP_ERR_UNSUPPORTED;
+    if (o.upper || o.unit_diag || o.rhs_per_lane || o.row_tiles) return CAP_ERR_UNSUPPORTED;
     hipStream_t st = S(stream);
     int dev = 0;
     CAP_TRY(check_device(&dev));

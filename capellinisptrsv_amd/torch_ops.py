@@ -50,6 +50,8 @@ class Plan:
     arith: "fused" (one fused multiply-add per term) or "reference" (a
     rounded product then a rounded sum per term: bit-identical to the
     reference's serial CPU executor; not with csc or partition).
+    row_tiles (value protocol): each ticket's col_idx/val range is copied to
+    LDS with coalesced loads first (algo auto times it).
     rhs_per_lane (algo wrtfst, value protocol): solves of several right-hand
     sides run in 1024-thread workgroups with that many per lane (1, 2, 4;
     8 for FP32); 0 leaves the choice to the workgroup size above (or to the
@@ -60,7 +62,8 @@ class Plan:
                  sync="auto", border: int = 0, validate: bool = True, spin_limit: int = 0,
                  blocks_per_cu: int = 0, adjoint="auto", lds_window: bool = False, readiness="auto",
                  upper: bool = False, unit_diagonal: bool = False, row_threads: int = 0, arith="fused",
-                 wait_limit_ms: int = 0, count_polls: bool = False, rhs_per_lane: int = 0, stream=None):
+                 wait_limit_ms: int = 0, count_polls: bool = False, rhs_per_lane: int = 0, row_tiles: bool = False,
+                 stream=None):
         for t, dt in ((row_ptr, torch.int32), (col_idx, torch.int32)):
             if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
                 raise TypeError("row_ptr/col_idx must be contiguous int32 CUDA tensors")
@@ -75,7 +78,7 @@ class Plan:
                               lds_window=int(lds_window), readiness=readiness, upper=int(upper),
                               unit_diag=int(unit_diagonal), row_threads=row_threads, arith=arith,
                               wait_limit_ms=wait_limit_ms, count_polls=int(count_polls),
-                              rhs_per_lane=rhs_per_lane)
+                              rhs_per_lane=rhs_per_lane, row_tiles=int(row_tiles))
         h = c_void_p()
         N.check(N.lib().cap_plan_create(byref(h), self.n, self.nnz, _ptr(row_ptr), _ptr(col_idx), _ptr(values),
                                         _vt(values.dtype), byref(opts), _stream(stream)), "cap_plan_create")
@@ -87,13 +90,14 @@ class Plan:
         wo, eo = c_double(), c_double()
         N.check(N.lib().cap_plan_info(self._h, byref(a), byref(s), byref(b), byref(ln), byref(wo), byref(eo)),
                 "cap_plan_info")
-        bpc, lds, adj, rdy, thr = c_int(), c_int(), c_int(), c_int(), c_int()
-        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj), byref(rdy), byref(thr)),
-                "cap_plan_tuning")
+        bpc, lds, adj, rdy, thr, tile = c_int(), c_int(), c_int(), c_int(), c_int(), c_int()
+        N.check(N.lib().cap_plan_tuning(self._h, byref(bpc), byref(lds), byref(adj), byref(rdy), byref(thr),
+                                        byref(tile)), "cap_plan_tuning")
         return dict(algo=N.ALGO_NAME[a.value], sync=N.SYNC_NAME[s.value], border=b.value, mix_len=ln.value,
                     warp_occupy=wo.value, element_occupy=eo.value, blocks_per_cu=bpc.value,
                     lds_window=bool(lds.value), adjoint={1: "scatter", 2: "gather"}.get(adj.value, "?"),
-                    readiness=N.READINESS_NAME.get(rdy.value, "?"), row_threads=thr.value)
+                    readiness=N.READINESS_NAME.get(rdy.value, "?"), row_threads=thr.value,
+                    row_tiles=bool(tile.value))
 
     def poll_count(self, stream=None) -> int:
         """Poll loads issued by the thread-per-row solves since the last call

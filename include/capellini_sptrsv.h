@@ -153,6 +153,14 @@ typedef struct cap_opts {
                           the autotuned RHS block with algo AUTO).  Fewer
                           lanes per row put more rows in each workgroup
                           ticket, whose atomics serialize on one counter. */
+    int row_tiles;     /* 1 = the thread-per-row solves (algo WRTFST at 128
+                          or 256 threads, algo MIX; value protocol) first
+                          copy each ticket's col_idx/val range (up to 2048
+                          entries, 4096 for MIX) into LDS with coalesced
+                          loads, and their rows read their entries there;
+                          0 = off, or timed by algo AUTO.  Not with
+                          count_polls, rhs_per_lane > 1 or batches
+                          (CAP_ERR_UNSUPPORTED). */
 } cap_opts;
 
 /* Fills *o with the defaults (auto algorithm, auto sync, validate on). */
@@ -210,10 +218,10 @@ int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
  * CAP_ERR_UNSUPPORTED.) */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
-/* Grid, window and readiness choices the plan uses (after autotuning);
- * *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE. */
+/* Grid, window, readiness and row-tile choices the plan uses (after
+ * autotuning); *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE. */
 int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint,
-                    int *readiness, int *row_threads);
+                    int *readiness, int *row_threads, int *row_tiles);
 int cap_plan_destroy(cap_plan plan);
 
 /* Adjoint of x = L^{-1} b: grad_b = L^{-T} g and grad_val[k] =

@@ -150,6 +150,77 @@ def test_rhs_per_lane_bit_exact(cuda, name, mat, dtype):
                 assert np.array_equal(x, ref), (name, k, r, arith)
 
 
+@pytest.mark.parametrize("lds,threads", [(False, 0), (True, 0), (False, 128), (True, 128)])
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_row_tiles_bit_exact(cuda, lds, threads, name, mat):
+    """Tickets' col_idx/val ranges staged in LDS: bit-exact against the fused
+    oracle at one and several right-hand sides, in FP64 and FP32.  cfg2_small
+    (~38 nnz/row) has ranges above the 2048-entry tile at 256 rows a ticket,
+    so its tickets read global memory; the others run from LDS."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = mat
+    n = len(rp) - 1
+    for dtype in (np.float64, np.float32):
+        vv = v.astype(dtype)
+        d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, vv)]
+        for k in (1, 3, 16):
+            b = api.uniform(300 + k, n * k).astype(dtype)
+            bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
+            plan = Plan(*d, algo="wrtfst", sync="value", lds_window=lds, row_threads=threads, row_tiles=True)
+            info = plan.info()
+            assert info["row_tiles"] and info["row_threads"] == (threads or 256)
+            x = plan.solve(bt).cpu().numpy().reshape(-1)
+            assert plan.status() == 0
+            plan.close()
+            ref = oracle.csr_forward(rp, ci, vv, b, rhs=k, fused=True)
+            assert np.array_equal(x, ref), (name, dtype, k, rel_maxnorm(x, ref))
+
+
+@pytest.mark.parametrize("border", [4, 16, 10**6])
+@pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
+def test_mix_row_tiles_bit_exact(cuda, border, name, mat):
+    """mix with the ticket's rows staged in LDS (thread-per-row units read
+    the tile, single-row units global memory): bit-exact in FP64 and FP32."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = mat
+    n = len(rp) - 1
+    for dtype in (np.float64, np.float32):
+        vv = v.astype(dtype)
+        b = api.uniform(3, n).astype(dtype)
+        plan = Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, vv)), algo="mix", sync="value",
+                    border=border, row_tiles=True)
+        assert plan.info()["row_tiles"]
+        x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+        assert plan.status() == 0
+        plan.close()
+        ref = oracle.csr_forward(rp, ci, vv, b, fused=True)
+        assert np.array_equal(x, ref), (name, dtype, rel_maxnorm(x, ref))
+
+
+@pytest.mark.parametrize("upper,unit", [(True, False), (False, True), (True, True)])
+def test_row_tiles_triangular_kinds(cuda, upper, unit):
+    """Row tiles on upper (reversed rows) and implicit-unit-diagonal solves."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    from conftest import drop_diagonal, upper_from_lower
+    rp, ci, v = random_lower(3000, max_off=6, band=200, seed=4, dup=True)
+    if upper:
+        rp, ci, v = upper_from_lower(rp, ci, v)
+    if unit:
+        rp, ci, v = drop_diagonal(rp, ci, v, upper=upper)
+    n = len(rp) - 1
+    b = api.uniform(41, n)
+    plan = Plan(*(torch.from_numpy(a).to(cuda) for a in (rp, ci, v)), algo="wrtfst", sync="value",
+                upper=upper, unit_diagonal=unit, row_tiles=True)
+    x = plan.solve(torch.from_numpy(b).to(cuda)).cpu().numpy()
+    assert plan.status() == 0
+    plan.close()
+    ref = oracle.csr_solve(rp, ci, v, b, upper=upper, unit=unit, fused=True)
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
+
+
 def test_local_deep_band_bit_exact(cuda):
     """One workgroup walks a deep banded matrix (n = 20,000, bandwidth 128,
     ~38 nnz/row, thousands of levels) and matches the oracle bit for bit."""

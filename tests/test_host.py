@@ -279,6 +279,32 @@ def test_plan_create_checks_rhs_per_lane():
     assert rc == _native.CAP_ERR_UNSUPPORTED and not h.value
 
 
+
+def test_plan_create_checks_row_tiles():
+    """row_tiles takes 0 or 1, with the write-first or mix algorithm in the
+    value protocol, without the poll counter or several right-hand sides per lane;
+    batches refuse it -- all before any device call."""
+    from ctypes import byref, c_void_p
+    rp, ci, v = random_lower(50, seed=3)
+    h = c_void_p()
+    cases = ((2, {}, _native.CAP_ERR_ARG), (-1, {}, _native.CAP_ERR_ARG),
+             (1, dict(algo="auto"), _native.CAP_ERR_UNSUPPORTED), (1, dict(algo="twopart"), _native.CAP_ERR_UNSUPPORTED),
+             (1, dict(algo="local"), _native.CAP_ERR_UNSUPPORTED),
+             (1, dict(sync="flag"), _native.CAP_ERR_UNSUPPORTED), (1, dict(count_polls=1), _native.CAP_ERR_UNSUPPORTED),
+             (1, dict(rhs_per_lane=2), _native.CAP_ERR_UNSUPPORTED))
+    for t, kw, want in cases:
+        kw = dict(dict(algo="wrtfst", sync="value"), **kw)
+        o = _native.default_opts(host_inputs=1, row_tiles=t, **kw)
+        rc = _native.lib().cap_plan_create(byref(h), 50, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                           _native.CAP_F64, byref(o), None)
+        assert rc == want, (t, kw, rc)
+        assert not h.value
+    ro = np.array([0, 50], np.int32)
+    o = _native.default_opts(host_inputs=1, row_tiles=1)
+    rc = _native.lib().cap_batch_create(byref(h), 1, ro.ctypes.data, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                        _native.CAP_F64, 4, byref(o), None)
+    assert rc == _native.CAP_ERR_UNSUPPORTED and not h.value
+
 def test_plan_create_checks_the_arithmetic_option():
     """arith takes CAP_ARITH_FUSED or CAP_ARITH_REFERENCE; the reference
     arithmetic is refused by the algorithms whose order is not the sequential
