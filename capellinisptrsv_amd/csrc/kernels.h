@@ -803,13 +803,15 @@ __device__ __forceinline__ void block_max_to(unsigned long long v, unsigned long
 // ------------------------------------------------ fused refinement sweep
 // The first two steps of mixed-precision refinement in one dependency sweep.
 // Row i computes, from the pairs (x0_j, d_j) of its dependencies j:
-//   x0_i = ((float)b_i - sum v32_j x0_j) / v32_ii          FP32 solve of b
+//   x0_i = ((float)b_i - sum v32_j x0_j) * r32_i           FP32 solve of b
 //   r_i  = b_i - (sum v_j x0_j + v_ii x0_i)                FP64 residual
-//   d_i  = ((float)r_i - sum v32_j d_j) / v32_ii           FP32 correction
+//   d_i  = ((float)r_i - sum v32_j d_j) * r32_i            FP32 correction
 //   x_i  = (double)x0_i + (double)d_i
-// (v32 = (float)v; every sum an fma chain in stored order), which is bit for
-// bit what the two steps give when run one after the other: FP32 solve,
-// residual, FP32 solve of the residual, accumulation.  Row i publishes
+// (v32 = (float)v, r32_i = 1.0f / v32_ii computed before the row's polling
+// loop, so no division is left on the critical path; the correction absorbs
+// the extra rounding of the product; every sum an fma chain in stored order),
+// which is bit for bit what the two steps give when run one after the other:
+// FP32 solve, residual, FP32 solve of the residual, accumulation.  Row i publishes
 // (x0_i, d_i) as one 64-bit word (x0 in the low half) in the value protocol;
 // a dependency's word is read once and feeds all three chains, so the sweep
 // has the dependency depth of a single solve.  The polling window is the one
@@ -841,6 +843,7 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
     double s = 0.0;
     const long long o = (long long)i * k + q;
     const double vd = val[d], bo = b[o];  // before the loop, off the critical path
+    const float rd32 = 1.0f / (float)vd;
     if (j < d) {
         c0 = ci[j];
         v0 = val[j];
@@ -887,10 +890,9 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
                 j = d;
             }
             if (j >= d) {  // write first: publish inside the polling loop
-                const float vd32 = (float)vd;
-                const float x0 = ((float)bo - s0) / vd32;
+                const float x0 = ((float)bo - s0) * rd32;
                 s = __builtin_fma(vd, (double)x0, s);
-                const float dc = ((float)(bo - s) - s1) / vd32;
+                const float dc = ((float)(bo - s) - s1) * rd32;
                 U u = ((U)__float_as_uint(dc) << 32) | (U)__float_as_uint(x0);
                 if (u == kS) u = kPairQNaN;
                 if (win.len) win.store(i, q, u);

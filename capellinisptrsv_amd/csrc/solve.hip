@@ -2646,17 +2646,30 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
                             double *last_rel, void *stream)
 {
     if (!bt || !b || !x) return CAP_ERR_ARG;
-    // the partitioned matrices first (FP64 partitioned solves and
-    // corrections), then the rest: iterations and correction are the largest
+    // The partitioned matrices first, then the rest; iterations and
+    // correction are the largest.  A partitioned plan's tables are FP64 and
+    // its creation checked its solves against a sequential FP64 solve to
+    // kPartStableTol, so at tol >= kPartStableTol its one FP64 solve is the
+    // answer (no FP32 step to refine; a correction would only re-solve the
+    // rounding noise).  A tighter tol refines it in FP64.
     int it_max = 0;
     double rel_max = 0.0;
     int rc = CAP_OK;
     const size_t row = (size_t)bt->k;
-    for (size_t h = 0; h < bt->parts.size() && rc == CAP_OK; ++h) {
+    // a part that does not converge still leaves its best x: the others run
+    for (size_t h = 0; h < bt->parts.size() && (rc == CAP_OK || rc == CAP_ERR_NOT_CONVERGED); ++h) {
+        const double *bh = b + bt->part_lo[h] * row;
+        double *xh = x + bt->part_lo[h] * row;
+        if (tol >= kPartStableTol) {
+            const int rh = cap_plan_solve(bt->parts[h], bh, xh, bt->k, stream);
+            if (rh != CAP_OK) rc = rh;
+            it_max = std::max(it_max, 1);
+            continue;
+        }
         int it = 0;
         double rel = 0.0;
-        rc = refined_impl(bt->parts[h], b + bt->part_lo[h] * row, x + bt->part_lo[h] * row, bt->k, tol, max_iter,
-                          &it, &rel, S(stream));
+        const int rh = refined_impl(bt->parts[h], bh, xh, bt->k, tol, max_iter, &it, &rel, S(stream));
+        if (rh != CAP_OK) rc = rh;
         it_max = std::max(it_max, it);
         rel_max = std::max(rel_max, rel);
     }
@@ -2666,7 +2679,7 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
         const int rc2 = refined_impl(bt->plan, b, x, bt->k, tol, max_iter, &it, &rel, S(stream));
         it_max = std::max(it_max, it);
         rel_max = std::max(rel_max, rel);
-        if (rc == CAP_OK) rc = rc2;
+        if (rc == CAP_OK || (rc2 != CAP_OK && rc2 != CAP_ERR_NOT_CONVERGED)) rc = rc2;
     }
     if (iters) *iters = it_max;
     if (last_rel) *last_rel = rel_max;

@@ -286,6 +286,11 @@ int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane, int *
  * how many there are, or a negative CAP_ERR_* code. */
 int cap_batch_partitioned(cap_batch batch, int *first_rows, int capacity);
 int cap_batch_solve(cap_batch batch, const void *b, void *x, void *stream);
+/* FP32 solves inside FP64 refinement for every thread-per-row or
+ * one-workgroup matrix.  A hybrid batch's partitioned matrices (FP64 tables,
+ * checked at creation against a sequential FP64 solve to 1e-13) get one FP64
+ * partitioned solve when tol >= 1e-13, FP64 refinement otherwise.  *iters
+ * and *last_rel_correction are the largest over the batch's parts. */
 int cap_batch_solve_refined(cap_batch batch64, const double *b, double *x,
                             double tol, int max_iter, int *iters,
                             double *last_rel_correction, void *stream);

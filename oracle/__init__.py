@@ -189,15 +189,17 @@ def residual(rp, ci, v, b, x, rhs=1):
     return r
 
 
-def refine2(rp, ci, v, b, rhs=1):
+def refine2(rp, ci, v, b, rhs=1, rcp=True):
     """Two refinement steps (FP32 solve, FP64 residual, FP32 correction):
-    returns (x0, d, x) with x = x0 + d in FP64."""
+    returns (x0, d, x) with x = x0 + d in FP64.  rcp: the FP32 rows end with a
+    product by the rounded reciprocal diagonal (the fused sweep) instead of a
+    division (the one-workgroup plans' two separate solves)."""
     n = len(rp) - 1
     x0 = np.zeros(n * rhs, np.float32)
     d = np.zeros(n * rhs, np.float32)
     x = np.zeros(n * rhs, np.float64)
     lib().orc_refine2_d(n, _pi(_i32(rp)), _pi(_i32(ci)), _pv(np.ascontiguousarray(v, np.float64)), rhs,
-                        _pv(np.ascontiguousarray(b, np.float64).reshape(-1)), _pv(x0), _pv(d), _pv(x))
+                        _pv(np.ascontiguousarray(b, np.float64).reshape(-1)), _pv(x0), _pv(d), _pv(x), int(rcp))
     return x0, d, x
 
 

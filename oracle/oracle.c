@@ -391,8 +391,25 @@ void orc_residual_d(int n, const int *rp, const int *ci, const double *v, int rh
         }
 }
 
+/* FP32 fused forward solve in stored order whose rows end with a product by
+ * the rounded reciprocal of the diagonal instead of a division. */
+static void orc_forward_rcp_s(int n, const int *rp, const int *ci, const float *v, int rhs,
+                              const float *b, float *x)
+{
+    int i, j, k;
+    for (i = 0; i < n; ++i) {
+        int diag = rp[i + 1] - 1;
+        float r = 1.0f / v[diag];
+        for (k = 0; k < rhs; ++k) {
+            float s = 0.0f;
+            for (j = rp[i]; j < diag; ++j) s = fmaf(x[(size_t)ci[j] * rhs + k], v[j], s);
+            x[(size_t)i * rhs + k] = (b[(size_t)i * rhs + k] - s) * r;
+        }
+    }
+}
+
 void orc_refine2_d(int n, const int *rp, const int *ci, const double *v, int rhs,
-                   const double *b, float *x0, float *d, double *x)
+                   const double *b, float *x0, float *d, double *x, int rcp)
 {
     size_t nnz = (size_t)rp[n], len = (size_t)n * rhs, t;
     float *v32 = (float *)calloc(nnz + 1, sizeof(float));
@@ -400,14 +417,16 @@ void orc_refine2_d(int n, const int *rp, const int *ci, const double *v, int rhs
     int i, j, k;
     for (t = 0; t < nnz; ++t) v32[t] = (float)v[t];
     for (t = 0; t < len; ++t) b32[t] = (float)b[t];
-    orc_csr_forward_s(n, rp, ci, v32, rhs, b32, x0, 1);
+    if (rcp) orc_forward_rcp_s(n, rp, ci, v32, rhs, b32, x0);
+    else orc_csr_forward_s(n, rp, ci, v32, rhs, b32, x0, 1);
     for (i = 0; i < n; ++i)
         for (k = 0; k < rhs; ++k) {
             double s = 0.0;
             for (j = rp[i]; j < rp[i + 1]; ++j) s = fma(v[j], (double)x0[(size_t)ci[j] * rhs + k], s);
             b32[(size_t)i * rhs + k] = (float)(b[(size_t)i * rhs + k] - s);
         }
-    orc_csr_forward_s(n, rp, ci, v32, rhs, b32, d, 1);
+    if (rcp) orc_forward_rcp_s(n, rp, ci, v32, rhs, b32, d);
+    else orc_csr_forward_s(n, rp, ci, v32, rhs, b32, d, 1);
     for (t = 0; t < len; ++t) x[t] = (double)x0[t] + (double)d[t];
     free(v32);
     free(b32);

@@ -126,9 +126,12 @@ void orc_residual_d(int n, const int *rp, const int *ci, const double *v,
  *      x0 = L32^{-1} b32 in FP32 (fused, stored order), r = b - L x0 in FP64
  *      (one fma chain over the row in stored order, diagonal included) rounded
  *      to FP32, d = L32^{-1} r32 in FP32, x = (double)x0 + (double)d.  L32 and
- *      b32 are the FP32 roundings of L and b.  Multi-RHS interleaved. */
+ *      b32 are the FP32 roundings of L and b.  Multi-RHS interleaved.
+ *      rcp = 0: each FP32 row ends with a division by its diagonal;
+ *      rcp = 1: with a product by the rounded reciprocal 1.0f / diag (the
+ *      fused sweep's form; the correction absorbs the extra rounding). */
 void orc_refine2_d(int n, const int *rp, const int *ci, const double *v, int rhs,
-                   const double *b, float *x0, float *d, double *x);
+                   const double *b, float *x0, float *d, double *x, int rcp);
 
 /* ---- metric formulas: CUDA/ourWrtFst/spts_syncfree_cuda_csr_wrt.h:206-215
  *      (dataSize; mix adds 4*Len, CUDA/mix/spts_syncfree_cuda_csr_wrt.h:317). */

@@ -43,8 +43,10 @@ def test_refined_reaches_1e12(cuda, cfg, scale, k, algo):
     ("wrtfst", "in_place", False, 1, 16, 128), ("local", "auto", False, 1, 1, 0), ("local", "auto", False, 1, 16, 0)])
 def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k, threads):
     """Thread-per-row plans run the first two refinement steps as one fused
-    sweep, one-workgroup plans as two solves with the residual between them;
-    both must give the oracle's two-step result bit for bit."""
+    sweep (FP32 rows end with a product by the reciprocal diagonal),
+    one-workgroup plans as two solves with the residual between them
+    (divisions); both must give the oracle's two-step result in that form
+    bit for bit."""
     import torch
     from capellinisptrsv_amd import Plan
     for name, (rp, ci, v) in (("cfg1_small", api.make_config(1, seed=2, scale=0.05)),
@@ -55,7 +57,7 @@ def test_refine_two_steps_bit_exact(cuda, algo, readiness, lds, bpc, k, threads)
                     row_threads=threads)
         b = api.uniform(21, n * k)
         bt = torch.from_numpy(b.reshape(n, k) if k > 1 else b).to(cuda)
-        _, _, want = oracle.refine2(rp, ci, v, b, rhs=k)
+        _, _, want = oracle.refine2(rp, ci, v, b, rhs=k, rcp=algo != "local")  # the sweep's reciprocal form
         out = torch.empty_like(bt)
         for rep in range(2):  # same b and x: the second call replays the captured graph
             x, iters, rel = plan.solve_refined(bt, tol=1e-12, max_iter=10, out=out)
@@ -164,20 +166,35 @@ def test_hybrid_batch_fp32(cuda):
 
 
 def _check_hybrid(bt, bd, mats, row_off, b, k):
+    import torch
     x = bt.solve(bd).cpu().numpy().reshape(-1)
     x2 = bt.solve(bd).cpu().numpy().reshape(-1)
     assert np.array_equal(x, x2)
     xr, iters, rel = bt.solve_refined(bd, tol=1e-12)
     xr = xr.cpu().numpy().reshape(-1)
+    # below the partitioned plans' creation check (1e-13) they are refined in
+    # FP64 too; a correction that stalls near the rounding noise of an
+    # ill-conditioned member may report non-convergence, the result must hold
+    from capellinisptrsv_amd import CapError
+    xt = torch.empty_like(bd)
+    iters_t = rel_t = None
+    try:
+        _, iters_t, rel_t = bt.solve_refined(bd, tol=9e-14, out=xt)
+    except CapError as e:
+        assert e.code == -12, e
+    xt = xt.cpu().numpy().reshape(-1)
     part = set(bt.partitioned_matrices)
     for m, (mrp, mci, mv) in enumerate(mats):
         lo, hi = row_off[m] * k, row_off[m + 1] * k
         ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
         if m in part:
             assert rel_maxnorm(x[lo:hi], ref) <= 1e-12, m
+            # at tol 1e-12 the refined batch solves them once in FP64
+            assert np.array_equal(xr[lo:hi], x[lo:hi]), m
         else:
             assert np.array_equal(x[lo:hi], ref), m
         assert rel_maxnorm(xr[lo:hi], ref) <= 1e-12, (m, iters, rel)
+        assert rel_maxnorm(xt[lo:hi], ref) <= 1e-12, (m, iters_t, rel_t)
 
 
 def _chain(n):
@@ -239,7 +256,8 @@ def test_batch_refined(cuda, algo):
         ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
         assert rel_maxnorm(x[lo:hi], ref) <= 1e-12, (m, iters, rel)
         if iters == 2:  # stopped after the first correction: the oracle's two steps exactly
-            assert np.array_equal(x[lo:hi], oracle.refine2(mrp, mci, mv, b[lo:hi], rhs=k)[2]), m
+            want = oracle.refine2(mrp, mci, mv, b[lo:hi], rhs=k, rcp=algo != "local")[2]
+            assert np.array_equal(x[lo:hi], want), m
 
 
 def test_batch_rejects_bad_matrix(cuda):

@@ -18,8 +18,9 @@ same checks -- see test_planted_mistakes_are_caught.
   csr_forward          [C] bit-equal to the executor (unfused), [D]
   adjoint / grad_val   [R][G] (backward executor on L^T), [D]
   residual, datasize   [C]
-  refine2              [C] x0 bit-equal to the FP32 forward solve, exact
-                       inputs give d = 0, per-RHS columns; [D] x within 1e-12
+  refine2              [C] x0 bit-equal to the FP32 forward solve (division form;
+                       the reciprocal form equals it on power-of-two diagonals),
+                       exact inputs give d = 0, per-RHS columns; [D] x within 1e-12
                        of the dense FP64 solve, d within 1e-5 of the dense
                        solve of the FP64 residual
 """
@@ -260,7 +261,7 @@ def test_refine2_pins():
     rp, ci, v = random_lower(1500, max_off=8, band=200, seed=11)
     n = 1500
     b = np.random.default_rng(3).uniform(-1, 1, size=n)
-    x0, d, x = oracle.refine2(rp, ci, v, b)
+    x0, d, x = oracle.refine2(rp, ci, v, b, rcp=False)
     # x0 is the FP32 solve on the rounded inputs
     assert np.array_equal(x0, oracle.csr_forward(rp, ci, v.astype(np.float32), b.astype(np.float32), fused=True))
     L = dense_lower(rp, ci, v)
@@ -278,10 +279,36 @@ def test_refine2_pins():
     assert np.all(x0 == 1.0) and np.all(d == 0.0) and np.all(x == 1.0)
     # interleaved right-hand sides: column k is the single-RHS result
     B = np.random.default_rng(4).uniform(-1, 1, size=(n, 3))
-    X0, D, X = (a.reshape(n, 3) for a in oracle.refine2(rp, ci, v, B.reshape(-1), rhs=3))
-    for k in range(3):
-        a0, ad, ax = oracle.refine2(rp, ci, v, B[:, k].copy())
-        assert np.array_equal(X0[:, k], a0) and np.array_equal(D[:, k], ad) and np.array_equal(X[:, k], ax)
+    for rcp in (False, True):
+        X0, D, X = (a.reshape(n, 3) for a in oracle.refine2(rp, ci, v, B.reshape(-1), rhs=3, rcp=rcp))
+        for k in range(3):
+            a0, ad, ax = oracle.refine2(rp, ci, v, B[:, k].copy(), rcp=rcp)
+            assert np.array_equal(X0[:, k], a0) and np.array_equal(D[:, k], ad) and np.array_equal(X[:, k], ax)
+
+
+def test_refine2_reciprocal_pins():
+    """The reciprocal form (rows end with a product by fl(1/diag)): with
+    power-of-two diagonals the product is the division exactly, so every
+    output equals the division form; otherwise x0 differs in the last bits
+    but the two steps still reach the dense FP64 solution."""
+    rp, ci, v = random_lower(1500, max_off=8, band=200, seed=11)
+    n = 1500
+    b = np.random.default_rng(3).uniform(-1, 1, size=n)
+    diag = rp[1:] - 1
+    v2 = v.copy()
+    v2[diag] = 2.0 ** np.random.default_rng(5).integers(-3, 4, size=n) * np.sign(v[diag])
+    for a, e in zip(oracle.refine2(rp, ci, v2, b, rcp=True), oracle.refine2(rp, ci, v2, b, rcp=False)):
+        assert np.array_equal(a, e)
+    x0, d, x = oracle.refine2(rp, ci, v, b, rcp=True)
+    x0d, _, _ = oracle.refine2(rp, ci, v, b, rcp=False)
+    assert not np.array_equal(x0, x0d)  # the rounded reciprocal is in effect
+    assert rel_maxnorm(x0.astype(np.float64), x0d.astype(np.float64)) <= 1e-5
+    L = dense_lower(rp, ci, v)
+    xd = scipy.linalg.solve_triangular(L, b, lower=True)
+    assert rel_maxnorm(x, xd) <= 1e-12
+    assert np.array_equal(x, x0.astype(np.float64) + d.astype(np.float64))
+    dd = scipy.linalg.solve_triangular(L, b - L @ x0.astype(np.float64), lower=True)
+    assert rel_maxnorm(d.astype(np.float64), dd) <= 1e-5
 
 
 def test_residual_and_datasize():
