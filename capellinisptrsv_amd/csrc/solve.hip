@@ -97,6 +97,9 @@ struct cap_plan_s {
     int *rp = nullptr, *ci = nullptr;
     void *val = nullptr;
     bool owns_matrix = false;
+    // hybrid batch: rows {first, count} that partitioned plans solve; the
+    // refinement's later steps leave them unchanged (their correction is 0)
+    std::vector<int2> skip_rows;
     // synchronisation state
     int *flags = nullptr;
     long long flags_len = 0;
@@ -168,6 +171,39 @@ struct cap_plan_s {
     struct PartState *part = nullptr;
 };
 
+// Side streams of a hybrid batch: its partitioned matrices' solves run
+// concurrently with each other (and with the rest's reset), forked from and
+// joined back into the caller's stream by events, so a batch solve stays
+// one stream-ordered operation (and can be captured into a graph).  Default
+// priority, non-blocking.
+struct Fork {
+    std::vector<hipStream_t> s;
+    std::vector<hipEvent_t> done;
+    hipEvent_t start = nullptr;
+    int make(int n)
+    {
+        if (hipEventCreateWithFlags(&start, hipEventDisableTiming) != hipSuccess) return CAP_ERR_HIP;
+        for (int i = 0; i < n; ++i) {
+            hipStream_t q = nullptr;
+            hipEvent_t e = nullptr;
+            if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) return CAP_ERR_HIP;
+            s.push_back(q);
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CAP_ERR_HIP;
+            done.push_back(e);
+        }
+        return CAP_OK;
+    }
+    void release()
+    {
+        for (hipStream_t q : s) (void)hipStreamDestroy(q);
+        for (hipEvent_t e : done) (void)hipEventDestroy(e);
+        if (start) (void)hipEventDestroy(start);
+        s.clear();
+        done.clear();
+        start = nullptr;
+    }
+};
+
 struct cap_batch_s {
     cap_plan plan = nullptr;
     int k = 1;
@@ -176,9 +212,40 @@ struct cap_batch_s {
     // (first rows part_lo), the rest by `plan` over a schedule without them
     std::vector<cap_plan> parts;
     std::vector<int> part_lo;
+    Fork fork;  // the partitioned solves' side streams
 };
 
 namespace {
+
+// The partitioned matrices' solves (each writes only its own rows), part h
+// on side stream h mod ns, forked from st (they start after what st holds
+// so far) and joined back into it (what st gets later waits for them);
+// without side streams they run on st in turn.  Whatever st gets between
+// the fork and the join runs alongside them.
+template <typename Solve>
+int fork_parts(size_t count, const Fork *fk, hipStream_t st, Solve solve)
+{
+    const size_t ns = fk ? fk->s.size() : 0;
+    if (ns == 0) {
+        for (size_t h = 0; h < count; ++h) CAP_TRY(solve(h, st));
+        return CAP_OK;
+    }
+    CAP_HIP_TRY(hipEventRecord(fk->start, st));
+    for (size_t i = 0; i < ns; ++i) CAP_HIP_TRY(hipStreamWaitEvent(fk->s[i], fk->start, 0));
+    for (size_t h = 0; h < count; ++h) CAP_TRY(solve(h, fk->s[h % ns]));
+    return CAP_OK;
+}
+
+int join_parts(const Fork *fk, hipStream_t st)
+{
+    const size_t ns = fk ? fk->s.size() : 0;
+    for (size_t i = 0; i < ns; ++i) {
+        CAP_HIP_TRY(hipEventRecord(fk->done[i], fk->s[i]));
+        CAP_HIP_TRY(hipStreamWaitEvent(st, fk->done[i], 0));
+    }
+    return CAP_OK;
+}
+
 
 // ------------------------------------------------------------ device check
 int check_device(int *dev)
@@ -2002,6 +2069,10 @@ int refine_step(cap_plan p, const double *b, double *x, int rhs, bool first, hip
     const int algo = p->algo == CAP_ALGO_LOCAL || p->algo == CAP_ALGO_PARTITION ? CAP_ALGO_LOCAL : CAP_ALGO_WRTFST;
     CAP_TRY(launch_prepare<float>(p, algo, CAP_SYNC_VALUE, p->d32, len, st));
     CAP_TRY(launch_kernel<float>(p, algo, CAP_SYNC_VALUE, p->val32, p->r32, p->d32, rhs, st, true));
+    for (const int2 &sk : p->skip_rows) {  // rows no schedule block covers: no correction
+        const long long c = (long long)sk.y * rhs;
+        k_fill<unsigned><<<blocks_for(c), kBlock, 0, st>>>((unsigned *)p->d32 + (long long)sk.x * rhs, c, 0u);
+    }
     k_accumulate<<<stream_blocks(p, len), kBlock, 0, st>>>(x, p->d32, len, p->norms, first ? 1 : 0);
     CAP_HIP_TRY(hipGetLastError());
     return CAP_OK;
@@ -2330,20 +2401,41 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std
 // the sentinel.  (On a second stream, overlapping the rest's persistent
 // kernel, the partitioned solves slowed its chain more than they saved:
 // 256 x 16 batch 11.5 against 10.9 ms.)
-int batch_solve_parts(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k,
-                      const void *b, void *x, hipStream_t st)
+int plan_reset_with(cap_plan p, int algo, int sync, void *x, int rhs, hipStream_t st)
 {
+    return p->vt == CAP_F64 ? reset_t<double>(p, algo, sync, x, rhs, st) : reset_t<float>(p, algo, sync, x, rhs, st);
+}
+
+int plan_launch_with(cap_plan p, int algo, int sync, const void *b, void *x, int rhs, hipStream_t st)
+{
+    return p->vt == CAP_F64
+               ? launch_kernel<double>(p, algo, sync, (const double *)p->val, (const double *)b, (double *)x, rhs, st)
+               : launch_kernel<float>(p, algo, sync, (const float *)p->val, (const float *)b, (float *)x, rhs, st);
+}
+
+// One solve of a batch on st: the partitioned matrices on the side streams
+// alongside the rest's solve on st.  The rest's reset and kernel touch only
+// the rest's rows of x and its own readiness state, except that a reset
+// with readiness in x fills every row of x: then it runs before the fork.
+int batch_solve_parts(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k,
+                      const void *b, void *x, hipStream_t st, const Fork *fk)
+{
+    if (parts.empty()) return plan_solve_with(p, p->algo, p->sync, b, x, k, st);
     const size_t row_bytes = (size_t)k * p->vbytes;
-    const bool rest_first = !p->staged && p->algo != CAP_ALGO_LOCAL;
-    if (rest_first) CAP_TRY(plan_solve_with(p, p->algo, p->sync, b, x, k, st));
-    for (size_t h = 0; h < parts.size(); ++h)
-        CAP_TRY(plan_solve_with(parts[h], parts[h]->algo, parts[h]->sync, (const char *)b + lo[h] * row_bytes,
-                                (char *)x + lo[h] * row_bytes, k, st));
-    return rest_first ? CAP_OK : plan_solve_with(p, p->algo, p->sync, b, x, k, st);
+    const bool x_reset = !p->staged && p->algo != CAP_ALGO_LOCAL;
+    if (x_reset) CAP_TRY(plan_reset_with(p, p->algo, p->sync, x, k, st));
+    int rc = fork_parts(parts.size(), fk, st, [&](size_t h, hipStream_t s) {
+        return plan_solve_with(parts[h], parts[h]->algo, parts[h]->sync, (const char *)b + lo[h] * row_bytes,
+                               (char *)x + lo[h] * row_bytes, k, s);
+    });
+    if (rc == CAP_OK && !x_reset) rc = plan_reset_with(p, p->algo, p->sync, x, k, st);
+    if (rc == CAP_OK) rc = plan_launch_with(p, p->algo, p->sync, b, x, k, st);
+    const int rj = join_parts(fk, st);  // joined whatever happened
+    return rc != CAP_OK ? rc : rj;
 }
 
 float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vector<int> &lo, int k, const void *b,
-                 void *x, hipStream_t st, int *rc)
+                 void *x, hipStream_t st, int *rc, const Fork *fk = nullptr)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 1e30f;
@@ -2352,7 +2444,7 @@ float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vect
     } else {
         for (int r = 0; r < 3 && *rc == CAP_OK; ++r) {
             (void)hipEventRecord(e0, st);
-            *rc = batch_solve_parts(p, parts, lo, k, b, x, st);
+            *rc = batch_solve_parts(p, parts, lo, k, b, x, st, fk);
             (void)hipEventRecord(e1, st);
             (void)hipEventSynchronize(e1);
             float t = 0;
@@ -2384,8 +2476,10 @@ float time_batch(cap_plan p, const std::vector<cap_plan> &parts, const std::vect
 constexpr int kHybridMax = 8;           // partitioned matrices per batch, at most
 constexpr int kHybridMinLevels = 1024;  // deep enough for a chain to matter
 
+constexpr int kSideStreams = 3;  // the partitioned solves' streams (hardware queues are few)
+
 int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int> &ro, const std::vector<int> &lv,
-                        hipStream_t st, std::vector<cap_plan> *parts, std::vector<int> *lo)
+                        hipStream_t st, std::vector<cap_plan> *parts, std::vector<int> *lo, Fork *fk)
 {
     if (p->unfused || !p->staged) return CAP_OK;
     const int n_mat = (int)ro.size() - 1;
@@ -2433,8 +2527,10 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
         cand_lo.push_back(ro[m]);
         skip[(size_t)m] = 1;
     }
+    if (rc == CAP_OK && cand.size() > 1) rc = fk->make(std::min((int)cand.size(), kSideStreams));
     if (rc != CAP_OK || cand.empty()) {
         for (cap_plan q : cand) cap_plan_destroy(q);
+        fk->release();
         return rc;
     }
     const size_t bytes = (size_t)p->n * k * p->vbytes + 8;
@@ -2468,7 +2564,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->rpl = sh.rpl;
             p->staged = sh.staged;
             if ((rc = build_batch_schedule(p, ro, lv, batch_solve_rows(p, k), st, &skip)) != CAP_OK) break;
-            const float t = time_batch(p, cand, cand_lo, k, b, x, st, &rc);
+            const float t = time_batch(p, cand, cand_lo, k, b, x, st, &rc, fk);
             if (rc != CAP_OK) break;
             if (t < best_h) {
                 best_h = t;
@@ -2509,6 +2605,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
         *lo = cand_lo;
     } else {
         for (cap_plan q : cand) cap_plan_destroy(q);
+        fk->release();
     }
     return rc;
 }
@@ -2596,8 +2693,9 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     }
     std::vector<cap_plan> parts;
     std::vector<int> part_lo;
+    Fork fork;
     if (o.algo == CAP_ALGO_AUTO &&
-        (rc = choose_batch_hybrid(p, k, o.blocks_per_cu > 0, ro, lv, st, &parts, &part_lo)) != CAP_OK) {
+        (rc = choose_batch_hybrid(p, k, o.blocks_per_cu > 0, ro, lv, st, &parts, &part_lo, &fork)) != CAP_OK) {
         cap_plan_destroy(p);
         return rc;
     }
@@ -2607,6 +2705,8 @@ int cap_batch_create(cap_batch *out, int n_mat, const int *row_off, const int *r
     bt->n_mat = n_mat;
     bt->parts = parts;
     bt->part_lo = part_lo;
+    bt->fork = fork;
+    for (size_t h = 0; h < parts.size(); ++h) p->skip_rows.push_back(make_int2(part_lo[h], parts[h]->n));
     *out = bt;
     return CAP_OK;
 }
@@ -2639,7 +2739,7 @@ int cap_batch_partitioned(cap_batch bt, int *first_rows, int capacity)
 int cap_batch_solve(cap_batch bt, const void *b, void *x, void *stream)
 {
     if (!bt || !b || !x) return CAP_ERR_ARG;
-    return batch_solve_parts(bt->plan, bt->parts, bt->part_lo, bt->k, b, x, S(stream));
+    return batch_solve_parts(bt->plan, bt->parts, bt->part_lo, bt->k, b, x, S(stream), &bt->fork);
 }
 
 int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol, int max_iter, int *iters,
@@ -2656,16 +2756,26 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
     double rel_max = 0.0;
     int rc = CAP_OK;
     const size_t row = (size_t)bt->k;
+    // tol >= kPartStableTol: the partitioned solves, concurrently on the side
+    // streams, joined before the rest's refinement, whose later steps read
+    // and update every row of x.  Running them alongside the rest's fused
+    // sweep slowed the sweep more than it saved (10.93 against 10.24 ms on
+    // 256 matrices x 16 right-hand sides), alongside its reset alone it
+    // saved nothing (10.28).
+    if (tol >= kPartStableTol && !bt->parts.empty()) {
+        rc = fork_parts(bt->parts.size(), &bt->fork, S(stream), [&](size_t h, hipStream_t s) {
+            return cap_plan_solve(bt->parts[h], b + bt->part_lo[h] * row, x + bt->part_lo[h] * row, bt->k, s);
+        });
+        const int rj = join_parts(&bt->fork, S(stream));  // joined whatever happened
+        if (rc == CAP_OK) rc = rj;
+        it_max = 1;
+    }
     // a part that does not converge still leaves its best x: the others run
-    for (size_t h = 0; h < bt->parts.size() && (rc == CAP_OK || rc == CAP_ERR_NOT_CONVERGED); ++h) {
+    for (size_t h = 0; h < bt->parts.size() && tol < kPartStableTol &&
+                       (rc == CAP_OK || rc == CAP_ERR_NOT_CONVERGED);
+         ++h) {
         const double *bh = b + bt->part_lo[h] * row;
         double *xh = x + bt->part_lo[h] * row;
-        if (tol >= kPartStableTol) {
-            const int rh = cap_plan_solve(bt->parts[h], bh, xh, bt->k, stream);
-            if (rh != CAP_OK) rc = rh;
-            it_max = std::max(it_max, 1);
-            continue;
-        }
         int it = 0;
         double rel = 0.0;
         const int rh = refined_impl(bt->parts[h], bh, xh, bt->k, tol, max_iter, &it, &rel, S(stream));
@@ -2691,6 +2801,7 @@ int cap_batch_destroy(cap_batch bt)
     if (!bt) return CAP_ERR_ARG;
     for (cap_plan q : bt->parts) cap_plan_destroy(q);
     cap_plan_destroy(bt->plan);
+    bt->fork.release();
     delete bt;
     return CAP_OK;
 }

@@ -113,11 +113,67 @@ def _hybrid_batch():
             api.generate("band_rowcount", n=40000, offdiag=9, band=129, seed=82),
             api.generate("band_rowcount", n=30000, offdiag=9, band=200, seed=83)]
     mats += [api.generate("level", n=8000, levels=4, width=2000, offdiag=6, depth=2, seed=90 + i) for i in range(20)]
+    return _concat(mats)
+
+
+def _concat(mats):
     row_off = np.cumsum([0] + [len(m[0]) - 1 for m in mats]).astype(np.int32)
     nnz_off = np.cumsum([0] + [len(m[1]) for m in mats])
     rp = np.concatenate([m[0][:-1] + o for m, o in zip(mats, nnz_off)] + [[nnz_off[-1]]]).astype(np.int32)
     return mats, row_off, rp, np.concatenate([m[1] for m in mats]).astype(np.int32), np.concatenate(
         [m[2] for m in mats])
+
+
+def _growing(n=3000, lag=200, c=1.9):
+    """x_i = b_i - c x_{i-lag}: 15 levels.  With b = L x for an x of order 1,
+    each level multiplies the error of the one before by 1.9, so an FP32
+    solve is accurate to ~1e-3 and refinement needs several steps."""
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        if i >= lag:
+            ci.append(i - lag)
+            v.append(c)
+        ci.append(i)
+        v.append(1.0)
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
+def test_hybrid_batch_refined_more_steps(cuda):
+    """A hybrid batch whose thread-per-row part needs more than two
+    refinement steps: the later steps (residual, FP32 correction, x += d over
+    the batch) must leave the partitioned matrices' rows as their FP64
+    solves left them, and every matrix ends within 1e-12."""
+    import torch
+    from capellinisptrsv_amd import Batch
+    mats = _hybrid_batch()[0] + [_growing()]
+    mats, row_off, rp, ci, v = _concat(mats)
+    k = 4
+    N = int(row_off[-1])
+    b = api.uniform(15, N * k)
+    grp, gci, gv = mats[-1]
+    rows = np.repeat(np.arange(len(grp) - 1), np.diff(grp))
+    xt = api.uniform(16, (len(grp) - 1) * k).reshape(-1, k)
+    bg = np.zeros_like(xt)
+    np.add.at(bg, rows, gv[:, None] * xt[gci])
+    b[row_off[-2] * k:] = bg.reshape(-1)
+    bt = Batch(*(torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)), k=k, algo="auto")
+    assert bt.partitioned >= 1
+    bd = torch.from_numpy(b.reshape(N, k)).to(cuda)
+    x = bt.solve(bd).cpu().numpy().reshape(-1)
+    xr, iters, rel = bt.solve_refined(bd, tol=1e-13, max_iter=10)
+    xr = xr.cpu().numpy().reshape(-1)
+    assert iters >= 3, (iters, rel)
+    part = set(bt.partitioned_matrices)
+    for m, (mrp, mci, mv) in enumerate(mats):
+        lo, hi = row_off[m] * k, row_off[m + 1] * k
+        ref = oracle.csr_forward(mrp, mci, mv, b[lo:hi], rhs=k, fused=True)
+        if m in part:
+            assert np.array_equal(xr[lo:hi], x[lo:hi]), m
+        # the growing member's condition number (~1.9^15) leaves the FP64
+        # oracle itself ~1e-12 from the exact solution
+        assert rel_maxnorm(xr[lo:hi], ref) <= (1e-10 if m == len(mats) - 1 else 1e-12), (m, iters, rel)
+    bt.close()
 
 
 @pytest.mark.parametrize("k", [1, 16])
