@@ -2093,13 +2093,15 @@ __device__ __forceinline__ long long part_vec(int p, int k, int j)
 // last u also goes to carry[g].  A step is sequential, so each product is
 // split over the columns (thread: row c, quarter kq) to have four times the
 // loads in flight; the first W threads own the vectors.  At W = 256 with
-// several right-hand sides the workgroup is 16 wavefronts; holding it to 80
-// VGPRs (6 waves per SIMD) lets it share a CU with a 1024-thread
-// thread-per-row workgroup of 42 VGPRs, so a batch's partitioned solve runs
-// beside the rest's persistent kernel instead of waiting for it (same speed
-// alone: 745 against 743 us at 16 right-hand sides, n = 160,000).
+// several right-hand sides the workgroup is 16 wavefronts; holding it to 72
+// VGPRs (7 waves per SIMD; 36 bytes per lane spill at KQ = 4) lets it share
+// a CU with a 1024-thread workgroup of the thread-per-row solve (42 VGPRs)
+// or of the fused refinement sweep (52), so a batch's partitioned solve runs
+// beside the rest's persistent kernel instead of waiting for it.  Alone it
+// is 2 % slower for that (758 against 743 us at 16 right-hand sides,
+// n = 160,000).
 template <int W, int KQ, bool NT>
-__global__ void __launch_bounds__(kPartSplit * W, W >= 256 && KQ >= 2 ? 6 : 1)
+__global__ void __launch_bounds__(kPartSplit * W, W >= 256 && KQ >= 2 ? 7 : 1)
     k_part_group_scan(const double *__restrict__ a, double *__restrict__ u, double *__restrict__ carry,
                       const double *__restrict__ bm, int gs, int q_count, int k)
 {

@@ -2156,8 +2156,12 @@ int refined_partition(cap_plan p, const double *b, double *x, int k, double tol,
     return rc;
 }
 
+// join (hybrid batches): side streams whose partitioned solves run
+// alongside the first two steps, which touch only the plan's scheduled rows;
+// they are joined into st before the convergence readback, since the later
+// steps read and update every row of x.
 int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, int max_iter, int *iters,
-                 double *last_rel, hipStream_t st)
+                 double *last_rel, hipStream_t st, const Fork *join = nullptr)
 {
     if (!p || !b || !x || rhs < 1 || rhs > kWave || max_iter < 1) return CAP_ERR_ARG;
     if (p->vt != CAP_F64 || p->reversed || p->unit_diag) return CAP_ERR_UNSUPPORTED;
@@ -2211,6 +2215,7 @@ int refined_impl(cap_plan p, const double *b, double *x, int rhs, double tol, in
     unsigned long long nb[2];
     int rc = CAP_OK;
     CAP_HIP_TRY(hipGraphLaunch(p->refine_graph, st));
+    if (join) CAP_TRY(join_parts(join, st));
     it = 2;
     while (true) {
         CAP_HIP_TRY(hipMemcpyAsync(nb, p->norms, sizeof(nb), hipMemcpyDeviceToHost, st));
@@ -2756,18 +2761,17 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
     double rel_max = 0.0;
     int rc = CAP_OK;
     const size_t row = (size_t)bt->k;
-    // tol >= kPartStableTol: the partitioned solves, concurrently on the side
-    // streams, joined before the rest's refinement, whose later steps read
-    // and update every row of x.  Running them alongside the rest's fused
-    // sweep slowed the sweep more than it saved (10.93 against 10.24 ms on
-    // 256 matrices x 16 right-hand sides), alongside its reset alone it
-    // saved nothing (10.28).
-    if (tol >= kPartStableTol && !bt->parts.empty()) {
+    // tol >= kPartStableTol: the partitioned solves on the side streams,
+    // enqueued first, alongside the rest's first two steps (joined inside
+    // refined_impl).  Enqueued after the sweep's launch instead, so that
+    // they start with it rather than ahead of it, the batch took 10.31 ms
+    // against 9.95 (256 matrices x 16 right-hand sides): the sweep loses
+    // less beside their late, lighter kernels.
+    const bool overlap = tol >= kPartStableTol && !bt->parts.empty();
+    if (overlap) {
         rc = fork_parts(bt->parts.size(), &bt->fork, S(stream), [&](size_t h, hipStream_t s) {
             return cap_plan_solve(bt->parts[h], b + bt->part_lo[h] * row, x + bt->part_lo[h] * row, bt->k, s);
         });
-        const int rj = join_parts(&bt->fork, S(stream));  // joined whatever happened
-        if (rc == CAP_OK) rc = rj;
         it_max = 1;
     }
     // a part that does not converge still leaves its best x: the others run
@@ -2786,10 +2790,15 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
     if (rc == CAP_OK || rc == CAP_ERR_NOT_CONVERGED) {
         int it = 0;
         double rel = 0.0;
-        const int rc2 = refined_impl(bt->plan, b, x, bt->k, tol, max_iter, &it, &rel, S(stream));
+        const int rc2 = refined_impl(bt->plan, b, x, bt->k, tol, max_iter, &it, &rel, S(stream),
+                                     overlap ? &bt->fork : nullptr);
         it_max = std::max(it_max, it);
         rel_max = std::max(rel_max, rel);
         if (rc == CAP_OK || (rc2 != CAP_OK && rc2 != CAP_ERR_NOT_CONVERGED)) rc = rc2;
+    }
+    if (overlap) {  // joined already unless an error came first; joining twice is harmless
+        const int rj = join_parts(&bt->fork, S(stream));
+        if (rc == CAP_OK) rc = rj;
     }
     if (iters) *iters = it_max;
     if (last_rel) *last_rel = rel_max;
