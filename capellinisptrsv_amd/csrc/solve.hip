@@ -2751,12 +2751,13 @@ int cap_batch_solve_refined(cap_batch bt, const double *b, double *x, double tol
                             double *last_rel, void *stream)
 {
     if (!bt || !b || !x) return CAP_ERR_ARG;
-    // The partitioned matrices first, then the rest; iterations and
-    // correction are the largest.  A partitioned plan's tables are FP64 and
-    // its creation checked its solves against a sequential FP64 solve to
-    // kPartStableTol, so at tol >= kPartStableTol its one FP64 solve is the
-    // answer (no FP32 step to refine; a correction would only re-solve the
-    // rounding noise).  A tighter tol refines it in FP64.
+    // Iterations and correction are the largest over the parts.  A
+    // partitioned plan's tables are FP64 and its creation checked its solves
+    // against a sequential FP64 solve to kPartStableTol, so at tol >=
+    // kPartStableTol its one FP64 solve is the answer (no FP32 step to
+    // refine; a correction would only re-solve the rounding noise).  A
+    // tighter tol refines each of them in FP64, one after another, before the
+    // rest.
     int it_max = 0;
     double rel_max = 0.0;
     int rc = CAP_OK;
