@@ -26,9 +26,24 @@ namespace {
 
 // Threshold of the mix selector for 64-row chunks: a chunk whose mean
 // nnz/row reaches it is solved wavefront-per-row.  The reference hard-codes
-// 10 for 32-lane warps (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:188); this
-// default is the value measured best on MI355X for the banded workloads.
-constexpr int kMixBorderDefault = 16;
+// 10 for 32-lane warps (CUDA/mix/spts_syncfree_cuda_csr_wrt.h:188).  Here it
+// follows the matrix's rows per level.  Measured on MI355X (n = 200,000; mix
+// time with every chunk wavefront-per-row over every chunk thread-per-row):
+//   level matrices (dependencies in the previous two levels), 250 to 4,000
+//   rows per level: 1.13 at 3 nnz/row (4,000 rows per level); 0.47-0.73 at
+//   5; 0.28-0.52 at 9; 0.17-0.32 at 17; 0.15 at 64;
+//   bands of 1,000 (dependencies among the previous 1,000 rows), 11 to 136
+//   rows per level: 1.02, 1.05, 1.06, 1.01 at 4, 7, 13, 25 nnz/row; 0.91 at 49.
+// So chunks of 4 or more nnz/row go wavefront-per-row where levels hold at
+// least 200 rows; below that the chain sets the time either way, and only
+// rows of about 48 entries gain from a wavefront.
+constexpr int kMixWideLevel = 200;    // rows per level from which levels count as wide
+constexpr int kMixBorderWide = 4;     // threshold where they are
+constexpr int kMixBorderNarrow = 48;  // threshold where they are not
+inline int mix_border_for(int n, int levels)
+{
+    return (long long)n >= (long long)kMixWideLevel * std::max(1, levels) ? kMixBorderWide : kMixBorderNarrow;
+}
 // A wait gives up after this much wall time without progress (10 ns ticks of
 // the real-time counter): far above any legitimate wait -- a lane waits only
 // for rows the persistent grid already holds, tens of milliseconds at most
@@ -92,7 +107,7 @@ struct RhsPick {
 
 struct cap_plan_s {
     int n = 0, nnz = 0, vt = CAP_F64, vbytes = 8;
-    int algo = CAP_ALGO_WRTFST, sync = CAP_SYNC_VALUE, border = kMixBorderDefault;
+    int algo = CAP_ALGO_WRTFST, sync = CAP_SYNC_VALUE, border = kMixBorderWide;
     int device = 0;
     int *rp = nullptr, *ci = nullptr;
     void *val = nullptr;
@@ -1468,8 +1483,10 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
                 cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
         }
     }
-    for (int bd : {8, 16, 32})
-        if (avg * 4 >= bd) {
+    // thresholds 4 and 48 (see mix_border_for) always; 16 where the mean row
+    // length is at least 4 (some chunks may then reach it)
+    for (int bd : {kMixBorderWide, 16, kMixBorderNarrow})
+        if (bd != 16 || avg * 4 >= bd) {
             cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg, B});
             cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 2, 0, stg, B});
             cands.push_back({CAP_ALGO_MIX, CAP_SYNC_VALUE, bd, 1, 0, stg, B, 1});
@@ -1685,9 +1702,18 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
         if (o.adjoint == CAP_ADJOINT_SCATTER || o.adjoint == CAP_ADJOINT_GATHER) p->adjoint_mode = o.adjoint;
         p->algo = o.algo;
         p->sync = o.sync ? o.sync : CAP_SYNC_VALUE;
-        const int border = o.border > 0 ? o.border : kMixBorderDefault;
         if (p->algo == CAP_ALGO_MIX) {
-            rc = build_mix(p, rp_host, border);
+            int border = o.border;
+            if (border <= 0) {  // from the matrix's rows per level
+                std::vector<int> ci_host((size_t)p->nnz);
+                if (p->nnz > 0 && hipMemcpy(ci_host.data(), p->ci, sizeof(int) * (size_t)p->nnz,
+                                            hipMemcpyDeviceToHost) != hipSuccess)
+                    rc = CAP_ERR_HIP;
+                const int levels =
+                    p->n ? cap_matrix_layer(p->n, rp_host.data(), ci_host.data(), nullptr, nullptr, nullptr) : 1;
+                border = mix_border_for(p->n, levels);
+            }
+            if (rc == CAP_OK) rc = build_mix(p, rp_host, border);
         } else if (p->algo == CAP_ALGO_CSC) {
             rc = ensure_csc(p, st);
         } else if (p->algo == CAP_ALGO_AUTO) {
@@ -2879,8 +2905,10 @@ int bench_csr(int algo, const int *rp, const int *ci, const T *val, int m, int n
         int len = 0;
         double wo = 0, eo = 0;
         auto t0 = std::chrono::steady_clock::now();
+        const int levels = m ? cap_matrix_layer(m, rp, ci, nullptr, nullptr, nullptr) : 1;
+        o.border = mix_border_for(m, levels);
         for (int r = 0; r < CAP_BENCH_REPEAT; ++r)
-            cap_matrix_warp4(m, nnzL, rp, kMixBorderDefault, kWave, &len, wn.data(), &wo, &eo);
+            cap_matrix_warp4(m, nnzL, rp, o.border, kWave, &len, wn.data(), &wo, &eo);
         auto t1 = std::chrono::steady_clock::now();
         pre_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / CAP_BENCH_REPEAT;
     }

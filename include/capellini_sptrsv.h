@@ -116,7 +116,10 @@ extern "C" {
 typedef struct cap_opts {
     int algo;         /* CAP_ALGO_* (0 = auto) */
     int sync;         /* CAP_SYNC_* (0 = auto) */
-    int border;       /* mix: nnz/row threshold for wavefront mode (0 = tuned) */
+    int border;       /* mix: nnz/row threshold for wavefront mode; 0 = from
+                         the matrix's rows per level (4 where levels hold at
+                         least 200 rows, else 48), timed among 4, 16 and 48
+                         by algo AUTO */
     int validate;     /* 1 = check the CSR layout before the first solve */
     int host_inputs;  /* 1 = row_ptr/col_idx/val are host pointers (copied) */
     int spin_limit;   /* polls without progress before a wait gives up

@@ -34,7 +34,9 @@ def test_csr_wrappers(cuda, dtype):
         assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize) / (1e6 * t)) <= 1e-9 * bw
         assert abs(gf - 2 * len(ci) / (1e6 * t)) <= 1e-9 * gf
     ret, t, gf, bw, pre, wo, eo = api.spts_syncfree_hip_mix(rp, ci, v, n, n, len(ci))
-    ln, _, wo_ref, eo_ref = oracle.matrix_warp4(n, len(ci), rp, 16, 64)
+    # the chunk threshold follows rows per level: 4 from 200 rows per level, else 48
+    levels = oracle.matrix_layer(n, rp, ci)[0]
+    ln, _, wo_ref, eo_ref = oracle.matrix_warp4(n, len(ci), rp, 4 if n >= 200 * levels else 48, 64)
     assert ret == 0 and t > 0 and pre >= 0 and wo == wo_ref and eo == eo_ref
     assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize, ln) / (1e6 * t)) <= 1e-9 * bw
 

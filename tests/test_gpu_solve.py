@@ -98,6 +98,23 @@ def test_mix(cuda, border, sync, name, mat):
     assert np.array_equal(x.view(np.uint64), ref.view(np.uint64)), rel_maxnorm(x, ref)
 
 
+@pytest.mark.parametrize("name,mat", [MATS[2], MATS[3], MATS[4]], ids=["cfg0", "cfg1_small", "cfg2_small"])
+def test_mix_threshold_from_rows_per_level(cuda, name, mat):
+    """mix without a border takes its chunk threshold from the matrix's rows
+    per level: 4 where levels hold at least 200 rows, else 48; the solve
+    stays bit-exact and the chunk table is the oracle's at that border."""
+    rp, ci, v = mat
+    n = len(rp) - 1
+    levels = oracle.matrix_layer(n, rp, ci)[0]
+    want = 4 if n >= 200 * levels else 48
+    b = api.uniform(9, n)
+    x, info = _solve(cuda, rp, ci, v, b, "mix", "value", border=0)
+    assert info["border"] == want, (name, n / levels)
+    ln, _, wo, _ = oracle.matrix_warp4(n, len(ci), rp, want, 64)
+    assert info["mix_len"] == ln and abs(info["warp_occupy"] - wo) < 1e-15
+    assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, fused=True))
+
+
 @pytest.mark.parametrize("name,mat", MATS, ids=[m[0] for m in MATS])
 def test_csc_syncfree(cuda, name, mat):
     rp, ci, v = mat
