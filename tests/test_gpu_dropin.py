@@ -86,6 +86,11 @@ def test_driver_appends_result_csv(cuda, tmp_path):
         assert f[0] == str(mtx) and int(f[1]) == m and int(f[2]) == nnzL and int(f[4]) == layer
         assert f[3] == f"{nnzL / m:4.2f}" and f[5] == f"{par:4.2f}"
         assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
+    # the mix row's occupancies: the chunk threshold follows rows per level
+    # (4 from 200 rows per level, else 48)
+    _, _, wo, eo = oracle.matrix_warp4(m, nnzL, rpL, 4 if m >= 200 * layer else 48, 64)
+    f = [c.strip() for c in rows[3].split(",")]
+    assert f[10] == f"{wo:4.2f}" and f[11] == f"{eo:4.2f}", (f[10:], wo, eo)
     # result.jsonl: one object per run; x against the FP64 serial solve of the
     # same L and b (FP64 kernels: order kept, fused terms, so within 1e-12;
     # the FP32 run within the FP32 bar), roofline figures from the wrapper
