@@ -35,11 +35,12 @@ namespace {
 //   bands of 1,000 (dependencies among the previous 1,000 rows), 11 to 136
 //   rows per level: 1.02, 1.05, 1.06, 1.01 at 4, 7, 13, 25 nnz/row; 0.91 at 49.
 // So chunks of 4 or more nnz/row go wavefront-per-row where levels hold at
-// least 200 rows; below that the chain sets the time either way, and only
-// rows of about 48 entries gain from a wavefront.
+// least 200 rows; below that the chain sets the time either way, and rows
+// gain from a wavefront from about 32 entries on (between the 1.01 at 25 and
+// the 0.91 at 49).
 constexpr int kMixWideLevel = 200;    // rows per level from which levels count as wide
 constexpr int kMixBorderWide = 4;     // threshold where they are
-constexpr int kMixBorderNarrow = 48;  // threshold where they are not
+constexpr int kMixBorderNarrow = 32;  // threshold where they are not
 inline int mix_border_for(int n, int levels)
 {
     return (long long)n >= (long long)kMixWideLevel * std::max(1, levels) ? kMixBorderWide : kMixBorderNarrow;
@@ -1483,7 +1484,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
                 cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
         }
     }
-    // thresholds 4 and 48 (see mix_border_for) always; 16 where the mean row
+    // thresholds 4 and 32 (see mix_border_for) always; 16 where the mean row
     // length is at least 4 (some chunks may then reach it)
     for (int bd : {kMixBorderWide, 16, kMixBorderNarrow})
         if (bd != 16 || avg * 4 >= bd) {

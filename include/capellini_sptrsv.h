@@ -118,7 +118,7 @@ typedef struct cap_opts {
     int sync;         /* CAP_SYNC_* (0 = auto) */
     int border;       /* mix: nnz/row threshold for wavefront mode; 0 = from
                          the matrix's rows per level (4 where levels hold at
-                         least 200 rows, else 48), timed among 4, 16 and 48
+                         least 200 rows, else 32), timed among 4, 16 and 32
                          by algo AUTO */
     int validate;     /* 1 = check the CSR layout before the first solve */
     int host_inputs;  /* 1 = row_ptr/col_idx/val are host pointers (copied) */

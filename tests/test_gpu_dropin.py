@@ -34,9 +34,9 @@ def test_csr_wrappers(cuda, dtype):
         assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize) / (1e6 * t)) <= 1e-9 * bw
         assert abs(gf - 2 * len(ci) / (1e6 * t)) <= 1e-9 * gf
     ret, t, gf, bw, pre, wo, eo = api.spts_syncfree_hip_mix(rp, ci, v, n, n, len(ci))
-    # the chunk threshold follows rows per level: 4 from 200 rows per level, else 48
+    # the chunk threshold follows rows per level: 4 from 200 rows per level, else 32
     levels = oracle.matrix_layer(n, rp, ci)[0]
-    ln, _, wo_ref, eo_ref = oracle.matrix_warp4(n, len(ci), rp, 4 if n >= 200 * levels else 48, 64)
+    ln, _, wo_ref, eo_ref = oracle.matrix_warp4(n, len(ci), rp, 4 if n >= 200 * levels else 32, 64)
     assert ret == 0 and t > 0 and pre >= 0 and wo == wo_ref and eo == eo_ref
     assert abs(bw - oracle.datasize(n, n, len(ci), v.itemsize, ln) / (1e6 * t)) <= 1e-9 * bw
 
@@ -87,8 +87,8 @@ def test_driver_appends_result_csv(cuda, tmp_path):
         assert f[3] == f"{nnzL / m:4.2f}" and f[5] == f"{par:4.2f}"
         assert float(f[7]) >= 0 and f[7] != "-1.00"  # -1 would mean the validation failed
     # the mix row's occupancies: the chunk threshold follows rows per level
-    # (4 from 200 rows per level, else 48)
-    _, _, wo, eo = oracle.matrix_warp4(m, nnzL, rpL, 4 if m >= 200 * layer else 48, 64)
+    # (4 from 200 rows per level, else 32)
+    _, _, wo, eo = oracle.matrix_warp4(m, nnzL, rpL, 4 if m >= 200 * layer else 32, 64)
     f = [c.strip() for c in rows[3].split(",")]
     assert f[10] == f"{wo:4.2f}" and f[11] == f"{eo:4.2f}", (f[10:], wo, eo)
     # result.jsonl: one object per run; x against the FP64 serial solve of the

@@ -101,12 +101,12 @@ def test_mix(cuda, border, sync, name, mat):
 @pytest.mark.parametrize("name,mat", [MATS[2], MATS[3], MATS[4]], ids=["cfg0", "cfg1_small", "cfg2_small"])
 def test_mix_threshold_from_rows_per_level(cuda, name, mat):
     """mix without a border takes its chunk threshold from the matrix's rows
-    per level: 4 where levels hold at least 200 rows, else 48; the solve
+    per level: 4 where levels hold at least 200 rows, else 32; the solve
     stays bit-exact and the chunk table is the oracle's at that border."""
     rp, ci, v = mat
     n = len(rp) - 1
     levels = oracle.matrix_layer(n, rp, ci)[0]
-    want = 4 if n >= 200 * levels else 48
+    want = 4 if n >= 200 * levels else 32
     b = api.uniform(9, n)
     x, info = _solve(cuda, rp, ci, v, b, "mix", "value", border=0)
     assert info["border"] == want, (name, n / levels)
