@@ -81,10 +81,10 @@ SIGNATURES = {
     "cap_plan_launch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "cap_plan_info": (c_int, [c_void_p, PI, PI, PI, PI, PD, PD]),
     "cap_plan_poll_count": (c_int, [c_void_p, POINTER(c_ulonglong), c_void_p]),
-    "cap_plan_rhs_block": (c_int, [c_void_p, c_int, PI, PI, PI, PI, PI]),
+    "cap_plan_rhs_block": (c_int, [c_void_p, c_int, PI, PI, PI, PI, PI, PI]),
     "cap_plan_tuning": (c_int, [c_void_p, PI, PI, PI, PI, PI, PI]),
     "cap_batch_info": (c_int, [c_void_p, PI, PI]),
-    "cap_batch_tuning": (c_int, [c_void_p, PI, PI, PI]),
+    "cap_batch_tuning": (c_int, [c_void_p, PI, PI, PI, PI]),
     "cap_batch_partitioned": (c_int, [c_void_p, PI, c_int]),
     "cap_plan_destroy": (c_int, [c_void_p]),
     "cap_plan_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

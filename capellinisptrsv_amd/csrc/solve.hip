@@ -104,6 +104,7 @@ struct RhsPick {
     int bpc = 0, block = 0, kr = 0, rpl = 0;
     int staged = -1;  // readiness: 1 staged, 0 in x itself, -1 the plan's
     int lds = -1;     // LDS window: 0 off (the 1024-thread shapes), -1 the plan's
+    int tile = -1;    // row tiles: 1 on, 0 off, -1 the plan's
 };
 
 struct cap_plan_s {
@@ -720,10 +721,11 @@ inline const int2 *sched_for(cap_plan p, int rows, int *n_sched)
 // of one launch (and puts the plan's own settings back).
 struct RhsScope {
     cap_plan p;
-    int bpc, block, kr, rpl, lds;
+    int bpc, block, kr, rpl, lds, tile;
     bool staged;
     RhsScope(cap_plan q, int k)
-        : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl), lds(q->lds), staged(q->staged)
+        : p(q), bpc(q->bpc), block(q->wblock), kr(q->part_kr), rpl(q->rpl), lds(q->lds), tile(q->tile),
+          staged(q->staged)
     {
         const auto it = k > 1 ? p->rhs_pick.find(k) : p->rhs_pick.end();
         if (it == p->rhs_pick.end()) {
@@ -740,6 +742,7 @@ struct RhsScope {
         if (it->second.rpl) p->rpl = it->second.rpl;
         if (it->second.staged >= 0) p->staged = it->second.staged != 0;
         if (it->second.lds >= 0) p->lds = it->second.lds;
+        if (it->second.tile >= 0) p->tile = it->second.tile;
     }
     ~RhsScope()
     {
@@ -748,6 +751,7 @@ struct RhsScope {
         p->part_kr = kr;
         p->rpl = rpl;
         p->lds = lds;
+        p->tile = tile;
         p->staged = staged;
     }
 };
@@ -805,7 +809,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const bool small = p->wblock == kBlockSmall;
         if (p->wblock == kBlockLarge) {  // the multi-RHS pick only
             if (sync != CAP_SYNC_VALUE || p->lds || p->count_polls || rhs < 2) return CAP_ERR_UNSUPPORTED;
-            go(k_wrtfst<T, 1, 0, kBlockLarge>);
+            p->tile ? go(k_wrtfst<T, 1, 0, kBlockLarge, false, true>) : go(k_wrtfst<T, 1, 0, kBlockLarge>);
         } else if (sync == CAP_SYNC_VALUE && p->count_polls) {  // the counting instantiations
             sc.polls = reinterpret_cast<unsigned long long *>(p->ctrl + 1024);
             if (p->lds)
@@ -1064,10 +1068,11 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         // the rows per ticket); these run without the LDS window whatever the
         // plan's single-RHS pick
         if (sync == CAP_SYNC_VALUE && !p->count_polls) {
-            cands.push_back({1, kBlockLarge, 0, 1, -1, 0});
+            cands.push_back({1, kBlockLarge, 0, 1, -1, 0, 0});
+            cands.push_back({1, kBlockLarge, 0, 1, -1, 0, 1});  // the ticket's entries staged in LDS
             for (int r : {2, 4, 8})
-                if (r <= k && (r < 8 || p->vt == CAP_F32)) cands.push_back({1, kBlockLarge, 0, r, -1, 0});
-            if (k >= 4) cands.push_back({1, kBlock, 0, 4, -1, 0});
+                if (r <= k && (r < 8 || p->vt == CAP_F32)) cands.push_back({1, kBlockLarge, 0, r, -1, 0, 0});
+            if (k >= 4) cands.push_back({1, kBlock, 0, 4, -1, 0, 0});
         }
     } else if (algo == CAP_ALGO_PARTITION) {
         const int dflt = part_chunk(k, std::min(kPartBlockRhs, std::max(1, k / 2)));
@@ -1078,7 +1083,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->rhs_pick[k] = RhsPick{};
         return CAP_OK;
     }
-    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr, rpl0 = p->rpl, lds0 = p->lds;
+    const int bpc0 = p->bpc, blk0 = p->wblock, kr0 = p->part_kr, rpl0 = p->rpl, lds0 = p->lds, tile0 = p->tile;
     float best = 1e30f;
     RhsPick pick = cands[0];
     int rc = CAP_OK;
@@ -1087,6 +1092,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         if (cands[c].block) p->wblock = cands[c].block;
         if (cands[c].rpl) p->rpl = cands[c].rpl;
         if (cands[c].lds >= 0) p->lds = cands[c].lds;
+        if (cands[c].tile >= 0) p->tile = cands[c].tile;
         p->part_kr = cands[c].kr;
         float t = 0;
         rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
@@ -1099,6 +1105,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->part_kr = kr0;
         p->rpl = rpl0;
         p->lds = lds0;
+        p->tile = tile0;
     }
     // readiness in x itself for the pick: one store per value instead of
     // two (x and the staged copy), at k > 1 the larger share of the bytes
@@ -1107,6 +1114,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         if (pick.block) p->wblock = pick.block;
         if (pick.rpl) p->rpl = pick.rpl;
         if (pick.lds >= 0) p->lds = pick.lds;
+        if (pick.tile >= 0) p->tile = pick.tile;
         p->staged = false;
         float t = 0;
         rc = time_candidate(p, algo, sync, b, x, k, st, best, &t);
@@ -1115,6 +1123,7 @@ int tune_rhs_block(cap_plan p, const void *b, void *x, int k, hipStream_t st)
         p->wblock = blk0;
         p->rpl = rpl0;
         p->lds = lds0;
+        p->tile = tile0;
         if (rc == CAP_OK && t < 0.98f * best) pick.staged = 0;
     }
     if (rc != CAP_OK) return rc;
@@ -1919,11 +1928,18 @@ int cap_plan_poll_count(cap_plan p, unsigned long long *polls, void *stream)
 }
 
 int cap_plan_rhs_block(cap_plan p, int k, int *blocks_per_cu, int *row_threads, int *rhs_per_wave,
-                       int *rhs_per_lane, int *readiness)
+                       int *rhs_per_lane, int *readiness, int *row_tiles)
 {
     if (!p || k < 1) return CAP_ERR_ARG;
     const auto it = p->rhs_pick.find(k);
     if (it == p->rhs_pick.end()) return CAP_ERR_UNSUPPORTED;
+    if (row_tiles) {
+        const int blk = it->second.block ? it->second.block : p->wblock;
+        const int rpl = it->second.rpl ? it->second.rpl : p->rpl;
+        const bool tiled = it->second.tile >= 0 ? it->second.tile != 0 : p->tile != 0;
+        // k_wrtfst_r (several right-hand sides per lane) has no tiled form
+        *row_tiles = algo_for(p, k) == CAP_ALGO_WRTFST && tiled && !(blk == kBlockLarge && rpl > 1);
+    }
     if (blocks_per_cu) *blocks_per_cu = it->second.bpc ? it->second.bpc : p->bpc;
     if (row_threads) *row_threads = it->second.block ? it->second.block : p->wblock;
     if (rhs_per_wave) *rhs_per_wave = it->second.kr;
@@ -2390,28 +2406,33 @@ int choose_batch_algo(cap_plan p, int requested, int bpc_fixed, int k, const std
     // more rows in each ticket (one atomic on one counter, ~12 ns each),
     // each with a schedule of its own block size
     if (rc == CAP_OK && k > 1 && !bpc_fixed && p->sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls) {
-        int best_r = 0;
-        for (int r : {1, 2, 4}) {
+        int best_r = 0, best_tile = 0;
+        for (int rt : {1, -1, 2, 4}) {  // R per lane; -1: one per lane with row tiles
+            const int r = rt < 0 ? 1 : rt, tile = rt < 0 ? 1 : 0;
             if (rc != CAP_OK || r > k) break;
-            const int keep_bpc = p->bpc, keep_blk = p->wblock, keep_rpl = p->rpl;
+            const int keep_bpc = p->bpc, keep_blk = p->wblock, keep_rpl = p->rpl, keep_tile = p->tile;
             p->bpc = 1;
             p->wblock = kBlockLarge;
             p->rpl = r;
+            p->tile = tile;
             rc = build_batch_schedule(p, ro, lv, kBlockLarge >> lpr_log2_for((k + r - 1) / r), st);
             float t = 1e30f;
             if (rc == CAP_OK) rc = time_candidate(p, CAP_ALGO_WRTFST, p->sync, b, x, k, st, t_w, &t);
             p->bpc = keep_bpc;
             p->wblock = keep_blk;
             p->rpl = keep_rpl;
+            p->tile = keep_tile;
             if (t < 0.98f * t_w) {
                 t_w = t;
                 best_r = r;
+                best_tile = tile;
             }
         }
         if (best_r) {
             p->bpc = 1;
             p->wblock = kBlockLarge;
             p->rpl = best_r;
+            p->tile = best_tile;
             p->rblock = kBlockLarge;  // the refinement sweep's tickets too
             rc = build_batch_schedule(p, ro, lv, batch_refine_rows(p, k), st);
         }
@@ -2573,16 +2594,20 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
     struct Shape {
         int bpc, block, rpl;
         bool staged;
+        int tile;
     };
-    std::vector<Shape> shapes = {{1, kBlock, 1, true}, {2, kBlock, 1, true}, {4, kBlock, 1, true}};
+    std::vector<Shape> shapes = {{1, kBlock, 1, true, 0}, {2, kBlock, 1, true, 0}, {4, kBlock, 1, true, 0}};
     if (k > 1 && p->sync == CAP_SYNC_VALUE && !p->lds && !p->count_polls)  // the multi-RHS shapes
         for (int r : {1, 2, 4})
             if (r <= k)
-                for (bool sg : {true, false}) shapes.push_back({1, kBlockLarge, r, sg});
-    if (bpc_fixed) shapes = {{p->bpc, p->wblock, p->rpl, true}};  // the caller's grid stays
-    const Shape plain{p->bpc, p->wblock, p->rpl, p->staged != 0};
+                for (bool sg : {true, false}) {
+                    shapes.push_back({1, kBlockLarge, r, sg, 0});
+                    if (r == 1) shapes.push_back({1, kBlockLarge, r, sg, 1});  // with row tiles
+                }
+    if (bpc_fixed) shapes = {{p->bpc, p->wblock, p->rpl, true, p->tile}};  // the caller's grid stays
+    const Shape plain{p->bpc, p->wblock, p->rpl, p->staged != 0, p->tile};
     const int plain_algo = p->algo;
-    Shape pick{0, 0, 0, true};
+    Shape pick{0, 0, 0, true, 0};
     float best = 1e30f;
     std::map<int, std::pair<int2 *, int>> trial;  // the rest's schedules, by rows per block
     if (rc == CAP_OK) best = time_batch(p, {}, {}, k, b, x, st, &rc);
@@ -2595,6 +2620,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->wblock = sh.block;
             p->rpl = sh.rpl;
             p->staged = sh.staged;
+            p->tile = sh.tile;
             if ((rc = build_batch_schedule(p, ro, lv, batch_solve_rows(p, k), st, &skip)) != CAP_OK) break;
             const float t = time_batch(p, cand, cand_lo, k, b, x, st, &rc, fk);
             if (rc != CAP_OK) break;
@@ -2610,6 +2636,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->wblock = pick.block;
             p->rpl = pick.rpl;
             p->staged = pick.staged;
+            p->tile = pick.tile;
             // the refinement sweep follows the solve's ticket size
             p->rblock = pick.block == kBlockLarge ? kBlockLarge : (p->rblock == kBlockLarge ? kBlock : p->rblock);
         } else {
@@ -2618,6 +2645,7 @@ int choose_batch_hybrid(cap_plan p, int k, bool bpc_fixed, const std::vector<int
             p->wblock = plain.block;
             p->rpl = plain.rpl;
             p->staged = plain.staged;
+            p->tile = plain.tile;
             pick.block = 0;
         }
     }
@@ -2751,9 +2779,10 @@ int cap_batch_info(cap_batch bt, int *algo, int *blocks_per_cu)
     return CAP_OK;
 }
 
-int cap_batch_tuning(cap_batch bt, int *row_threads, int *rhs_per_lane, int *partitioned)
+int cap_batch_tuning(cap_batch bt, int *row_threads, int *rhs_per_lane, int *partitioned, int *row_tiles)
 {
     if (!bt) return CAP_ERR_ARG;
+    if (row_tiles) *row_tiles = bt->plan->algo == CAP_ALGO_WRTFST ? bt->plan->tile : 0;
     if (row_threads) *row_threads = bt->plan->wblock;
     if (rhs_per_lane) *rhs_per_lane = bt->plan->wblock == kBlockLarge ? bt->plan->rpl : 1;
     if (partitioned) *partitioned = (int)bt->parts.size();

@@ -110,14 +110,15 @@ class Plan:
     def rhs_block(self, k: int):
         """The RHS block the autotuner picked for k right-hand sides (after a
         solve with k): dict(blocks_per_cu, row_threads, rhs_per_wave,
-        rhs_per_lane, readiness), or None."""
-        bpc, thr, kr, rpl, rdy = c_int(), c_int(), c_int(), c_int(), c_int()
-        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr), byref(rpl), byref(rdy))
+        rhs_per_lane, readiness, row_tiles), or None."""
+        bpc, thr, kr, rpl, rdy, tile = c_int(), c_int(), c_int(), c_int(), c_int(), c_int()
+        rc = N.lib().cap_plan_rhs_block(self._h, k, byref(bpc), byref(thr), byref(kr), byref(rpl), byref(rdy),
+                                        byref(tile))
         if rc == N.CAP_ERR_UNSUPPORTED:
             return None
         N.check(rc, "cap_plan_rhs_block")
         return dict(blocks_per_cu=bpc.value, row_threads=thr.value, rhs_per_wave=kr.value, rhs_per_lane=rpl.value,
-                    readiness=N.READINESS_NAME.get(rdy.value, "?"))
+                    readiness=N.READINESS_NAME.get(rdy.value, "?"), row_tiles=bool(tile.value))
 
     def status(self, stream=None) -> int:
         """Synchronizes, returns and clears the watchdog code (0 = every wait completed)."""
@@ -208,9 +209,10 @@ class Batch:
         N.check(N.lib().cap_batch_info(h, byref(a), byref(bpc)), "cap_batch_info")
         self.algo = N.ALGO_NAME[a.value]
         self.blocks_per_cu = bpc.value
-        thr, rpl, npart = c_int(), c_int(), c_int()
-        N.check(N.lib().cap_batch_tuning(h, byref(thr), byref(rpl), byref(npart)), "cap_batch_tuning")
+        thr, rpl, npart, tile = c_int(), c_int(), c_int(), c_int()
+        N.check(N.lib().cap_batch_tuning(h, byref(thr), byref(rpl), byref(npart), byref(tile)), "cap_batch_tuning")
         self.row_threads, self.rhs_per_lane, self.partitioned = thr.value, rpl.value, npart.value
+        self.row_tiles = bool(tile.value)
         rows = (c_int * max(1, npart.value))()
         got = N.lib().cap_batch_partitioned(h, rows, npart.value)
         if got < 0:

@@ -207,13 +207,15 @@ int cap_plan_poll_count(cap_plan plan, unsigned long long *polls, void *stream);
  * sides per wavefront of the block solve (*rhs_per_wave; 0 for the others).
  * The thread-per-row pick is then also timed with readiness in x itself
  * (*readiness: CAP_READY_STAGED or CAP_READY_IN_PLACE), which saves the
- * staged copy's stores.  Every candidate gives the same bits.
+ * staged copy's stores; the 1024-thread shape with one right-hand side per
+ * lane is timed with and without row tiles (*row_tiles, see cap_opts).
+ * Every candidate gives the same bits.
  * CAP_ERR_UNSUPPORTED until a solve with that k has run.  (The reference's
  * OPT_WARP_RHS / OPT_WARP_AUTO switches, openCL/WrtFst/common.h:28-30, set
  * this by hand at compile time.) */
 int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
                        int *row_threads, int *rhs_per_wave, int *rhs_per_lane,
-                       int *readiness);
+                       int *readiness, int *row_tiles);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table.
  * (An autotuned plan that resolved to mix, two-part or CSC, which solve one
  * right-hand side, solves several with the write-first kernel in the value
@@ -280,10 +282,12 @@ int cap_batch_create(cap_batch *batch, int n_mat, const int *row_off,
  * and its persistent grid size in workgroups per compute unit. */
 int cap_batch_info(cap_batch batch, int *algo, int *blocks_per_cu);
 /* No counterpart in the reference: the thread-per-row solve's workgroup size
- * (128, 256 or 1024) and right-hand sides per lane the batch resolved to,
- * and how many of its matrices it solves with partitioned plans of their own
- * (the hybrid batch of CAP_ALGO_AUTO: see cap_batch_create). */
-int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane, int *partitioned);
+ * (128, 256 or 1024), right-hand sides per lane and row tiles (1: each
+ * block's col_idx/val range staged in LDS) the batch resolved to, and how
+ * many of its matrices it solves with partitioned plans of their own (the
+ * hybrid batch of CAP_ALGO_AUTO: see cap_batch_create). */
+int cap_batch_tuning(cap_batch batch, int *row_threads, int *rhs_per_lane, int *partitioned,
+                     int *row_tiles);
 /* No counterpart in the reference: the first rows of the matrices a hybrid
  * batch solves with partitioned plans (up to `capacity` of them); returns
  * how many there are, or a negative CAP_ERR_* code. */

@@ -190,6 +190,8 @@ def test_hybrid_batch(cuda, k):
     t = [torch.from_numpy(a).to(cuda) for a in (row_off, rp, ci, v)]
     bt = Batch(*t, k=k, algo="auto")
     assert bt.algo == "wrtfst" and bt.partitioned >= 1 and set(bt.partitioned_matrices) <= {0, 1, 2}
+    # row tiles only where the rest runs 1024-thread workgroups, one right-hand side per lane
+    assert not bt.row_tiles or (bt.row_threads == 1024 and bt.rhs_per_lane == 1)
     bd = torch.from_numpy(b.reshape(N, k)).to(cuda)
     _check_hybrid(bt, bd, mats, row_off, b, k)
     bt.close()

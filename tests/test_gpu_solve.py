@@ -216,6 +216,29 @@ def test_mix_row_tiles_bit_exact(cuda, border, name, mat):
         assert np.array_equal(x, ref), (name, dtype, rel_maxnorm(x, ref))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("name,mat", [MATS[0], MATS[3], MATS[4], MATS[5]],
+                         ids=["band", "cfg1_small", "cfg2_small", "chain"])
+def test_row_tiles_1024_threads_bit_exact(cuda, name, mat, dtype):
+    """The 1024-thread shape with one right-hand side per lane and row tiles
+    (the batch tuner's pick on many matrices): bit-exact against the fused
+    oracle at several right-hand-side counts; cfg2_small's blocks overflow
+    the tile and read global memory."""
+    import torch
+    from capellinisptrsv_amd import Plan
+    rp, ci, v = mat
+    v = v.astype(dtype)
+    n = len(rp) - 1
+    d = [torch.from_numpy(a).to(cuda) for a in (rp, ci, v)]
+    plan = Plan(*d, algo="wrtfst", sync="value", rhs_per_lane=1, row_tiles=True)
+    for k in (2, 3, 16, 64):
+        b = api.uniform(400 + k, n * k).astype(dtype)
+        x = plan.solve(torch.from_numpy(b.reshape(n, k)).to(cuda)).cpu().numpy().reshape(-1)
+        assert plan.status() == 0
+        assert np.array_equal(x, oracle.csr_forward(rp, ci, v, b, rhs=k, fused=True)), (name, k)
+    plan.close()
+
+
 @pytest.mark.parametrize("upper,unit", [(True, False), (False, True), (True, True)])
 def test_row_tiles_triangular_kinds(cuda, upper, unit):
     """Row tiles on upper (reversed rows) and implicit-unit-diagonal solves."""
@@ -505,4 +528,6 @@ def test_rhs_block_tuned_per_count(cuda, cfg, scale):
             assert pick["blocks_per_cu"] in (1, 2, 4) and pick["row_threads"] in (128, 256, 1024)
             assert pick["rhs_per_lane"] in (1, 2, 4, 8) and pick["rhs_per_lane"] <= k
             assert pick["readiness"] in ("staged", "in_place")
+            # several right-hand sides per lane (1024 threads) have no tiled form
+            assert not pick["row_tiles"] or not (pick["row_threads"] == 1024 and pick["rhs_per_lane"] > 1), pick
     plan.close()
