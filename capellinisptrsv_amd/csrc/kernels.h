@@ -913,8 +913,8 @@ __device__ __forceinline__ void refine_pair_row(const int *__restrict__ ci, cons
 // tickets, batch schedule and LDS window as in k_wrtfst); pr holds the pairs
 // and starts filled with the sentinel.  norms[0] = max|d|, norms[1] =
 // max|x0| (bits of non-negative doubles, atomicMax per workgroup).  (Row
-// tiles, which speed up the batch's 1024-thread solve, slowed this sweep:
-// 9.60 against 8.79 ms on 253 matrices x 16 right-hand sides.)
+// tiles, which speed up the batch's 1024-thread solve, gave this sweep
+// nothing: 9.60 against 9.45-9.58 ms on 253 matrices x 16 right-hand sides.)
 template <int LDS, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
     k_refine2(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
