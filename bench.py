@@ -453,7 +453,8 @@ def extra_workloads(args):
     res["batched_256x16"] = {
         "matrices": 256, "rows": N, "nnzL": int(len(ci_all)), "k": k,
         "pick": {"algo": bt.algo, "blocks_per_cu": bt.blocks_per_cu, "row_threads": bt.row_threads,
-                 "rhs_per_lane": bt.rhs_per_lane, "partitioned_matrices": bt.partitioned_matrices},
+                 "rhs_per_lane": bt.rhs_per_lane, "row_tiles": bt.row_tiles,
+                 "partitioned_matrices": bt.partitioned_matrices},
         "fp64_ms": round(fp64_ms, 3), "fp64_gflops": round(2 * len(ci_all) * k / (fp64_ms * 1e6), 2),
         "fp32_refined_ms": round(ref_ms, 3), "refined_iterations": iters, "refined_over_fp64": round(ref_ms / fp64_ms, 3),
         "fp64_rel_err_max_over_matrices": worst64, "refined_rel_err_max_over_matrices": worstr,
