@@ -344,9 +344,19 @@ def extra_workloads(args):
         assert p.status() == 0
         ref = oracle.csr_forward(rp, ci, v, b.cpu().numpy(), fused=True)
         err = float(np.max(np.abs(x.cpu().numpy() - ref)))
-        res[name] = {"n": n, "nnzL": nnz, "levels": api.matrix_layer(n, n, nnz, rp, ci, v)[0], "plan": p.info(),
+        levels = api.matrix_layer(n, n, nnz, rp, ci, v)[0]
+        ds = api.datasize(n, n, nnz, 8)
+        res[name] = {"n": n, "nnzL": nnz, "levels": levels, "plan": p.info(),
                      "ms_per_solve": round(ms, 4), "gflops": round(2 * nnz / (ms * 1e6), 3),
+                     "us_per_level": round(ms * 1e3 / levels, 3),
+                     "datasize_gb_per_s": round(ds / (ms * 1e6), 2), "frac_hbm": round(ds / (ms * 1e6) / 8000.0, 5),
                      "max_abs_err_vs_oracle": err, "rel_err_vs_oracle": err / float(np.max(np.abs(ref)))}
+        if cfg == 0:
+            # the thread-per-row choices it competes with, for context
+            for algo, kw in (("mix_row_tiles", dict(algo="mix", row_tiles=True)), ("wrtfst", dict(algo="wrtfst"))):
+                q = Plan(*d, **kw)
+                res[name][f"{algo}_ms_per_solve"] = round(_time_gpu(lambda: q.solve(b, out=x), 20, warmup=2), 4)
+                q.close()
         if not args.no_cpu:
             kind, cpu_ms, reps = _cpu_serial_ms(rp, ci, v, b.cpu().numpy(), seconds=3.0)
             res[name]["cpu_serial"] = {"ms_per_solve": round(cpu_ms, 3), "kind": kind, "cores": 1, "solves": reps,

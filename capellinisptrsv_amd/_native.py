@@ -27,7 +27,7 @@ CAP_ERR_NOT_CONVERGED = -12
 CAP_ERR_WATCHDOG = -13
 
 CAP_F64, CAP_F32 = 0, 1
-ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4, "local": 5, "partition": 6}
+ALGO = {"auto": 0, "wrtfst": 1, "twopart": 2, "mix": 3, "csc": 4, "local": 5, "partition": 6, "level": 7}
 ALGO_NAME = {v: k for k, v in ALGO.items()}
 SYNC = {"auto": 0, "value": 1, "flag": 2}
 ADJOINT = {"auto": 0, "scatter": 1, "gather": 2}

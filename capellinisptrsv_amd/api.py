@@ -234,9 +234,9 @@ def spts_syncfree_hip_mix(row_ptr, col_idx, val, m: int, n: int, nnzL: int):
 
 def spts_syncfree_hip_csr_algo(row_ptr, col_idx, val, m: int, n: int, nnzL: int, algo: str = "auto"):
     """The benchmark-compatible wrapper with a chosen algorithm (auto, wrtfst,
-    twopart, csc, local, partition).  Returns (ret, solve_time_ms, gflops, bandwith)."""
+    twopart, csc, local, partition, level).  Returns (ret, solve_time_ms, gflops, bandwith)."""
     if algo not in N.ALGO or algo == "mix":
-        raise ValueError(f"algorithm {algo!r}: one of auto, wrtfst, twopart, csc, local, partition")
+        raise ValueError(f"algorithm {algo!r}: one of auto, wrtfst, twopart, csc, local, partition, level")
     return _bench("spts_syncfree_hip_csr_algo", row_ptr, col_idx, val, m, n, nnzL, mix=False, algo=N.ALGO[algo])
 
 

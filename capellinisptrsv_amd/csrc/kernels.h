@@ -1374,6 +1374,139 @@ __global__ void __launch_bounds__(kLocalBlock)
         });
 }
 
+// ------------------------------------------------ level steps in one workgroup
+// CAP_ALGO_LEVEL.  One workgroup walks the matrix level by level: the rows
+// of a level (in steps of at most kLevelBlock rows) are solved in parallel,
+// one row per thread, and a workgroup barrier ends each step.  No thread
+// waits on another row's value, so there is no polling and no readiness
+// state; a step costs LDS reads, the row's chain and one barrier inside one
+// compute unit, against the memory round trip of a publish -> observe hop
+// between compute units.
+//
+// Values travel through an LDS ring, row i in slot i & mask.  The plan
+// checks, for every off-diagonal entry c of a row in level L, that c > M_L -
+// R (M_L: the largest row index in levels <= L; R = mask + 1): no row that
+// maps to c's slot has been written by then.  The row data is laid out by
+// the plan in step order (slot s * kLevelBlock + thread): {row, off-diagonal
+// count}, the first E column indices and values in stored order, and the
+// diagonal; padding slots have row -1.  It is loaded D steps ahead into
+// registers, the {row, count} pair 2D steps ahead (b is gathered through the
+// row index), so the barrier waits only for LDS traffic.  Longer rows read
+// their remaining entries from the CSR.  The sum is the thread-per-row
+// kernels' stored-order chain and division, so x is the same bit for bit.
+constexpr int kLevelBlock = 256;
+constexpr int kLevelDepth4 = 10;  // steps of row data in flight with E = 4 entries carried
+constexpr int kLevelDepth8 = 6;   // with E = 8 (the load counter holds 63 loads in flight)
+
+template <typename T, int E>
+struct LevelStage {
+    int row, cnt;
+    int c[E];
+    T v[E];
+    T d, bo;
+};
+
+// 16-byte loads of E consecutive elements (E * sizeof(T) is a multiple of
+// 16 for the instantiated E; the plan's arrays are 16-byte aligned)
+template <typename V, int E>
+__device__ __forceinline__ void load16(const V *__restrict__ src, V (&dst)[E])
+{
+    static_assert((E * sizeof(V)) % 16 == 0, "E values must fill 16-byte loads");
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (int h = 0; h < (int)(E * sizeof(V) / 16); ++h) {
+        const uint4 w = s4[h];
+        __builtin_memcpy(reinterpret_cast<char *>(dst) + 16 * h, &w, 16);
+    }
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void level_load(LevelStage<T, E> &st, int2 rec, long long q, const int *__restrict__ lci,
+                                           const T *__restrict__ lv, const T *__restrict__ ld, const T *__restrict__ b)
+{
+    st.row = rec.x;
+    st.cnt = rec.y;
+    load16<int, E>(lci + q * E, st.c);
+    load16<T, E>(lv + q * E, st.v);
+    st.d = ld[q];
+    st.bo = b[rec.x < 0 ? 0 : rec.x];  // padding slots read b[0], unused
+}
+
+__device__ __forceinline__ void level_barrier()
+{
+    // only LDS carries values between steps: the fences order LDS alone, so
+    // the barrier does not wait for the row-data loads in flight
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T, int E, int D>
+__global__ void __launch_bounds__(kLevelBlock)
+    k_level(const int2 *__restrict__ rec, const int *__restrict__ lci, const T *__restrict__ lv,
+            const T *__restrict__ ld, const int *__restrict__ rp, const int *__restrict__ ci,
+            const T *__restrict__ val, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ sink, int steps,
+            int mask, SyncCtx sc)
+{
+    typedef typename VBits<T>::U U;
+    extern __shared__ unsigned long long smem_level[];
+    T *ring = reinterpret_cast<T *>(smem_level);
+    const int t = threadIdx.x;
+    LevelStage<T, E> st[D];
+    int2 ahead[D], first[D];
+    // the row pairs first, so no stage's b load waits behind another stage
+#pragma unroll
+    for (int r = 0; r < D; ++r) first[r] = rec[(long long)r * kLevelBlock + t];
+#pragma unroll
+    for (int r = 0; r < D; ++r) ahead[r] = rec[(long long)(D + r) * kLevelBlock + t];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        const long long q = (long long)r * kLevelBlock + t;
+        level_load<T, E>(st[r], first[r], q, lci, lv, ld, b);
+    }
+    // Two rounds of the D stages per iteration: the compiler's load-counter
+    // waits are exact in the second round (the first round's waits, merged
+    // at the loop head, assume about half the loads in flight).
+    for (int s0 = 0; s0 < steps; s0 += 2 * D) {
+#pragma unroll
+        for (int h = 0; h < 2 * D; ++h) {
+            const int r = h % D;
+            const int s = s0 + h;
+            if (s >= steps) return;  // uniform
+            const LevelStage<T, E> &w = st[r];
+            T xo = (T)0;
+            if (w.row >= 0) {
+                // every carried slot is read (padding entries point at slot 0),
+                // so the E LDS reads issue back to back
+                T xv[E];
+#pragma unroll
+                for (int u = 0; u < E; ++u) xv[u] = ring[w.c[u] & mask];
+                T sum = (T)0;
+#pragma unroll
+                for (int u = 0; u < E; ++u)
+                    if (u < w.cnt) sum = acc(sc, w.v[u], xv[u], sum);
+                if (w.cnt > E) {  // the rest of a long row, from the CSR
+                    const int j1 = rp[w.row] + w.cnt;
+                    for (int j = j1 - w.cnt + E; j < j1; ++j) sum = acc(sc, val[j], ring[ci[j] & mask], sum);
+                }
+                U o = as_bits((w.bo - sum) / w.d);
+                if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;  // as the publishing kernels do
+                xo = as_val(o);
+                ring[w.row & mask] = xo;
+            }
+            // one store per step on every path (padding lanes write the
+            // sink), so the load-counter waits do not depend on the branch
+            *(w.row >= 0 ? x + w.row : sink) = xo;
+            // refill this stage for step s + D (padded: the arrays hold 2D
+            // steps past the last), and its row pair for step s + 2D
+            const long long q = (long long)(s + D) * kLevelBlock + t;
+            level_load<T, E>(st[r], ahead[r], q, lci, lv, ld, b);
+            ahead[r] = rec[(long long)(s + 2 * D) * kLevelBlock + t];
+            level_barrier();
+        }
+    }
+}
+
 // ------------------------------------------------ SyncFree CSC (K4 / K5)
 // sptrsv_syncfree_cuda_analyser (CUDA/SyncFree_csc/spts_syncfree_cuda2.h:10-21)
 __global__ void __launch_bounds__(kBlock) k_indegree(const int *__restrict__ idx, int nnz, int *__restrict__ deg)

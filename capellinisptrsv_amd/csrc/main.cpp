@@ -1,4 +1,4 @@
-// Drop-in driver: ./sptrsv_main A.mtx [-a wrtfst|2part|mix|csc|auto|local|partition|serial] [--values]
+// Drop-in driver: ./sptrsv_main A.mtx [-a wrtfst|2part|mix|csc|auto|local|partition|level|serial] [--values]
 //
 // Same flow and output as the reference's drivers (CUDA/ourWrtFst/main.cu:9-115,
 // CUDA/mix/main.cu:104-106, CUDA/SyncFree_csc/main.cu:92-134): read the
@@ -222,7 +222,7 @@ int main(int argc, char **argv)
         else filename = argv[a];
     }
     if (!filename) {
-        std::fprintf(stderr, "usage: %s A.mtx [-a wrtfst|2part|mix|csc|auto|local|partition|serial] [--values] [--no-sidecar]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s A.mtx [-a wrtfst|2part|mix|csc|auto|local|partition|level|serial] [--values] [--no-sidecar]\n", argv[0]);
         std::fclose(fp);
         return -1;
     }
@@ -296,9 +296,12 @@ int main(int argc, char **argv)
     } else if (algo == "wrtfst") {
         for (int r = 0; r < 2 && rc == 0; ++r)
             rc = spts_syncfree_hip_csr_wrt(rpL, ciL, vL, m, n, nnzL, &solve_time, &gflops, &bandwith);
-    } else if (algo == "auto" || algo == "local" || algo == "partition") {
+    } else if (algo == "auto" || algo == "local" || algo == "partition" || algo == "level") {
         // this library's autotuner and deep-matrix solvers, same columns
-        const int a = algo == "auto" ? CAP_ALGO_AUTO : algo == "local" ? CAP_ALGO_LOCAL : CAP_ALGO_PARTITION;
+        const int a = algo == "auto"    ? CAP_ALGO_AUTO
+                      : algo == "local" ? CAP_ALGO_LOCAL
+                      : algo == "level" ? CAP_ALGO_LEVEL
+                                        : CAP_ALGO_PARTITION;
         for (int r = 0; r < 2 && rc == 0; ++r)
             rc = spts_syncfree_hip_csr_algo(rpL, ciL, vL, m, n, nnzL, a, &solve_time, &gflops, &bandwith);
     } else {

@@ -96,6 +96,17 @@ struct PartState {
     double *res = nullptr, *cor = nullptr;  // refinement: FP64 residual and correction (n * kcap)
 };
 
+// Step-ordered row data of CAP_ALGO_LEVEL (layout in kernels.h, k_level):
+// steps + 2 * depth steps of kLevelBlock slots; E entries carried per slot;
+// the LDS ring has mask + 1 slots.
+constexpr int kCtrlSink = 2048;  // k_level's padding-lane stores (8 bytes, never read)
+struct LevelState {
+    int2 *rec = nullptr;
+    int *ci = nullptr;
+    void *v = nullptr, *d = nullptr;
+    int steps = 0, E = 4, mask = 0;
+};
+
 // The RHS-block pick of an autotuned plan for one right-hand-side count:
 // thread-per-row workgroups per CU, threads per workgroup and right-hand
 // sides per lane, or the partitioned block solve's right-hand sides per
@@ -122,7 +133,8 @@ struct cap_plan_s {
     long long flags_len = 0;
     int epoch = 0;
     // control block (one 4 KiB allocation): tickets at +0, watchdog word at
-    // +256, refinement norms at +512, poll counters at +1024
+    // +256, refinement norms at +512, poll counters at +1024, the level
+    // steps' store sink at +kCtrlSink
     unsigned char *ctrl = nullptr;
     unsigned *ticket = nullptr;
     int *err = nullptr;
@@ -186,6 +198,8 @@ struct cap_plan_s {
     int graph_rhs = 0;
     // partitioned banded solve (CAP_ALGO_PARTITION)
     struct PartState *part = nullptr;
+    // level steps in one workgroup (CAP_ALGO_LEVEL)
+    LevelState *lvl = nullptr;
 };
 
 // Side streams of a hybrid batch: its partitioned matrices' solves run
@@ -373,10 +387,22 @@ void free_partition(cap_plan p)
     p->part = nullptr;
 }
 
+void free_level(cap_plan p)
+{
+    LevelState *ls = p->lvl;
+    if (!ls) return;
+    (void)hipDeviceSynchronize();
+    for (void *b : {(void *)ls->rec, (void *)ls->ci, ls->v, ls->d})
+        if (b) (void)hipFree(b);
+    delete ls;
+    p->lvl = nullptr;
+}
+
 void plan_free(cap_plan p)
 {
     if (!p) return;
     free_partition(p);
+    free_level(p);
     if (p->ready) {
         (void)hipDeviceSynchronize();  // no kernel of this plan may still poll the block
         ready_release(p->ready);
@@ -443,7 +469,8 @@ template <typename T>
 int launch_prepare(cap_plan p, int algo, int sync, void *x, long long count, hipStream_t st)
 {
     typedef typename VBits<T>::U U;
-    if (algo == CAP_ALGO_PARTITION) return CAP_OK;  // the partitioned solve keeps no readiness state
+    // the partitioned solve and the level steps keep no readiness state
+    if (algo == CAP_ALGO_PARTITION || algo == CAP_ALGO_LEVEL) return CAP_OK;
     if (algo == CAP_ALGO_LOCAL) sync = CAP_SYNC_VALUE;
     if (sync == CAP_SYNC_VALUE) {
         void *target = x;
@@ -768,7 +795,8 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     const RhsScope scope(p, rhs);
     if (algo == CAP_ALGO_PARTITION) return launch_partition<T>(p, b, x, rhs, st);
     SyncCtx sc{p->flags, p->epoch, p->spin_limit, p->err, nullptr, p->unfused, p->wait_ticks};
-    if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL) CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
+    if (algo != CAP_ALGO_CSC && algo != CAP_ALGO_LOCAL && algo != CAP_ALGO_LEVEL)
+        CAP_TRY(sync_ctx<T>(p, sync, (long long)n * rhs, &sc));
     if (fused) sc.unfused = 0;
     if (algo == CAP_ALGO_WRTFST && p->rpl > 1 && rhs > 1) {  // several right-hand sides per lane
         if (rhs > kWave || sync != CAP_SYNC_VALUE || p->lds || p->count_polls) return CAP_ERR_UNSUPPORTED;
@@ -854,6 +882,19 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const int rl = local_ring_log2<T>(p, rhs, &bytes);
         k_local<T><<<nb, kLocalBlock, bytes, st>>>(p->rp, p->ci, val, b, x, sc, rhs, p->seg, p->n_seg, p->ticket,
                                                     rl);
+    } else if (algo == CAP_ALGO_LEVEL) {
+        // one right-hand side, the plan's own values (the tables hold them)
+        const LevelState *ls = p->lvl;
+        if (rhs != 1 || !ls || (const void *)val != p->val) return CAP_ERR_UNSUPPORTED;
+        const size_t bytes = (size_t)(ls->mask + 1) * sizeof(T);
+        const T *lv = (const T *)ls->v, *ld = (const T *)ls->d;
+        T *sink = reinterpret_cast<T *>(p->ctrl + kCtrlSink);
+        if (ls->E == 4)
+            k_level<T, 4, kLevelDepth4><<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b,
+                                                                       x, sink, ls->steps, ls->mask, sc);
+        else
+            k_level<T, 8, kLevelDepth8><<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b,
+                                                                       x, sink, ls->steps, ls->mask, sc);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         // lanes per column from the mean column length (diagonal included)
@@ -908,12 +949,13 @@ int reset_t(cap_plan p, int algo, int sync, void *x, int rhs, hipStream_t st)
 
 // The algorithm and protocol that solve rhs right-hand sides: the plan's,
 // except that an autotuned plan whose pick has no multi-RHS form (mix,
-// two-part, CSC) solves several with the write-first kernel in the value
+// two-part, CSC, level steps) solves several with the write-first kernel in the value
 // protocol (its RHS block tuned like any other).
 inline int algo_for(cap_plan p, int rhs)
 {
     if (rhs > 1 && p->auto_algo &&
-        (p->algo == CAP_ALGO_MIX || p->algo == CAP_ALGO_TWOPART || p->algo == CAP_ALGO_CSC))
+        (p->algo == CAP_ALGO_MIX || p->algo == CAP_ALGO_TWOPART || p->algo == CAP_ALGO_CSC ||
+         p->algo == CAP_ALGO_LEVEL))
         return CAP_ALGO_WRTFST;
     return p->algo;
 }
@@ -1436,6 +1478,92 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
     return rc;
 }
 
+// Tables of CAP_ALGO_LEVEL (k_level), from host copies of the rows.  The
+// rows of each level fill whole steps of kLevelBlock slots in row order; the
+// ring is the smallest power of two (>= 64) that keeps every off-diagonal
+// entry's value in place until its last reader (c > M_L - R, see k_level).
+// CAP_ERR_UNSUPPORTED when that ring exceeds kLevelRingBytes, or the padded
+// steps exceed kLevelPadMax slots per row (many narrow levels: a thread per
+// slot would mostly idle).
+constexpr size_t kLevelRingBytes = 65536;
+constexpr int kLevelPadMax = 4;
+
+template <typename T>
+int build_level_t(cap_plan p, const std::vector<int> &rp)
+{
+    const int n = p->n, nnz = p->nnz;
+    std::vector<int> ci((size_t)nnz);
+    std::vector<T> v((size_t)nnz);
+    if (nnz) {
+        CAP_HIP_TRY(hipMemcpy(ci.data(), p->ci, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+        CAP_HIP_TRY(hipMemcpy(v.data(), p->val, sizeof(T) * v.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<int> lv((size_t)std::max(n, 1));
+    const int L = n ? cap_matrix_layer(n, rp.data(), ci.data(), nullptr, nullptr, lv.data()) : 0;  // 1-based
+    std::vector<long long> width((size_t)L + 1, 0);
+    std::vector<int> top((size_t)L + 1, -1);  // largest row index in levels <= l
+    int maxoff = 0;
+    for (int i = 0; i < n; ++i) {
+        ++width[(size_t)lv[i]];
+        top[(size_t)lv[i]] = std::max(top[(size_t)lv[i]], i);
+        maxoff = std::max(maxoff, rp[i + 1] - 1 - rp[i]);
+    }
+    for (int l = 1; l <= L; ++l) top[(size_t)l] = std::max(top[(size_t)l], top[(size_t)l - 1]);
+    long long need = 1;
+    for (int i = 0; i < n; ++i)
+        for (int j = rp[i]; j < rp[i + 1] - 1; ++j) need = std::max(need, (long long)top[(size_t)lv[i]] - ci[j] + 1);
+    long long R = 64;
+    while (R < need) R *= 2;
+    if ((size_t)R * sizeof(T) > kLevelRingBytes) return CAP_ERR_UNSUPPORTED;
+    const int E = maxoff <= 4 ? 4 : 8;
+    const int D = E == 4 ? kLevelDepth4 : kLevelDepth8;
+    std::vector<long long> next((size_t)L + 1, 0);  // next free slot of each level
+    long long steps = 0;
+    for (int l = 1; l <= L; ++l) {
+        next[(size_t)l] = steps * kLevelBlock;
+        steps += (width[(size_t)l] + kLevelBlock - 1) / kLevelBlock;
+    }
+    if (steps * kLevelBlock > std::max((long long)kLevelPadMax * n, 1LL << 16)) return CAP_ERR_UNSUPPORTED;
+    const size_t S = (size_t)(steps + 2 * D) * kLevelBlock;
+    std::vector<int2> rec(S, make_int2(-1, 0));
+    std::vector<int> lci(S * E, 0);
+    std::vector<T> lval(S * E, (T)0), ldiag(S, (T)1);
+    for (int i = 0; i < n; ++i) {
+        const size_t q = (size_t)next[(size_t)lv[i]]++;
+        const int off = rp[i + 1] - 1 - rp[i];
+        rec[q] = make_int2(i, off);
+        for (int u = 0; u < std::min(off, E); ++u) {
+            lci[q * E + u] = ci[(size_t)rp[i] + u];
+            lval[q * E + u] = v[(size_t)rp[i] + u];
+        }
+        ldiag[q] = v[(size_t)rp[i + 1] - 1];
+    }
+    free_level(p);
+    LevelState *ls = new LevelState();
+    p->lvl = ls;
+    ls->steps = (int)steps;
+    ls->E = E;
+    ls->mask = (int)R - 1;
+    if (hipMalloc(&ls->rec, sizeof(int2) * S) != hipSuccess || hipMalloc(&ls->ci, sizeof(int) * lci.size()) != hipSuccess ||
+        hipMalloc(&ls->v, sizeof(T) * lval.size()) != hipSuccess || hipMalloc(&ls->d, sizeof(T) * S) != hipSuccess) {
+        free_level(p);
+        return CAP_ERR_ALLOC;
+    }
+    if (hipMemcpy(ls->rec, rec.data(), sizeof(int2) * S, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ls->ci, lci.data(), sizeof(int) * lci.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ls->v, lval.data(), sizeof(T) * lval.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ls->d, ldiag.data(), sizeof(T) * S, hipMemcpyHostToDevice) != hipSuccess) {
+        free_level(p);
+        return CAP_ERR_HIP;
+    }
+    return CAP_OK;
+}
+
+int build_level(cap_plan p, const std::vector<int> &rp_host)
+{
+    return p->vt == CAP_F64 ? build_level_t<double>(p, rp_host) : build_level_t<float>(p, rp_host);
+}
+
 // Candidates are timed on the plan's own matrix (b = 0; the dependency
 // structure, not the values, sets the time) and the fastest is cached per
 // matrix shape: thread-per-row with and without the LDS window, at 1 or 2
@@ -1456,6 +1584,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             apply_choice(p, it->second);
             if (p->algo == CAP_ALGO_MIX) return build_mix(p, rp_host, it->second.border);
             if (p->algo == CAP_ALGO_LOCAL) return ensure_segments(p, nullptr);
+            if (p->algo == CAP_ALGO_LEVEL) return build_level(p, rp_host);
             if (p->algo != CAP_ALGO_PARTITION) return CAP_OK;
             // the pick was cached for this structure; these values may not
             // keep the partition stable, and then the plan is tuned afresh
@@ -1491,6 +1620,9 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             // (reassociated sums: not a candidate in the reference arithmetic)
             if (!p->unfused && build_partition(p, rp_host, st) == CAP_OK)
                 cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
+            // level steps in one workgroup, where the ring and the padding allow
+            if (build_level(p, rp_host) == CAP_OK)
+                cands.insert(cands.begin(), {CAP_ALGO_LEVEL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
         }
     }
     // thresholds 4 and 32 (see mix_border_for) always; 16 where the mean row
@@ -1527,6 +1659,7 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
     if (pick.algo == CAP_ALGO_MIX) CAP_TRY(build_mix(p, rp_host, pick.border));
     if (pick.algo == CAP_ALGO_LOCAL) CAP_TRY(ensure_segments(p, nullptr));
     if (pick.algo != CAP_ALGO_PARTITION) free_partition(p);
+    if (pick.algo != CAP_ALGO_LEVEL) free_level(p);
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune_cache[key] = pick;
     return CAP_OK;
@@ -1732,6 +1865,8 @@ int plan_create_impl(cap_plan *out, int n, int nnz, const int *rp, const int *ci
             rc = ensure_segments(p, nullptr);
         } else if (p->algo == CAP_ALGO_PARTITION) {
             rc = build_partition(p, rp_host, st);
+        } else if (p->algo == CAP_ALGO_LEVEL) {
+            rc = build_level(p, rp_host);
         } else if (p->algo != CAP_ALGO_WRTFST && p->algo != CAP_ALGO_TWOPART) {
             rc = CAP_ERR_ARG;
         }
@@ -3197,13 +3332,13 @@ int spts_syncfree_hip_csr_wrt_s(const int *rp, const int *ci, const float *v, in
 int spts_syncfree_hip_csr_algo_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, int algo,
                                  double *t, double *gf, double *bw)
 {
-    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_PARTITION) return CAP_ERR_ARG;
+    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_LEVEL) return CAP_ERR_ARG;
     return bench_csr<double>(algo, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
 }
 int spts_syncfree_hip_csr_algo_s(const int *rp, const int *ci, const float *v, int m, int n, int nnzL, int algo,
                                  double *t, double *gf, double *bw)
 {
-    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_PARTITION) return CAP_ERR_ARG;
+    if (algo == CAP_ALGO_MIX || algo < CAP_ALGO_AUTO || algo > CAP_ALGO_LEVEL) return CAP_ERR_ARG;
     return bench_csr<float>(algo, rp, ci, v, m, n, nnzL, t, gf, bw, nullptr, nullptr, nullptr);
 }
 int spts_syncfree_hip_csr_d(const int *rp, const int *ci, const double *v, int m, int n, int nnzL, double *t,

@@ -27,7 +27,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ALGOS = ("wrtfst", "2part", "mix", "csc", "auto", "local", "partition", "serial")
+ALGOS = ("wrtfst", "2part", "mix", "csc", "auto", "local", "partition", "level", "serial")
 
 
 def matrices(source: str, root: str | None = None) -> list[str]:

@@ -38,7 +38,7 @@ class Plan:
 
     row_ptr/col_idx: int32 CUDA tensors; values: float64/float32 CUDA tensor.
     The plan keeps references to the three tensors (their memory is used in
-    place).  algo: auto|wrtfst|twopart|mix|csc|local|partition; sync:
+    place).  algo: auto|wrtfst|twopart|mix|csc|local|partition|level; sync:
     auto|value|flag.
     row_threads: 128 or 256 threads per workgroup of the thread-per-row
     kernels (0: tuned with algo auto, else 256).

@@ -71,6 +71,17 @@ extern "C" {
                               which waits for the device); the refined solve
                               uses FP64 partitioned solves (no FP32 tables)
                               with one FP64 correction. */
+#define CAP_ALGO_LEVEL 7     /* one workgroup walks the levels: a level's rows
+                              in parallel (thread per row, steps of 256 rows),
+                              a workgroup barrier between steps, values in an
+                              LDS ring; no polling.  Needs every dependency
+                              within a ring of <= 64 KiB of the newest row of
+                              its level (CAP_ERR_UNSUPPORTED otherwise, also
+                              when the padded steps exceed 4 slots per row);
+                              one right-hand side per solve; bit-exact (the
+                              thread-per-row sum order).  Timed by algo AUTO
+                              with the few-rows-per-level candidates; its
+                              refined solves use the write-first kernels. */
 
 /* adjoint method (cap_opts.adjoint) */
 #define CAP_ADJOINT_AUTO 0
@@ -378,7 +389,7 @@ int spts_syncfree_hip_mix_s(const int *csrRowPtrL, const int *csrColIdxL,
 /* No counterpart in the reference: the same benchmark-compatible wrapper
  * (get_x_b, validation, CAP_BENCH_REPEAT timed solves, the same outputs)
  * with the algorithm chosen by the caller: CAP_ALGO_AUTO, WRTFST, TWOPART,
- * CSC, LOCAL or PARTITION (for MIX use the mix wrapper, which also reports
+ * CSC, LOCAL, PARTITION or LEVEL (for MIX use the mix wrapper, which also reports
  * the chunk table).  It lets the drop-in driver run the autotuned and the
  * deep-matrix solvers with the same result.csv columns.  A matrix the
  * algorithm refuses returns its CAP_ERR_* code. */
