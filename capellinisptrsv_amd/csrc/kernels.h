@@ -1386,14 +1386,21 @@ __global__ void __launch_bounds__(kLocalBlock)
 // Values travel through an LDS ring, row i in slot i & mask.  The plan
 // checks, for every off-diagonal entry c of a row in level L, that c > M_L -
 // R (M_L: the largest row index in levels <= L; R = mask + 1): no row that
-// maps to c's slot has been written by then.  The row data is laid out by
-// the plan in step order (slot s * kLevelBlock + thread): {row, off-diagonal
-// count}, the first E column indices and values in stored order, and the
-// diagonal; padding slots have row -1.  It is loaded D steps ahead into
-// registers, the {row, count} pair 2D steps ahead (b is gathered through the
-// row index), so the barrier waits only for LDS traffic.  Longer rows read
-// their remaining entries from the CSR.  The sum is the thread-per-row
-// kernels' stored-order chain and division, so x is the same bit for bit.
+// maps to c's slot has been written by then.  Two slots follow the ring: R
+// holds 0 (never written) and R + 1 takes the padding rows' results.
+//
+// The row data is laid out by the plan in step order (slot s * kLevelBlock +
+// thread): {row, off-diagonal count}, the first E entries in stored order as
+// ring slots (c & mask) and values, and the diagonal.  Unused entries are
+// (slot R, value 0) and padding rows are {-1, 0} with diagonal 1, so every
+// thread runs the same straight-line step: fma(0, 0, s) = s exactly (a sum
+// that starts at +0 is never -0), and padding results go to slot R + 1 and
+// to the sink.  Rows longer than E read the rest from the CSR (LONG).  The
+// data is loaded D steps ahead into registers, the {row, count} pair 2D
+// steps ahead (b is gathered through the row index); the barriers order LDS
+// alone, so they do not wait for those loads.  The plan pads the steps to a
+// multiple of 2D with padding slots.  Each sum is the thread-per-row
+// kernels' stored-order chain and division: the same x bit for bit.
 constexpr int kLevelBlock = 256;
 constexpr int kLevelDepth4 = 10;  // steps of row data in flight with E = 4 entries carried
 constexpr int kLevelDepth8 = 6;   // with E = 8 (the load counter holds 63 loads in flight)
@@ -1421,13 +1428,13 @@ __device__ __forceinline__ void load16(const V *__restrict__ src, V (&dst)[E])
 }
 
 template <typename T, int E>
-__device__ __forceinline__ void level_load(LevelStage<T, E> &st, int2 rec, long long q, const int *__restrict__ lci,
+__device__ __forceinline__ void level_load(LevelStage<T, E> &st, int2 rec, unsigned q, const int *__restrict__ lci,
                                            const T *__restrict__ lv, const T *__restrict__ ld, const T *__restrict__ b)
 {
     st.row = rec.x;
     st.cnt = rec.y;
-    load16<int, E>(lci + q * E, st.c);
-    load16<T, E>(lv + q * E, st.v);
+    load16<int, E>(lci + (size_t)q * E, st.c);
+    load16<T, E>(lv + (size_t)q * E, st.v);
     st.d = ld[q];
     st.bo = b[rec.x < 0 ? 0 : rec.x];  // padding slots read b[0], unused
 }
@@ -1441,12 +1448,12 @@ __device__ __forceinline__ void level_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <typename T, int E, int D>
+template <typename T, int E, int D, bool UNFUSED, bool LONG>
 __global__ void __launch_bounds__(kLevelBlock)
     k_level(const int2 *__restrict__ rec, const int *__restrict__ lci, const T *__restrict__ lv,
             const T *__restrict__ ld, const int *__restrict__ rp, const int *__restrict__ ci,
             const T *__restrict__ val, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ sink, int steps,
-            int mask, SyncCtx sc)
+            int mask)
 {
     typedef typename VBits<T>::U U;
     extern __shared__ unsigned long long smem_level[];
@@ -1456,52 +1463,49 @@ __global__ void __launch_bounds__(kLevelBlock)
     int2 ahead[D], first[D];
     // the row pairs first, so no stage's b load waits behind another stage
 #pragma unroll
-    for (int r = 0; r < D; ++r) first[r] = rec[(long long)r * kLevelBlock + t];
+    for (int r = 0; r < D; ++r) first[r] = rec[r * kLevelBlock + t];
 #pragma unroll
-    for (int r = 0; r < D; ++r) ahead[r] = rec[(long long)(D + r) * kLevelBlock + t];
+    for (int r = 0; r < D; ++r) ahead[r] = rec[(D + r) * kLevelBlock + t];
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-        const long long q = (long long)r * kLevelBlock + t;
-        level_load<T, E>(st[r], first[r], q, lci, lv, ld, b);
-    }
-    // Two rounds of the D stages per iteration: the compiler's load-counter
-    // waits are exact in the second round (the first round's waits, merged
-    // at the loop head, assume about half the loads in flight).
+    for (int r = 0; r < D; ++r) level_load<T, E>(st[r], first[r], r * kLevelBlock + t, lci, lv, ld, b);
+    if (t == 0) ring[mask + 1] = (T)0;
+    level_barrier();
+    // Two rounds of the D stages per iteration (steps is a multiple of 2D,
+    // padded by the plan, so the body has no exit): the compiler's
+    // load-counter waits are exact in the second round (the first round's,
+    // merged at the loop head, assume about half the loads in flight).
     for (int s0 = 0; s0 < steps; s0 += 2 * D) {
 #pragma unroll
         for (int h = 0; h < 2 * D; ++h) {
             const int r = h % D;
             const int s = s0 + h;
-            if (s >= steps) return;  // uniform
             const LevelStage<T, E> &w = st[r];
-            T xo = (T)0;
-            if (w.row >= 0) {
-                // every carried slot is read (padding entries point at slot 0),
-                // so the E LDS reads issue back to back
-                T xv[E];
+            T xv[E];
 #pragma unroll
-                for (int u = 0; u < E; ++u) xv[u] = ring[w.c[u] & mask];
-                T sum = (T)0;
+            for (int u = 0; u < E; ++u) xv[u] = ring[w.c[u]];
+            T sum = (T)0;
 #pragma unroll
-                for (int u = 0; u < E; ++u)
-                    if (u < w.cnt) sum = acc(sc, w.v[u], xv[u], sum);
-                if (w.cnt > E) {  // the rest of a long row, from the CSR
-                    const int j1 = rp[w.row] + w.cnt;
-                    for (int j = j1 - w.cnt + E; j < j1; ++j) sum = acc(sc, val[j], ring[ci[j] & mask], sum);
+            for (int u = 0; u < E; ++u) sum = UNFUSED ? sum + w.v[u] * xv[u] : fmac(w.v[u], xv[u], sum);
+            if (LONG && w.cnt > E) {  // the rest of a long row, from the CSR
+                const int j1 = rp[w.row] + w.cnt;
+                for (int j = j1 - w.cnt + E; j < j1; ++j) {
+                    const T a = val[j], xj = ring[ci[j] & mask];
+                    sum = UNFUSED ? sum + a * xj : fmac(a, xj, sum);
                 }
-                U o = as_bits((w.bo - sum) / w.d);
-                if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;  // as the publishing kernels do
-                xo = as_val(o);
-                ring[w.row & mask] = xo;
             }
-            // one store per step on every path (padding lanes write the
-            // sink), so the load-counter waits do not depend on the branch
-            *(w.row >= 0 ? x + w.row : sink) = xo;
+            U o = as_bits((w.bo - sum) / w.d);
+            if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;  // as the publishing kernels do
+            const bool real = w.row >= 0;
+            ring[real ? w.row & mask : mask + 2] = as_val(o);
+            *(real ? x + w.row : sink) = as_val(o);
             // refill this stage for step s + D (padded: the arrays hold 2D
-            // steps past the last), and its row pair for step s + 2D
-            const long long q = (long long)(s + D) * kLevelBlock + t;
-            level_load<T, E>(st[r], ahead[r], q, lci, lv, ld, b);
-            ahead[r] = rec[(long long)(s + 2 * D) * kLevelBlock + t];
+            // steps past the last), and its row pair for step s + 2D.  Kept
+            // after the step's own work: interleaved with it, the compiler
+            // parks in-flight load results in accumulation registers, and
+            // each such move waits for its load.
+            __builtin_amdgcn_sched_barrier(0);
+            level_load<T, E>(st[r], ahead[r], (s + D) * kLevelBlock + t, lci, lv, ld, b);
+            ahead[r] = rec[(s + 2 * D) * kLevelBlock + t];
             level_barrier();
         }
     }

@@ -105,6 +105,7 @@ struct LevelState {
     int *ci = nullptr;
     void *v = nullptr, *d = nullptr;
     int steps = 0, E = 4, mask = 0;
+    bool long_rows = false;  // some row has more than E off-diagonal entries
 };
 
 // The RHS-block pick of an autotuned plan for one right-hand-side count:
@@ -886,15 +887,20 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         // one right-hand side, the plan's own values (the tables hold them)
         const LevelState *ls = p->lvl;
         if (rhs != 1 || !ls || (const void *)val != p->val) return CAP_ERR_UNSUPPORTED;
-        const size_t bytes = (size_t)(ls->mask + 1) * sizeof(T);
+        const size_t bytes = (size_t)(ls->mask + 3) * sizeof(T);  // the ring, its zero slot and its sink slot
         const T *lv = (const T *)ls->v, *ld = (const T *)ls->d;
         T *sink = reinterpret_cast<T *>(p->ctrl + kCtrlSink);
+        auto go = [&](auto kernel) {
+            kernel<<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b, x, sink, ls->steps,
+                                                  ls->mask);
+        };
+        const bool uf = sc.unfused != 0;
         if (ls->E == 4)
-            k_level<T, 4, kLevelDepth4><<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b,
-                                                                       x, sink, ls->steps, ls->mask, sc);
+            uf ? go(k_level<T, 4, kLevelDepth4, true, false>) : go(k_level<T, 4, kLevelDepth4, false, false>);
+        else if (!ls->long_rows)
+            uf ? go(k_level<T, 8, kLevelDepth8, true, false>) : go(k_level<T, 8, kLevelDepth8, false, false>);
         else
-            k_level<T, 8, kLevelDepth8><<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b,
-                                                                       x, sink, ls->steps, ls->mask, sc);
+            uf ? go(k_level<T, 8, kLevelDepth8, true, true>) : go(k_level<T, 8, kLevelDepth8, false, true>);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         // lanes per column from the mean column length (diagonal included)
@@ -1481,7 +1487,8 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
 // Tables of CAP_ALGO_LEVEL (k_level), from host copies of the rows.  The
 // rows of each level fill whole steps of kLevelBlock slots in row order; the
 // ring is the smallest power of two (>= 64) that keeps every off-diagonal
-// entry's value in place until its last reader (c > M_L - R, see k_level).
+// entry's value in place until its last reader (c > M_L - R, see k_level);
+// entries are stored as ring slots, unused ones as (slot R, value 0).
 // CAP_ERR_UNSUPPORTED when that ring exceeds kLevelRingBytes, or the padded
 // steps exceed kLevelPadMax slots per row (many narrow levels: a thread per
 // slot would mostly idle).
@@ -1524,16 +1531,18 @@ int build_level_t(cap_plan p, const std::vector<int> &rp)
         steps += (width[(size_t)l] + kLevelBlock - 1) / kLevelBlock;
     }
     if (steps * kLevelBlock > std::max((long long)kLevelPadMax * n, 1LL << 16)) return CAP_ERR_UNSUPPORTED;
+    steps = (steps + 2 * D - 1) / (2 * D) * (2 * D);  // the kernel runs whole rounds of 2D steps
     const size_t S = (size_t)(steps + 2 * D) * kLevelBlock;
+    if (S * E > (size_t)INT32_MAX) return CAP_ERR_UNSUPPORTED;  // the kernel's 32-bit slot indices
     std::vector<int2> rec(S, make_int2(-1, 0));
-    std::vector<int> lci(S * E, 0);
+    std::vector<int> lci(S * E, (int)R);
     std::vector<T> lval(S * E, (T)0), ldiag(S, (T)1);
     for (int i = 0; i < n; ++i) {
         const size_t q = (size_t)next[(size_t)lv[i]]++;
         const int off = rp[i + 1] - 1 - rp[i];
         rec[q] = make_int2(i, off);
         for (int u = 0; u < std::min(off, E); ++u) {
-            lci[q * E + u] = ci[(size_t)rp[i] + u];
+            lci[q * E + u] = ci[(size_t)rp[i] + u] & (int)(R - 1);
             lval[q * E + u] = v[(size_t)rp[i] + u];
         }
         ldiag[q] = v[(size_t)rp[i + 1] - 1];
@@ -1544,6 +1553,28 @@ int build_level_t(cap_plan p, const std::vector<int> &rp)
     ls->steps = (int)steps;
     ls->E = E;
     ls->mask = (int)R - 1;
+    ls->long_rows = maxoff > E;
+    // the ring and its two extra slots may pass the default 64 KiB of dynamic LDS
+    const int bytes = (int)((R + 2) * sizeof(T));
+    auto allow = [&](auto kernel) {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   bytes);
+    };
+    hipError_t e1, e2;
+    if (E == 4) {
+        e1 = allow(&k_level<T, 4, kLevelDepth4, true, false>);
+        e2 = allow(&k_level<T, 4, kLevelDepth4, false, false>);
+    } else if (!ls->long_rows) {
+        e1 = allow(&k_level<T, 8, kLevelDepth8, true, false>);
+        e2 = allow(&k_level<T, 8, kLevelDepth8, false, false>);
+    } else {
+        e1 = allow(&k_level<T, 8, kLevelDepth8, true, true>);
+        e2 = allow(&k_level<T, 8, kLevelDepth8, false, true>);
+    }
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        free_level(p);
+        return CAP_ERR_HIP;
+    }
     if (hipMalloc(&ls->rec, sizeof(int2) * S) != hipSuccess || hipMalloc(&ls->ci, sizeof(int) * lci.size()) != hipSuccess ||
         hipMalloc(&ls->v, sizeof(T) * lval.size()) != hipSuccess || hipMalloc(&ls->d, sizeof(T) * S) != hipSuccess) {
         free_level(p);
