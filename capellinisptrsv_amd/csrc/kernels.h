@@ -1376,7 +1376,8 @@ __global__ void __launch_bounds__(kLocalBlock)
 
 // ------------------------------------------------ level steps in one workgroup
 // CAP_ALGO_LEVEL.  One workgroup walks the matrix level by level: the rows
-// of a level (in steps of at most kLevelBlock rows) are solved in parallel,
+// of a level (in steps of at most B rows, B = the workgroup's 64 to
+// kLevelBlock threads, chosen by the plan) are solved in parallel,
 // one row per thread, and a workgroup barrier ends each step.  No thread
 // waits on another row's value, so there is no polling and no readiness
 // state; a step costs LDS reads, the row's chain and one barrier inside one
@@ -1389,7 +1390,7 @@ __global__ void __launch_bounds__(kLocalBlock)
 // maps to c's slot has been written by then.  Two slots follow the ring: R
 // holds 0 (never written) and R + 1 takes the padding rows' results.
 //
-// The row data is laid out by the plan in step order (slot s * kLevelBlock +
+// The row data is laid out by the plan in step order (slot s * B +
 // thread): {row, off-diagonal count}, the first E entries in stored order as
 // ring slots (c & mask) and values, and the diagonal.  Unused entries are
 // (slot R, value 0) and padding rows are {-1, 0} with diagonal 1, so every
@@ -1455,6 +1456,7 @@ __global__ void __launch_bounds__(kLevelBlock)
             const T *__restrict__ val, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ sink, int steps,
             int mask)
 {
+    const int B = blockDim.x;
     typedef typename VBits<T>::U U;
     extern __shared__ unsigned long long smem_level[];
     T *ring = reinterpret_cast<T *>(smem_level);
@@ -1463,11 +1465,11 @@ __global__ void __launch_bounds__(kLevelBlock)
     int2 ahead[D], first[D];
     // the row pairs first, so no stage's b load waits behind another stage
 #pragma unroll
-    for (int r = 0; r < D; ++r) first[r] = rec[r * kLevelBlock + t];
+    for (int r = 0; r < D; ++r) first[r] = rec[r * B + t];
 #pragma unroll
-    for (int r = 0; r < D; ++r) ahead[r] = rec[(D + r) * kLevelBlock + t];
+    for (int r = 0; r < D; ++r) ahead[r] = rec[(D + r) * B + t];
 #pragma unroll
-    for (int r = 0; r < D; ++r) level_load<T, E>(st[r], first[r], r * kLevelBlock + t, lci, lv, ld, b);
+    for (int r = 0; r < D; ++r) level_load<T, E>(st[r], first[r], r * B + t, lci, lv, ld, b);
     if (t == 0) ring[mask + 1] = (T)0;
     level_barrier();
     // Two rounds of the D stages per iteration (steps is a multiple of 2D,
@@ -1504,8 +1506,8 @@ __global__ void __launch_bounds__(kLevelBlock)
             // parks in-flight load results in accumulation registers, and
             // each such move waits for its load.
             __builtin_amdgcn_sched_barrier(0);
-            level_load<T, E>(st[r], ahead[r], (s + D) * kLevelBlock + t, lci, lv, ld, b);
-            ahead[r] = rec[(s + 2 * D) * kLevelBlock + t];
+            level_load<T, E>(st[r], ahead[r], (s + D) * B + t, lci, lv, ld, b);
+            ahead[r] = rec[(s + 2 * D) * B + t];
             level_barrier();
         }
     }

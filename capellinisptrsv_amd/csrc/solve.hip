@@ -97,14 +97,15 @@ struct PartState {
 };
 
 // Step-ordered row data of CAP_ALGO_LEVEL (layout in kernels.h, k_level):
-// steps + 2 * depth steps of kLevelBlock slots; E entries carried per slot;
+// steps + 2 * depth steps of `block` slots (the workgroup size); E entries
+// carried per slot;
 // the LDS ring has mask + 1 slots.
 constexpr int kCtrlSink = 2048;  // k_level's padding-lane stores (8 bytes, never read)
 struct LevelState {
     int2 *rec = nullptr;
     int *ci = nullptr;
     void *v = nullptr, *d = nullptr;
-    int steps = 0, E = 4, mask = 0;
+    int steps = 0, E = 4, mask = 0, block = kLevelBlock;
     bool long_rows = false;  // some row has more than E off-diagonal entries
 };
 
@@ -891,7 +892,7 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
         const T *lv = (const T *)ls->v, *ld = (const T *)ls->d;
         T *sink = reinterpret_cast<T *>(p->ctrl + kCtrlSink);
         auto go = [&](auto kernel) {
-            kernel<<<1, kLevelBlock, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b, x, sink, ls->steps,
+            kernel<<<1, ls->block, bytes, st>>>(ls->rec, ls->ci, lv, ld, p->rp, p->ci, val, b, x, sink, ls->steps,
                                                   ls->mask);
         };
         const bool uf = sc.unfused != 0;
@@ -1485,7 +1486,10 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
 }
 
 // Tables of CAP_ALGO_LEVEL (k_level), from host copies of the rows.  The
-// rows of each level fill whole steps of kLevelBlock slots in row order; the
+// workgroup size B is the smallest of 64, 128, 192 and 256 threads that
+// needs at most 1/8 more steps than 256 would (fewer wavefronts per
+// barrier, fewer padding slots).  The rows of each level fill whole steps of B slots in
+// row order; the
 // ring is the smallest power of two (>= 64) that keeps every off-diagonal
 // entry's value in place until its last reader (c > M_L - R, see k_level);
 // entries are stored as ring slots, unused ones as (slot R, value 0).
@@ -1524,15 +1528,24 @@ int build_level_t(cap_plan p, const std::vector<int> &rp)
     if ((size_t)R * sizeof(T) > kLevelRingBytes) return CAP_ERR_UNSUPPORTED;
     const int E = maxoff <= 4 ? 4 : 8;
     const int D = E == 4 ? kLevelDepth4 : kLevelDepth8;
+    auto steps_for = [&](int B) {
+        long long c = 0;
+        for (int l = 1; l <= L; ++l) c += (width[(size_t)l] + B - 1) / B;
+        return c;
+    };
+    int B = kLevelBlock;
+    const long long full = steps_for(kLevelBlock);
+    for (int cand = kLevelBlock - kWave; cand >= kWave; cand -= kWave)
+        if (steps_for(cand) <= full + full / 8) B = cand;
     std::vector<long long> next((size_t)L + 1, 0);  // next free slot of each level
     long long steps = 0;
     for (int l = 1; l <= L; ++l) {
-        next[(size_t)l] = steps * kLevelBlock;
-        steps += (width[(size_t)l] + kLevelBlock - 1) / kLevelBlock;
+        next[(size_t)l] = steps * B;
+        steps += (width[(size_t)l] + B - 1) / B;
     }
-    if (steps * kLevelBlock > std::max((long long)kLevelPadMax * n, 1LL << 16)) return CAP_ERR_UNSUPPORTED;
+    if (steps * B > std::max((long long)kLevelPadMax * n, 1LL << 16)) return CAP_ERR_UNSUPPORTED;
     steps = (steps + 2 * D - 1) / (2 * D) * (2 * D);  // the kernel runs whole rounds of 2D steps
-    const size_t S = (size_t)(steps + 2 * D) * kLevelBlock;
+    const size_t S = (size_t)(steps + 2 * D) * B;
     if (S * E > (size_t)INT32_MAX) return CAP_ERR_UNSUPPORTED;  // the kernel's 32-bit slot indices
     std::vector<int2> rec(S, make_int2(-1, 0));
     std::vector<int> lci(S * E, (int)R);
@@ -1554,6 +1567,7 @@ int build_level_t(cap_plan p, const std::vector<int> &rp)
     ls->E = E;
     ls->mask = (int)R - 1;
     ls->long_rows = maxoff > E;
+    ls->block = B;
     // the ring and its two extra slots may pass the default 64 KiB of dynamic LDS
     const int bytes = (int)((R + 2) * sizeof(T));
     auto allow = [&](auto kernel) {

@@ -72,7 +72,8 @@ extern "C" {
                               uses FP64 partitioned solves (no FP32 tables)
                               with one FP64 correction. */
 #define CAP_ALGO_LEVEL 7     /* one workgroup walks the levels: a level's rows
-                              in parallel (thread per row, steps of 256 rows),
+                              in parallel (thread per row, steps of 64 to 256
+                              rows, the workgroup size chosen by the plan),
                               a workgroup barrier between steps, values in an
                               LDS ring; no polling.  Needs every dependency
                               within a ring of <= 64 KiB of the newest row of

@@ -17,15 +17,15 @@ from conftest import random_lower, rel_maxnorm, upper_from_lower, drop_diagonal
 pytestmark = pytest.mark.gpu
 
 RING_BYTES = 65536  # the plan's LDS ring limit
-STEP = 256          # rows per step (one per thread of the workgroup)
 
 
 def accepts(rp, ci, vbytes):
     """The plan's acceptance rule, from the oracle's row levels: every
     off-diagonal entry c of a row in level L satisfies c > M_L - R (M_L the
     largest row index in levels <= L, R a power of two >= 64 within the
-    ring's bytes), and the levels padded to whole steps of 256 rows hold at
-    most max(4 n, 65536) slots."""
+    ring's bytes), and the levels padded to whole steps of B rows hold at
+    most max(4 n, 65536) slots, B the smallest of 64, 128, 192, 256 that
+    needs at most 1/8 more steps than 256."""
     n = len(rp) - 1
     lv = np.asarray(oracle.row_levels(n, rp, ci))
     L = int(lv.max())
@@ -39,8 +39,10 @@ def accepts(rp, ci, vbytes):
     ring = 64
     while ring < need:
         ring *= 2
-    steps = int(np.sum((width[1:] + STEP - 1) // STEP))
-    return ring * vbytes <= RING_BYTES and steps * STEP <= max(4 * n, 65536)
+    def steps(B):
+        return int(np.sum((width[1:] + B - 1) // B))
+    B = min(B for B in (64, 128, 192, 256) if steps(B) <= steps(256) + steps(256) // 8)
+    return ring * vbytes <= RING_BYTES and steps(B) * B <= max(4 * n, 65536)
 
 
 def _plan(cuda, rp, ci, v, **kw):
@@ -66,6 +68,8 @@ def _mats():
     yield "level_o6", api.generate("level", n=6000, levels=20, width=300, depth=2, offdiag=6, seed=2)
     yield "level_o12", api.generate("level", n=20000, levels=40, width=500, depth=2, offdiag=12, seed=5)
     yield "dup", random_lower(4000, max_off=6, band=300, seed=2, dup=True)
+    # a few rows per level: one-wavefront workgroups
+    yield "narrow_band", random_lower(3000, max_off=5, band=40, seed=1)
 
 
 MATS = list(_mats())
@@ -115,7 +119,7 @@ def test_level_refusals(cuda):
     import torch
     cases = [("wide_levels", api.make_config(1, seed=0, scale=0.05)),
              ("deep_band", api.make_config(2, seed=0, scale=0.02)),
-             ("narrow_band", random_lower(3000, max_off=5, band=40, seed=1))]
+             ("deep_band_2", api.make_config(2, seed=1, scale=0.04))]
     for name, (rp, ci, v) in cases:
         assert not accepts(rp, ci, 8), name
         with pytest.raises(RuntimeError, match=str(_native.CAP_ERR_UNSUPPORTED)):
