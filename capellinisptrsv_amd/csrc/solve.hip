@@ -2138,7 +2138,7 @@ int cap_plan_tuning(cap_plan p, int *blocks_per_cu, int *lds_window, int *adjoin
     if (!p) return CAP_ERR_ARG;
     if (row_tiles) *row_tiles = p->tile;
     if (blocks_per_cu) *blocks_per_cu = p->bpc;
-    if (row_threads) *row_threads = p->wblock;
+    if (row_threads) *row_threads = p->algo == CAP_ALGO_LEVEL && p->lvl ? p->lvl->block : p->wblock;
     if (lds_window) *lds_window = p->lds;
     if (adjoint) *adjoint = p->adjoint_mode;
     if (readiness) *readiness = p->staged ? CAP_READY_STAGED : CAP_READY_IN_PLACE;

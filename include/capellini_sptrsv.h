@@ -229,14 +229,15 @@ int cap_plan_rhs_block(cap_plan plan, int k, int *blocks_per_cu,
                        int *row_threads, int *rhs_per_wave, int *rhs_per_lane,
                        int *readiness, int *row_tiles);
 /* Which algorithm / sync protocol the plan resolved to, and its mix table.
- * (An autotuned plan that resolved to mix, two-part or CSC, which solve one
- * right-hand side, solves several with the write-first kernel in the value
- * protocol; a plan created with one of those algorithms refuses k > 1 with
- * CAP_ERR_UNSUPPORTED.) */
+ * (An autotuned plan that resolved to mix, two-part, CSC or the level
+ * steps, which solve one right-hand side, solves several with the
+ * write-first kernel in the value protocol; a plan created with one of
+ * those algorithms refuses k > 1 with CAP_ERR_UNSUPPORTED.) */
 int cap_plan_info(cap_plan plan, int *algo, int *sync, int *border,
                   int *mix_len, double *warp_occupy, double *element_occupy);
 /* Grid, window, readiness and row-tile choices the plan uses (after
- * autotuning); *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE. */
+ * autotuning); *readiness is CAP_READY_STAGED or CAP_READY_IN_PLACE;
+ * *row_threads of a CAP_ALGO_LEVEL plan is its one workgroup's size. */
 int cap_plan_tuning(cap_plan plan, int *blocks_per_cu, int *lds_window, int *adjoint,
                     int *readiness, int *row_threads, int *row_tiles);
 int cap_plan_destroy(cap_plan plan);

@@ -82,7 +82,8 @@ def test_level_bit_exact(cuda, name, mat):
     assert accepts(rp, ci, 8), name  # every case here is one the plan must take
     b = api.uniform(11, n)
     plan = _plan(cuda, rp, ci, v, algo="level")
-    assert plan.info()["algo"] == "level"
+    info = plan.info()
+    assert info["algo"] == "level" and info["row_threads"] in (64, 128, 192, 256)
     x = _run(plan, cuda, b)
     assert plan.status() == 0
     assert np.array_equal(x.view(np.uint64), oracle.csr_forward(rp, ci, v, b, fused=True).view(np.uint64)), name
