@@ -111,7 +111,7 @@ def test_driver_appends_result_csv(cuda, tmp_path):
 
 
 def test_driver_autotuned_and_deep_matrix_solvers(cuda, tmp_path):
-    """-a auto / local / partition (this library's additions to the driver):
+    """-a auto / local / partition / level (this library's additions to the driver):
     the same result.csv columns as wrtfst, none failing the validation, on a
     diagonally dominant band of 3000 rows (bandwidth 100, values kept)."""
     mtx = tmp_path / "band.mtx"
@@ -123,12 +123,12 @@ def test_driver_autotuned_and_deep_matrix_solvers(cuda, tmp_path):
             lines.append(f"{i + 1} {int(c) + 1} {rng.uniform(-1, 1):.6f}")
         lines.append(f"{i + 1} {i + 1} 4.0")
     mtx.write_text("\n".join(lines) + "\n")
-    for algo in ("auto", "local", "partition"):
+    for algo in ("auto", "local", "partition", "level"):
         r = subprocess.run([os.path.join(ROOT, "capellinisptrsv_amd", "sptrsv_main"), str(mtx), "-a", algo, "--values"],
                            cwd=tmp_path, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, (algo, r.stdout, r.stderr)
     rows = (tmp_path / "result.csv").read_text().splitlines()
-    assert len(rows) == 6
+    assert len(rows) == 8
     for row in rows[1::2]:
         f = [s.strip() for s in row.split(",")]
         assert f[0] == str(mtx) and int(f[1]) == n and float(f[7]) > 0, row
