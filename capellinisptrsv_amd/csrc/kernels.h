@@ -1384,6 +1384,9 @@ __global__ void __launch_bounds__(kLocalBlock)
 // compute unit, against the memory round trip of a publish -> observe hop
 // between compute units.
 //
+// K right-hand sides (1, 2 or 4) are solved per row by the same thread,
+// interleaved in b and x (row * K + h); the ring holds K values per slot.
+//
 // Values travel through an LDS ring, row i in slot i & mask.  The plan
 // checks, for every off-diagonal entry c of a row in level L, that c > M_L -
 // R (M_L: the largest row index in levels <= L; R = mask + 1): no row that
@@ -1405,13 +1408,15 @@ __global__ void __launch_bounds__(kLocalBlock)
 constexpr int kLevelBlock = 256;
 constexpr int kLevelDepth4 = 10;  // steps of row data in flight with E = 4 entries carried
 constexpr int kLevelDepth8 = 6;   // with E = 8 (the load counter holds 63 loads in flight)
+constexpr int kLevelDepthK2 = 8;  // E = 4, 2 right-hand sides per row (registers)
+constexpr int kLevelDepthK4 = 6;  // E = 4, 4 right-hand sides per row
 
-template <typename T, int E>
+template <typename T, int E, int K>
 struct LevelStage {
     int row, cnt;
     int c[E];
     T v[E];
-    T d, bo;
+    T d, bo[K];
 };
 
 // 16-byte loads of E consecutive elements (E * sizeof(T) is a multiple of
@@ -1428,8 +1433,8 @@ __device__ __forceinline__ void load16(const V *__restrict__ src, V (&dst)[E])
     }
 }
 
-template <typename T, int E>
-__device__ __forceinline__ void level_load(LevelStage<T, E> &st, int2 rec, unsigned q, const int *__restrict__ lci,
+template <typename T, int E, int K>
+__device__ __forceinline__ void level_load(LevelStage<T, E, K> &st, int2 rec, unsigned q, const int *__restrict__ lci,
                                            const T *__restrict__ lv, const T *__restrict__ ld, const T *__restrict__ b)
 {
     st.row = rec.x;
@@ -1437,7 +1442,14 @@ __device__ __forceinline__ void level_load(LevelStage<T, E> &st, int2 rec, unsig
     load16<int, E>(lci + (size_t)q * E, st.c);
     load16<T, E>(lv + (size_t)q * E, st.v);
     st.d = ld[q];
-    st.bo = b[rec.x < 0 ? 0 : rec.x];  // padding slots read b[0], unused
+    // padding slots read row 0's b, unused
+    if constexpr (K == 1) {
+        st.bo[0] = b[rec.x < 0 ? 0 : rec.x];
+    } else {
+        const size_t o = (size_t)(rec.x < 0 ? 0 : rec.x) * K;
+#pragma unroll
+        for (int h = 0; h < K; ++h) st.bo[h] = b[o + h];
+    }
 }
 
 __device__ __forceinline__ void level_barrier()
@@ -1449,7 +1461,7 @@ __device__ __forceinline__ void level_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <typename T, int E, int D, bool UNFUSED, bool LONG>
+template <typename T, int E, int D, bool UNFUSED, bool LONG, int K>
 __global__ void __launch_bounds__(kLevelBlock)
     k_level(const int2 *__restrict__ rec, const int *__restrict__ lci, const T *__restrict__ lv,
             const T *__restrict__ ld, const int *__restrict__ rp, const int *__restrict__ ci,
@@ -1461,7 +1473,7 @@ __global__ void __launch_bounds__(kLevelBlock)
     extern __shared__ unsigned long long smem_level[];
     T *ring = reinterpret_cast<T *>(smem_level);
     const int t = threadIdx.x;
-    LevelStage<T, E> st[D];
+    LevelStage<T, E, K> st[D];
     int2 ahead[D], first[D];
     // the row pairs first, so no stage's b load waits behind another stage
 #pragma unroll
@@ -1469,8 +1481,8 @@ __global__ void __launch_bounds__(kLevelBlock)
 #pragma unroll
     for (int r = 0; r < D; ++r) ahead[r] = rec[(D + r) * B + t];
 #pragma unroll
-    for (int r = 0; r < D; ++r) level_load<T, E>(st[r], first[r], r * B + t, lci, lv, ld, b);
-    if (t == 0) ring[mask + 1] = (T)0;
+    for (int r = 0; r < D; ++r) level_load<T, E, K>(st[r], first[r], r * B + t, lci, lv, ld, b);
+    if (t < K) ring[(mask + 1) * K + t] = (T)0;
     level_barrier();
     // Two rounds of the D stages per iteration (steps is a multiple of 2D,
     // padded by the plan, so the body has no exit): the compiler's
@@ -1481,32 +1493,48 @@ __global__ void __launch_bounds__(kLevelBlock)
         for (int h = 0; h < 2 * D; ++h) {
             const int r = h % D;
             const int s = s0 + h;
-            const LevelStage<T, E> &w = st[r];
-            T xv[E];
+            const LevelStage<T, E, K> &w = st[r];
+            T xv[E][K];
 #pragma unroll
-            for (int u = 0; u < E; ++u) xv[u] = ring[w.c[u]];
-            T sum = (T)0;
+            for (int u = 0; u < E; ++u)
 #pragma unroll
-            for (int u = 0; u < E; ++u) sum = UNFUSED ? sum + w.v[u] * xv[u] : fmac(w.v[u], xv[u], sum);
+                for (int h = 0; h < K; ++h) xv[u][h] = ring[w.c[u] * K + h];
+            T sum[K];
+#pragma unroll
+            for (int h = 0; h < K; ++h) {
+                sum[h] = (T)0;
+#pragma unroll
+                for (int u = 0; u < E; ++u)
+                    sum[h] = UNFUSED ? sum[h] + w.v[u] * xv[u][h] : fmac(w.v[u], xv[u][h], sum[h]);
+            }
             if (LONG && w.cnt > E) {  // the rest of a long row, from the CSR
                 const int j1 = rp[w.row] + w.cnt;
                 for (int j = j1 - w.cnt + E; j < j1; ++j) {
-                    const T a = val[j], xj = ring[ci[j] & mask];
-                    sum = UNFUSED ? sum + a * xj : fmac(a, xj, sum);
+                    const T a = val[j];
+                    const int c = (ci[j] & mask) * K;
+#pragma unroll
+                    for (int h = 0; h < K; ++h) sum[h] = UNFUSED ? sum[h] + a * ring[c + h] : fmac(a, ring[c + h], sum[h]);
                 }
             }
-            U o = as_bits((w.bo - sum) / w.d);
-            if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;  // as the publishing kernels do
             const bool real = w.row >= 0;
-            ring[real ? w.row & mask : mask + 2] = as_val(o);
-            *(real ? x + w.row : sink) = as_val(o);
+            const int slot = (real ? w.row & mask : mask + 2) * K;
+#pragma unroll
+            for (int h = 0; h < K; ++h) {
+                U o = as_bits((w.bo[h] - sum[h]) / w.d);
+                if (o == VBits<T>::kSentinel) o = VBits<T>::kQNaN;  // as the publishing kernels do
+                ring[slot + h] = as_val(o);
+                if constexpr (K == 1)
+                    *(real ? x + w.row : sink) = as_val(o);
+                else
+                    *(real ? x + (size_t)w.row * K + h : sink + h) = as_val(o);
+            }
             // refill this stage for step s + D (padded: the arrays hold 2D
             // steps past the last), and its row pair for step s + 2D.  Kept
             // after the step's own work: interleaved with it, the compiler
             // parks in-flight load results in accumulation registers, and
             // each such move waits for its load.
             __builtin_amdgcn_sched_barrier(0);
-            level_load<T, E>(st[r], ahead[r], (s + D) * B + t, lci, lv, ld, b);
+            level_load<T, E, K>(st[r], ahead[r], (s + D) * B + t, lci, lv, ld, b);
             ahead[r] = rec[(s + 2 * D) * B + t];
             level_barrier();
         }

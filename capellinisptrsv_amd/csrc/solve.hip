@@ -100,13 +100,14 @@ struct PartState {
 // steps + 2 * depth steps of `block` slots (the workgroup size); E entries
 // carried per slot;
 // the LDS ring has mask + 1 slots.
-constexpr int kCtrlSink = 2048;  // k_level's padding-lane stores (8 bytes, never read)
+constexpr int kCtrlSink = 2048;  // k_level's padding-lane stores (up to 4 values, never read)
 struct LevelState {
     int2 *rec = nullptr;
     int *ci = nullptr;
     void *v = nullptr, *d = nullptr;
     int steps = 0, E = 4, mask = 0, block = kLevelBlock;
     bool long_rows = false;  // some row has more than E off-diagonal entries
+    int max_rhs = 1;         // right-hand sides per solve: 1, or 2 / 4 where their ring fits LDS (E = 4)
 };
 
 // The RHS-block pick of an autotuned plan for one right-hand-side count:
@@ -203,6 +204,13 @@ struct cap_plan_s {
     // level steps in one workgroup (CAP_ALGO_LEVEL)
     LevelState *lvl = nullptr;
 };
+
+// Right-hand-side counts a level plan solves: 1, and 2 or 4 where a ring of
+// that many values per slot fits LDS (rows of at most 4 off-diagonals).
+inline bool level_takes(const cap_plan_s *p, int rhs)
+{
+    return p->lvl && (rhs == 1 || ((rhs == 2 || rhs == 4) && rhs <= p->lvl->max_rhs));
+}
 
 // Side streams of a hybrid batch: its partitioned matrices' solves run
 // concurrently with each other (and with the rest's reset), forked from and
@@ -887,8 +895,9 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
     } else if (algo == CAP_ALGO_LEVEL) {
         // one right-hand side, the plan's own values (the tables hold them)
         const LevelState *ls = p->lvl;
-        if (rhs != 1 || !ls || (const void *)val != p->val) return CAP_ERR_UNSUPPORTED;
-        const size_t bytes = (size_t)(ls->mask + 3) * sizeof(T);  // the ring, its zero slot and its sink slot
+        if (!level_takes(p, rhs) || (const void *)val != p->val) return CAP_ERR_UNSUPPORTED;
+        // the ring, its zero slot and its sink slot, rhs values each
+        const size_t bytes = (size_t)(ls->mask + 3) * rhs * sizeof(T);
         const T *lv = (const T *)ls->v, *ld = (const T *)ls->d;
         T *sink = reinterpret_cast<T *>(p->ctrl + kCtrlSink);
         auto go = [&](auto kernel) {
@@ -896,12 +905,16 @@ int launch_kernel(cap_plan p, int algo, int sync, const T *val, const T *b, T *x
                                                   ls->mask);
         };
         const bool uf = sc.unfused != 0;
-        if (ls->E == 4)
-            uf ? go(k_level<T, 4, kLevelDepth4, true, false>) : go(k_level<T, 4, kLevelDepth4, false, false>);
+        if (rhs == 2)
+            uf ? go(k_level<T, 4, kLevelDepthK2, true, false, 2>) : go(k_level<T, 4, kLevelDepthK2, false, false, 2>);
+        else if (rhs == 4)
+            uf ? go(k_level<T, 4, kLevelDepthK4, true, false, 4>) : go(k_level<T, 4, kLevelDepthK4, false, false, 4>);
+        else if (ls->E == 4)
+            uf ? go(k_level<T, 4, kLevelDepth4, true, false, 1>) : go(k_level<T, 4, kLevelDepth4, false, false, 1>);
         else if (!ls->long_rows)
-            uf ? go(k_level<T, 8, kLevelDepth8, true, false>) : go(k_level<T, 8, kLevelDepth8, false, false>);
+            uf ? go(k_level<T, 8, kLevelDepth8, true, false, 1>) : go(k_level<T, 8, kLevelDepth8, false, false, 1>);
         else
-            uf ? go(k_level<T, 8, kLevelDepth8, true, true>) : go(k_level<T, 8, kLevelDepth8, false, true>);
+            uf ? go(k_level<T, 8, kLevelDepth8, true, true, 1>) : go(k_level<T, 8, kLevelDepth8, false, true, 1>);
     } else if (algo == CAP_ALGO_CSC) {
         if (rhs != 1) return CAP_ERR_UNSUPPORTED;
         // lanes per column from the mean column length (diagonal included)
@@ -962,7 +975,7 @@ inline int algo_for(cap_plan p, int rhs)
 {
     if (rhs > 1 && p->auto_algo &&
         (p->algo == CAP_ALGO_MIX || p->algo == CAP_ALGO_TWOPART || p->algo == CAP_ALGO_CSC ||
-         p->algo == CAP_ALGO_LEVEL))
+         (p->algo == CAP_ALGO_LEVEL && !level_takes(p, rhs))))
         return CAP_ALGO_WRTFST;
     return p->algo;
 }
@@ -1568,24 +1581,30 @@ int build_level_t(cap_plan p, const std::vector<int> &rp)
     ls->mask = (int)R - 1;
     ls->long_rows = maxoff > E;
     ls->block = B;
-    // the ring and its two extra slots may pass the default 64 KiB of dynamic LDS
-    const int bytes = (int)((R + 2) * sizeof(T));
-    auto allow = [&](auto kernel) {
+    // the ring and its two extra slots (K values each) may pass the default
+    // 64 KiB of dynamic LDS; 2 and 4 right-hand sides where they fit
+    ls->max_rhs = E == 4 && (size_t)(R + 2) * 4 * sizeof(T) <= (size_t)p->lds_max   ? 4
+                  : E == 4 && (size_t)(R + 2) * 2 * sizeof(T) <= (size_t)p->lds_max ? 2
+                                                                                     : 1;
+    auto allow = [&](auto kernel, int K) {
         return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   bytes);
+                                   (int)((R + 2) * K * sizeof(T))) == hipSuccess;
     };
-    hipError_t e1, e2;
+    bool ok;
     if (E == 4) {
-        e1 = allow(&k_level<T, 4, kLevelDepth4, true, false>);
-        e2 = allow(&k_level<T, 4, kLevelDepth4, false, false>);
+        ok = allow(&k_level<T, 4, kLevelDepth4, true, false, 1>, 1) && allow(&k_level<T, 4, kLevelDepth4, false, false, 1>, 1);
+        if (ok && ls->max_rhs >= 2)
+            ok = allow(&k_level<T, 4, kLevelDepthK2, true, false, 2>, 2) &&
+                 allow(&k_level<T, 4, kLevelDepthK2, false, false, 2>, 2);
+        if (ok && ls->max_rhs >= 4)
+            ok = allow(&k_level<T, 4, kLevelDepthK4, true, false, 4>, 4) &&
+                 allow(&k_level<T, 4, kLevelDepthK4, false, false, 4>, 4);
     } else if (!ls->long_rows) {
-        e1 = allow(&k_level<T, 8, kLevelDepth8, true, false>);
-        e2 = allow(&k_level<T, 8, kLevelDepth8, false, false>);
+        ok = allow(&k_level<T, 8, kLevelDepth8, true, false, 1>, 1) && allow(&k_level<T, 8, kLevelDepth8, false, false, 1>, 1);
     } else {
-        e1 = allow(&k_level<T, 8, kLevelDepth8, true, true>);
-        e2 = allow(&k_level<T, 8, kLevelDepth8, false, true>);
+        ok = allow(&k_level<T, 8, kLevelDepth8, true, true, 1>, 1) && allow(&k_level<T, 8, kLevelDepth8, false, true, 1>, 1);
     }
-    if (e1 != hipSuccess || e2 != hipSuccess) {
+    if (!ok) {
         free_level(p);
         return CAP_ERR_HIP;
     }

@@ -79,7 +79,9 @@ extern "C" {
                               within a ring of <= 64 KiB of the newest row of
                               its level (CAP_ERR_UNSUPPORTED otherwise, also
                               when the padded steps exceed 4 slots per row);
-                              one right-hand side per solve; bit-exact (the
+                              1 right-hand side per solve, or 2 or 4 when
+                              rows hold at most 4 off-diagonals and the wider
+                              ring fits LDS; bit-exact (the
                               thread-per-row sum order).  Timed by algo AUTO
                               with the few-rows-per-level candidates; its
                               refined solves use the write-first kernels. */

@@ -225,3 +225,54 @@ def test_level_graph_replay(cuda):
         torch.cuda.synchronize()
         assert np.array_equal(xt.cpu().numpy(), oracle.csr_forward(rp, ci, v, b, fused=True)), seed
     plan.close()
+
+
+@pytest.mark.parametrize("k", [2, 4])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_level_several_rhs(cuda, k, dtype):
+    """Rows of at most 4 off-diagonals take 2 or 4 right-hand sides per solve
+    (one thread per row, the ring K values wide), each column bit-exact;
+    3 right-hand sides, or rows past 4 entries, are refused."""
+    import torch
+    for name, (rp, ci, v) in MATS[:4]:
+        n = len(rp) - 1
+        vv = v.astype(dtype)
+        bk = api.uniform(30 + k, n * k).astype(dtype).reshape(n, k)
+        plan = _plan(cuda, rp, ci, vv, algo="level")
+        x = plan.solve(torch.from_numpy(np.ascontiguousarray(bk)).to(cuda))
+        torch.cuda.synchronize()
+        x = x.cpu().numpy().reshape(n, k)
+        for q in range(k):
+            assert np.array_equal(x[:, q], oracle.csr_forward(rp, ci, vv, bk[:, q].copy(), fused=True)), (name, q)
+        with pytest.raises(RuntimeError, match=str(_native.CAP_ERR_UNSUPPORTED)):
+            plan.solve(torch.zeros(n, 3, dtype=torch.from_numpy(vv).dtype, device=cuda))
+        plan.close()
+    rp, ci, v = MATS[6][1]  # rows of up to 7 off-diagonals: 8 carried, one right-hand side only
+    plan = _plan(cuda, rp, ci, v.astype(dtype), algo="level")
+    with pytest.raises(RuntimeError, match=str(_native.CAP_ERR_UNSUPPORTED)):
+        plan.solve(torch.zeros(len(rp) - 1, k, dtype=torch.from_numpy(v.astype(dtype)).dtype, device=cuda))
+    plan.close()
+
+
+def test_level_several_rhs_reference_arith_and_autotuned(cuda):
+    """Two right-hand sides in the reference arithmetic match the reference
+    executor's bits; an autotuned plan on the 100k band solves 2 and 4
+    right-hand sides bit-exact (level steps or write-first, whichever it runs)."""
+    import torch
+    rp, ci, v = MATS[1][1]
+    n = len(rp) - 1
+    b2 = api.uniform(40, n * 2).reshape(n, 2)
+    plan = _plan(cuda, rp, ci, v, algo="level", arith="reference")
+    x = plan.solve(torch.from_numpy(b2).to(cuda)).cpu().numpy()
+    for q in range(2):
+        assert np.array_equal(x[:, q], oracle.csr_forward(rp, ci, v, b2[:, q].copy(), fused=False)), q
+    plan.close()
+    rp, ci, v = MATS[0][1]
+    n = len(rp) - 1
+    plan = _plan(cuda, rp, ci, v, algo="auto")
+    for k in (2, 4):
+        bk = api.uniform(41 + k, n * k).reshape(n, k)
+        x = plan.solve(torch.from_numpy(bk).to(cuda)).cpu().numpy()
+        for q in range(k):
+            assert np.array_equal(x[:, q], oracle.csr_forward(rp, ci, v, bk[:, q].copy(), fused=True)), (k, q)
+    plan.close()
