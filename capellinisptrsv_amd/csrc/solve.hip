@@ -1510,6 +1510,7 @@ int build_partition_tables(cap_plan p, const std::vector<int> &rp_host, hipStrea
 // steps exceed kLevelPadMax slots per row (many narrow levels: a thread per
 // slot would mostly idle).
 constexpr size_t kLevelRingBytes = 65536;
+constexpr double kLevelMaxWidth = 4096;  // the autotuner tries the level steps up to these rows per level
 constexpr int kLevelPadMax = 4;
 
 template <typename T>
@@ -1684,10 +1685,12 @@ int autotune(cap_plan p, const std::vector<int> &rp_host, hipStream_t st)
             // (reassociated sums: not a candidate in the reference arithmetic)
             if (!p->unfused && build_partition(p, rp_host, st) == CAP_OK)
                 cands.insert(cands.begin(), {CAP_ALGO_PARTITION, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
-            // level steps in one workgroup, where the ring and the padding allow
-            if (build_level(p, rp_host) == CAP_OK)
-                cands.insert(cands.begin(), {CAP_ALGO_LEVEL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
         }
+        // level steps in one workgroup, where the ring and the padding allow:
+        // up to a few thousand rows per level a step per 256 rows can still
+        // beat a cross-CU hop per level
+        if ((double)p->n / std::max(1, levels) <= kLevelMaxWidth && build_level(p, rp_host) == CAP_OK)
+            cands.insert(cands.begin(), {CAP_ALGO_LEVEL, CAP_SYNC_VALUE, 0, 1, 0, 0, B});
     }
     // thresholds 4 and 32 (see mix_border_for) always; 16 where the mean row
     // length is at least 4 (some chunks may then reach it)
